@@ -1,0 +1,211 @@
+/*
+ * sphmhd.h -- C ABI of the MI355X (gfx950) SPH-MHD step library (libsphmhd.so).
+ *
+ * The library replaces the PySPH evaluation of the reference's equation-group
+ * plan (reference: src/scheme.py:67-81) and its PEC integrator stages
+ * (src/integrator.py:6-82) with hand-written HIP kernels.  Every entry point
+ * below names the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative code on error and a
+ *     positive code for a warning (see sphmhd_status).  No C++ exception ever
+ *     crosses this boundary; sphmhd_last_error() gives the message of the
+ *     most recent non-zero return on the calling thread.
+ *   - Field pointers are DEVICE pointers to FP64 arrays in particle storage
+ *     order.  They are borrowed: the caller keeps them alive while bound.
+ *   - One host thread per context.  All device work is ordered on the stream
+ *     given to sphmhd_create (NULL = the default stream).
+ *   - A context holds one or more independent instances (an ensemble).  The
+ *     particles of instance k occupy storage [offsets[k], offsets[k+1]).  A
+ *     plain run is an ensemble of one.
+ */
+#ifndef SPHMHD_H
+#define SPHMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHMHD_ABI_VERSION 1
+
+typedef enum sphmhd_status {
+    SPHMHD_OK = 0,
+    SPHMHD_EINVAL = -1,     /* bad argument */
+    SPHMHD_EHIP = -2,       /* HIP runtime error */
+    SPHMHD_ECAP = -3,       /* a capacity was exceeded */
+    SPHMHD_ESTATE = -4,     /* call out of order (e.g. no fields bound) */
+    SPHMHD_ENOTCONV = -5,   /* warm start: a particle did not converge
+                               (the reference asserts, application.py:120,131) */
+    SPHMHD_WNOCONV = 1,     /* density iteration hit max_iterations */
+    SPHMHD_WNAN = 2         /* a non-finite acceleration was produced */
+} sphmhd_status;
+
+/* SPH kernels, in the order of the reference's name map (scheme.py:48-56). */
+typedef enum sphmhd_kernel_id {
+    SPHMHD_CUBIC = 0,
+    SPHMHD_QUINTIC = 1,
+    SPHMHD_WENDLAND_C2 = 2,
+    SPHMHD_WENDLAND_C4 = 3,
+    SPHMHD_WENDLAND_C6 = 4,
+    SPHMHD_GAUSSIAN = 5,
+    SPHMHD_SUPER_GAUSSIAN = 6
+} sphmhd_kernel_id;
+
+/* Per-instance physics and numerics (config.py:8-54 plus the scheme's
+ * constants).  dim, kernel and periodic must agree across the instances of
+ * one context; every other member may differ per instance. */
+typedef struct sphmhd_params {
+    int32_t dim;                /* 2 or 3 (the reference hard-wires 3, scheme.py:57) */
+    int32_t kernel;             /* sphmhd_kernel_id */
+    int32_t density_iterate;    /* 1: numerics.density == "iterate", 0: "single" */
+    int32_t min_iterations;     /* scheme.py:71: 2 when iterating, else 0 */
+    int32_t max_iterations;     /* scheme.py:28,71: 250 when iterating, else 1 */
+    int32_t periodic;           /* 1: minimum-image periodic box [lo, hi) on every axis */
+    int32_t adaptive_timestep;  /* solver.adaptive_timestep */
+    int32_t reserved;
+    double lo[3];
+    double hi[3];
+    double gamma;
+    double mu0;
+    double hfact;                               /* DensityIteration k */
+    double artificial_viscosity;
+    double artificial_thermal_conductivity;
+    double artificial_magnetic_dissipation;
+    double cleaning_speed_factor;
+    double cleaning_damping_factor;
+    double timestep_factor;
+    double alpha_min;           /* floor of alpha_loc; 0 reproduces equations.py:428 */
+    double alpha_max;           /* multiplier of alpha_loc; 1 reproduces equations.py:428 */
+    double tf;                  /* final time: dt is clipped to tf - t */
+} sphmhd_params;
+
+/* Bound particle properties.  The names are the reference's property names
+ * (particles.py:1-162).  Properties the reference writes but never reads
+ * again (grho*, arho, rho_sum, dwdh, dt_force, cd_*, dvdx_*) are not part of
+ * the device path. */
+typedef enum sphmhd_field {
+    SPHMHD_F_X = 0, SPHMHD_F_Y, SPHMHD_F_Z,
+    SPHMHD_F_U, SPHMHD_F_V, SPHMHD_F_W,
+    SPHMHD_F_RHO, SPHMHD_F_H, SPHMHD_F_M, SPHMHD_F_CS, SPHMHD_F_P, SPHMHD_F_E,
+    SPHMHD_F_AU, SPHMHD_F_AV, SPHMHD_F_AW, SPHMHD_F_AE, SPHMHD_F_AH,
+    SPHMHD_F_X0, SPHMHD_F_Y0, SPHMHD_F_Z0,
+    SPHMHD_F_U0, SPHMHD_F_V0, SPHMHD_F_W0,
+    SPHMHD_F_UPRED, SPHMHD_F_VPRED, SPHMHD_F_WPRED,
+    SPHMHD_F_EPRED, SPHMHD_F_E0, SPHMHD_F_H0,
+    SPHMHD_F_DIV, SPHMHD_F_DT_CFL, SPHMHD_F_DT_ADAPT,
+    SPHMHD_F_OMEGA, SPHMHD_F_CONVERGED,
+    SPHMHD_F_ALPHA1, SPHMHD_F_ALPHA10,
+    SPHMHD_F_BX, SPHMHD_F_BY, SPHMHD_F_BZ,
+    SPHMHD_F_BEVOLX, SPHMHD_F_BEVOLY, SPHMHD_F_BEVOLZ, SPHMHD_F_PSI,
+    SPHMHD_F_BEVOLXPRED, SPHMHD_F_BEVOLYPRED, SPHMHD_F_BEVOLZPRED, SPHMHD_F_PSIPRED,
+    SPHMHD_F_BXPRED, SPHMHD_F_BYPRED, SPHMHD_F_BZPRED,
+    SPHMHD_F_ABEVOLX, SPHMHD_F_ABEVOLY, SPHMHD_F_ABEVOLZ, SPHMHD_F_APSI,
+    SPHMHD_F_BEVOLX0, SPHMHD_F_BEVOLY0, SPHMHD_F_BEVOLZ0, SPHMHD_F_PSI0,
+    SPHMHD_F_DIVBSYMM, SPHMHD_F_DIVBDIFF, SPHMHD_F_DIV_FOR_PSI,
+    SPHMHD_F_ALPHALOC, SPHMHD_F_XILIMITER, SPHMHD_F_DIVVDT,
+    SPHMHD_NFIELDS
+} sphmhd_field;
+
+/* Conserved totals per instance, in this order (sphmhd_totals). */
+enum {
+    SPHMHD_T_MASS = 0,
+    SPHMHD_T_MOMX, SPHMHD_T_MOMY, SPHMHD_T_MOMZ,
+    SPHMHD_T_EKIN,     /* sum m |v|^2 / 2 */
+    SPHMHD_T_ETHERM,   /* sum m e */
+    SPHMHD_T_EMAG,     /* sum m |B|^2 / (2 mu0 rho) */
+    SPHMHD_T_DIVB,     /* sum (m/rho) h |divB| / |B|  over particles with |B| > 0 */
+    SPHMHD_T_VOLUME,   /* sum m/rho */
+    SPHMHD_NTOTALS
+};
+
+typedef struct sphmhd_stats {
+    int64_t evals;               /* force evaluations since create */
+    int64_t density_iterations;  /* density passes summed over instances */
+    int64_t pairs_force;         /* neighbour entries of the last force pass */
+    int64_t cells;               /* cells of the last neighbour-search build */
+    int32_t last_iterations_max; /* most density passes any instance needed in the last eval */
+    int32_t max_neighbours;      /* largest neighbour count in the last force pass */
+} sphmhd_stats;
+
+typedef struct sphmhd_ctx sphmhd_ctx;
+
+int sphmhd_abi_version(void);
+const char* sphmhd_last_error(void);
+
+/* Create a context for n_instances instances; params[k] belongs to instance
+ * k and offsets has n_instances + 1 ascending entries (offsets[0] == 0).
+ * Replaces the PySPH solver set-up that compiles get_equations()
+ * (scheme.py:47-81) and allocates NNPS state. */
+int sphmhd_create(const sphmhd_params* params, int32_t n_instances,
+                  const int64_t* offsets, void* hip_stream, sphmhd_ctx** out);
+int sphmhd_destroy(sphmhd_ctx* ctx);
+
+/* Bind device arrays: fields[f] for every sphmhd_field f (z, z0 may be NULL
+ * when dim == 2) and itype (int32, particles.py:7).  No copies are made.
+ * Replaces ParticleArray / pa.gpu property binding. */
+int sphmhd_bind(sphmhd_ctx* ctx, double* const* fields, int32_t* itype);
+
+/* One force evaluation: neighbour-search update, then groups G0..G9 of
+ * scheme.py:70-81 (density iteration to convergence, EOS, Cullen-Dehnen
+ * switch, MHD and Maxwell-stress forces, induction and cleaning, energy
+ * limiter, divB correction, adaptive time step).  Leaves min dt_adapt per
+ * instance as the pending next time step.  Replaces
+ * AccelerationEval.compute / Integrator.initial_acceleration. */
+int sphmhd_eval(sphmhd_ctx* ctx);
+
+/* MHDStep.initialize / stage1 / stage2 (integrator.py:12-82), with the
+ * periodic wrap of PySPH's update_domain after each stage.  stage1 and
+ * stage2 use each instance's current dt (sphmhd_get_clock). */
+int sphmhd_pec_initialize(sphmhd_ctx* ctx);
+int sphmhd_pec_stage1(sphmhd_ctx* ctx);
+int sphmhd_pec_stage2(sphmhd_ctx* ctx);
+
+/* t += dt; dt := the pending step from the last evaluation (min dt_adapt,
+ * clipped to tf - t).  Replaces solver.py:57-61. */
+int sphmhd_advance_clock(sphmhd_ctx* ctx);
+
+/* nsteps full PEC steps with the clock advanced after each: the body of
+ * MHDSolver.solve (solver.py:42-73) without dumps.  Kernels are fused
+ * (stage2 of step n with initialize+stage1 of step n+1); the fields at
+ * return are exactly those after the last stage2. */
+int sphmhd_step(sphmhd_ctx* ctx, int64_t nsteps);
+
+/* MHDApplication._sync_initial_mhd_state (application.py:100-108):
+ * Bevol = B / rho, psi = 0, predicted = current. */
+int sphmhd_sync_initial_state(sphmhd_ctx* ctx);
+
+/* MHDApplication._warm_initial_derivatives + _compute_initial_dt
+ * (application.py:110-147) without the per-case thermodynamics hook. */
+int sphmhd_warm_start(sphmhd_ctx* ctx);
+
+/* Per-instance clock (t, dt) as host arrays of n_instances doubles. */
+int sphmhd_get_clock(sphmhd_ctx* ctx, double* t, double* dt);
+int sphmhd_set_clock(sphmhd_ctx* ctx, const double* t, const double* dt);
+/* Pending dt (min dt_adapt over each instance from the last evaluation). */
+int sphmhd_get_pending_dt(sphmhd_ctx* ctx, double* dt);
+
+/* Conserved totals, host array of n_instances * SPHMHD_NTOTALS doubles. */
+int sphmhd_totals(sphmhd_ctx* ctx, double* out);
+
+/* Neighbour-search exports for parity checks, into HOST buffers of n
+ * entries: sorted cell key (instance-local key plus the instance's cell
+ * base) and the storage index at each sorted position. */
+int sphmhd_cells(sphmhd_ctx* ctx, uint32_t* keys, int32_t* perm);
+/* Neighbour lists of the current h, symmetric radius test
+ * r^2 < (R max(h_i, h_j))^2 with minimum image: offsets[n + 1] and
+ * idx[cap] (storage indices, ascending per particle), *total = entries. */
+int sphmhd_neighbours(sphmhd_ctx* ctx, int64_t* offsets, int32_t* idx,
+                      int64_t cap, int64_t* total);
+
+int sphmhd_get_stats(sphmhd_ctx* ctx, sphmhd_stats* out);
+
+/* Block until the context's stream is idle. */
+int sphmhd_synchronize(sphmhd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHMHD_H */

@@ -1,0 +1,540 @@
+// sph_capi.hip -- context management and the extern "C" boundary (sphmhd.h).
+//
+// The host side only sequences launches on the context's stream; every
+// per-particle and per-pair operation runs in the kernels of sph_nnps.hip,
+// sph_pairs.hip and sph_stage.hip.  Host synchronisation happens once per
+// density iteration (the reference also blocks there: equations.py:91-94
+// pulls equation_has_converged), once after the neighbour-list build (to
+// grow the list if a particle has more neighbours than its slots) and once
+// at the end of an evaluation to collect warnings.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sph_internal.h"
+
+using namespace sph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(SPHMHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) return fail(SPHMHD_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(*p, 0, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) return fail(SPHMHD_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+struct sphmhd_ctx {
+    Dev d{};
+    std::vector<Inst> hinst;
+    std::vector<int64_t> offsets;
+    bool bound = false;
+    int max_iter_all = 1;
+    Ctl* hctl = nullptr;  // pinned readback
+    sphmhd_stats stats{};
+    int64_t tiles = 0;
+};
+
+static int read_ctl(sphmhd_ctx* c) {
+    HIPCHK(hipMemcpyAsync(c->hctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+static int pull_inst(sphmhd_ctx* c) {
+    HIPCHK(hipMemcpyAsync(c->hinst.data(), c->d.inst, sizeof(Inst) * c->hinst.size(), hipMemcpyDeviceToHost,
+                          c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+static int push_inst(sphmhd_ctx* c) {
+    HIPCHK(hipMemcpyAsync(c->d.inst, c->hinst.data(), sizeof(Inst) * c->hinst.size(), hipMemcpyHostToDevice,
+                          c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+static int grow_nb(sphmhd_ctx* c, int need) {
+    int stride = ((need + 31) / 32) * 32;
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    HIPCHK(hipFree(c->d.nb));
+    c->d.nb = nullptr;
+    int rc = dalloc(&c->d.nb, (int64_t)stride * c->d.n);
+    if (rc) return rc;
+    c->d.nb_stride = stride;
+    return 0;
+}
+
+// neighbour lists for the current pack (x, h); grows the slots if needed
+static int build_lists(sphmhd_ctx* c) {
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        launch_nb_build(c->d);
+        int rc = read_ctl(c);
+        if (rc) return rc;
+        if (c->hctl->max_nb <= c->d.nb_stride) return 0;
+        rc = grow_nb(c, c->hctl->max_nb);
+        if (rc) return rc;
+        launch_reset_nb_stats(c->d);
+    }
+    return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
+}
+
+static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions) {
+    launch_eval_begin(c->d, reset_clock_reductions);
+    launch_hmax(c->d, false);
+    launch_grid(c->d);
+    launch_keys_sort_ranges(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int do_eval(sphmhd_ctx* c) {
+    Dev& d = c->d;
+    int rc = nnps_rebuild(c, true);
+    if (rc) return rc;
+    launch_pack_density(d);
+    int passes = 0;
+    for (;;) {
+        launch_density(d);
+        launch_g0_control(d);
+        HIPCHK(hipGetLastError());
+        rc = read_ctl(c);
+        if (rc) return rc;
+        ++passes;
+        if (c->hctl->all_done) break;
+        if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
+    }
+    c->stats.density_iterations += passes;
+    c->stats.last_iterations_max = c->hctl->iters_max;
+    launch_prep_force(d);
+    rc = build_lists(c);
+    if (rc) return rc;
+    c->stats.max_neighbours = c->hctl->max_nb;
+    c->stats.pairs_force = c->hctl->pairs;
+    c->stats.cells = c->hctl->total_cells;
+    launch_force(d);
+    launch_eval_end(d);
+    HIPCHK(hipGetLastError());
+    rc = read_ctl(c);
+    if (rc) return rc;
+    c->stats.evals += 1;
+    int st = c->hctl->status;
+    if (st & 2) return SPHMHD_WNAN;
+    if (st & 1) return SPHMHD_WNOCONV;
+    return 0;
+}
+
+static int check_ready(sphmhd_ctx* c) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    if (!c->bound) return fail(SPHMHD_ESTATE, "no fields bound");
+    return 0;
+}
+
+static int count_unconverged(sphmhd_ctx* c, int64_t* out) {
+    launch_count_unconverged(c->d);
+    int rc = read_ctl(c);
+    if (rc) return rc;
+    *out = c->hctl->count;
+    return 0;
+}
+
+extern "C" {
+
+int sphmhd_abi_version(void) { return SPHMHD_ABI_VERSION; }
+
+const char* sphmhd_last_error(void) { return g_err.c_str(); }
+
+int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_t* offsets, void* hip_stream,
+                  sphmhd_ctx** out) {
+    try {
+        if (!out) return fail(SPHMHD_EINVAL, "out is null");
+        *out = nullptr;
+        if (!params || !offsets || n_instances < 1) return fail(SPHMHD_EINVAL, "need at least one instance");
+        if (offsets[0] != 0) return fail(SPHMHD_EINVAL, "offsets[0] must be 0");
+        const sphmhd_params& p0 = params[0];
+        if (p0.dim != 2 && p0.dim != 3) return fail(SPHMHD_EINVAL, "dim must be 2 or 3");
+        if (p0.kernel < SPHMHD_CUBIC || p0.kernel > SPHMHD_SUPER_GAUSSIAN) return fail(SPHMHD_EINVAL, "unknown kernel");
+        for (int k = 0; k < n_instances; ++k) {
+            const sphmhd_params& p = params[k];
+            if (offsets[k + 1] < offsets[k]) return fail(SPHMHD_EINVAL, "offsets must be ascending");
+            if (p.dim != p0.dim || p.kernel != p0.kernel || p.periodic != p0.periodic)
+                return fail(SPHMHD_EINVAL, "dim, kernel and periodic must agree across instances");
+            if (!(p.gamma > 1.0)) return fail(SPHMHD_EINVAL, "gamma must exceed 1 (integrator.py:8)");
+            if (!(p.mu0 > 0.0) || !(p.hfact > 0.0)) return fail(SPHMHD_EINVAL, "mu0 and hfact must be positive");
+            if (p.max_iterations < 1) return fail(SPHMHD_EINVAL, "max_iterations must be >= 1");
+            for (int a = 0; a < p.dim; ++a)
+                if (!(p.hi[a] > p.lo[a])) return fail(SPHMHD_EINVAL, "domain bounds need hi > lo");
+        }
+        const int64_t n = offsets[n_instances];
+        if (n < 1 || n > (int64_t)0x3FFFFFFF) return fail(SPHMHD_EINVAL, "particle count out of range");
+
+        sphmhd_ctx* c = new sphmhd_ctx();
+        c->offsets.assign(offsets, offsets + n_instances + 1);
+        Dev& d = c->d;
+        d.n = n;
+        d.ninst = n_instances;
+        d.dim = p0.dim;
+        d.kernel = p0.kernel;
+        d.stream = (hipStream_t)hip_stream;
+        int64_t cap = 0;
+        for (int k = 0; k < n_instances; ++k) {
+            int64_t nk = offsets[k + 1] - offsets[k];
+            cap += 2 * nk > 1 ? 2 * nk : 1;
+        }
+        d.cell_cap = cap;
+        int bits = 1;
+        while (bits < 32 && ((int64_t)1 << bits) < cap) ++bits;
+        d.key_bits = bits;
+        d.nb_stride = p0.dim == 2 ? 64 : 96;
+        c->tiles = (n + 1023) / 1024;
+
+        c->hinst.resize(n_instances);
+        for (int k = 0; k < n_instances; ++k) {
+            const sphmhd_params& p = params[k];
+            Inst& s = c->hinst[k];
+            std::memset(&s, 0, sizeof(Inst));
+            s.off0 = offsets[k];
+            s.off1 = offsets[k + 1];
+            s.dim = p.dim;
+            s.kernel = p.kernel;
+            s.density_iterate = p.density_iterate;
+            s.min_iter = p.min_iterations;
+            s.max_iter = p.max_iterations;
+            s.periodic = p.periodic;
+            s.adaptive = p.adaptive_timestep;
+            for (int a = 0; a < 3; ++a) {
+                s.lo[a] = p.lo[a];
+                s.hi[a] = p.hi[a];
+                s.L[a] = a < p.dim ? p.hi[a] - p.lo[a] : 0.0;
+                s.halfL[a] = 0.5 * s.L[a];
+            }
+            s.gamma = p.gamma;
+            s.gamma1 = p.gamma - 1.0;
+            s.mu0 = p.mu0;
+            s.hfact = p.hfact;
+            s.av = p.artificial_viscosity;
+            s.ac = p.artificial_thermal_conductivity;
+            s.ar = p.artificial_magnetic_dissipation;
+            s.chf = p.cleaning_speed_factor;
+            s.sigma = p.cleaning_damping_factor;
+            s.tsf = p.timestep_factor;
+            s.alpha_min = p.alpha_min;
+            s.alpha_max = p.alpha_max;
+            s.tf = p.tf;
+            s.cleaning_active = p.cleaning_speed_factor > 0.0 ? 1.0 : 0.0;
+            s.c_cour = p.timestep_factor * 0.3;
+            s.c_force = p.timestep_factor * 0.25;
+            s.dtp_bits = 0x7FF0000000000000ull;
+            c->max_iter_all = std::max(c->max_iter_all, p.density_iterate ? p.max_iterations : 1);
+        }
+
+        std::vector<int32_t> inst_of(n);
+        for (int k = 0; k < n_instances; ++k)
+            for (int64_t i = offsets[k]; i < offsets[k + 1]; ++i) inst_of[i] = k;
+
+        int rc = 0;
+        auto cleanup = [&](int code) {
+            sphmhd_destroy(c);
+            return code;
+        };
+        if ((rc = dalloc(&d.inst, n_instances))) return cleanup(rc);
+        if ((rc = dalloc(&d.ctl, 1))) return cleanup(rc);
+        if ((rc = dalloc(&d.inst_of, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.keys, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.keys_alt, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.perm, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.perm_alt, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.rs_hist, 256 * c->tiles))) return cleanup(rc);
+        if ((rc = dalloc(&d.rs_base, 256))) return cleanup(rc);
+        if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.fpack, (int64_t)FREC * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.nb, (int64_t)d.nb_stride * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.nbcnt, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.scratch3, 3 * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
+        hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocDefault);
+        if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostMalloc failed"));
+        e = hipMemcpy(d.inst, c->hinst.data(), sizeof(Inst) * n_instances, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
+        e = hipMemcpy(d.inst_of, inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
+        *out = c;
+        return 0;
+    } catch (const std::bad_alloc&) {
+        return fail(SPHMHD_ECAP, "host allocation failed");
+    } catch (...) {
+        return fail(SPHMHD_EINVAL, "unexpected exception");
+    }
+}
+
+int sphmhd_destroy(sphmhd_ctx* c) {
+    if (!c) return 0;
+    Dev& d = c->d;
+    (void)hipStreamSynchronize(d.stream);
+    void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
+                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.totals};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->hctl) (void)hipHostFree(c->hctl);
+    delete c;
+    return 0;
+}
+
+int sphmhd_bind(sphmhd_ctx* c, double* const* fields, int32_t* itype) {
+    if (!c || !fields || !itype) return fail(SPHMHD_EINVAL, "null argument");
+    for (int f = 0; f < SPHMHD_NFIELDS; ++f) {
+        bool optional = c->d.dim == 2 && (f == SPHMHD_F_Z || f == SPHMHD_F_Z0);
+        if (!fields[f] && !optional) return fail(SPHMHD_EINVAL, "field " + std::to_string(f) + " is null");
+        c->d.fields.f[f] = fields[f];
+    }
+    c->d.fields.itype = itype;
+    c->bound = true;
+    return 0;
+}
+
+int sphmhd_eval(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    try {
+        return do_eval(c);
+    } catch (...) {
+        return fail(SPHMHD_EINVAL, "unexpected exception");
+    }
+}
+
+int sphmhd_pec_initialize(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_initialize(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_pec_stage1(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_stage1(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_pec_stage2(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_stage2(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_advance_clock(sphmhd_ctx* c) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    launch_clock(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(SPHMHD_EINVAL, "nsteps < 0");
+    if (nsteps == 0) return 0;
+    int worst = 0;
+    try {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            if (s == 0) {
+                launch_initialize(c->d);
+                launch_stage1(c->d);
+            } else {
+                launch_stage2_init_stage1(c->d);
+                launch_clock(c->d);
+            }
+            HIPCHK(hipGetLastError());
+            rc = do_eval(c);
+            if (rc < 0) return rc;
+            if (rc > worst) worst = rc;
+        }
+        launch_stage2(c->d);
+        launch_clock(c->d);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->d.stream));
+    } catch (...) {
+        return fail(SPHMHD_EINVAL, "unexpected exception");
+    }
+    return worst;
+}
+
+int sphmhd_sync_initial_state(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_sync_initial(c->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_warm_start(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    Dev& d = c->d;
+    double** f = d.fields.f;
+    const int64_t n = d.n;
+    launch_copy(d, d.scratch3, f[SPHMHD_F_BX], n);
+    launch_copy(d, d.scratch3 + n, f[SPHMHD_F_BY], n);
+    launch_copy(d, d.scratch3 + 2 * n, f[SPHMHD_F_BZ], n);
+    launch_copy(d, f[SPHMHD_F_H0], f[SPHMHD_F_H], n);
+    launch_sync_initial(d);
+    rc = do_eval(c);
+    if (rc < 0) return rc;
+    int64_t bad = 0;
+    if ((rc = count_unconverged(c, &bad))) return rc;
+    if (bad) return fail(SPHMHD_ENOTCONV, "warm start: " + std::to_string(bad) + " particles did not converge");
+    launch_copy(d, f[SPHMHD_F_BX], d.scratch3, n);
+    launch_copy(d, f[SPHMHD_F_BY], d.scratch3 + n, n);
+    launch_copy(d, f[SPHMHD_F_BZ], d.scratch3 + 2 * n, n);
+    launch_copy(d, f[SPHMHD_F_H0], f[SPHMHD_F_H], n);
+    launch_sync_initial(d);
+    for (int r = 0; r < 2; ++r) {
+        rc = do_eval(c);
+        if (rc < 0) return rc;
+    }
+    if ((rc = count_unconverged(c, &bad))) return rc;
+    if (bad) return fail(SPHMHD_ENOTCONV, "warm start: " + std::to_string(bad) + " particles did not converge");
+    launch_warm_dt(d);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_ctl(c))) return rc;
+    if (c->hctl->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
+    return 0;
+}
+
+int sphmhd_get_clock(sphmhd_ctx* c, double* t, double* dt) {
+    if (!c || !t || !dt) return fail(SPHMHD_EINVAL, "null argument");
+    int rc = pull_inst(c);
+    if (rc) return rc;
+    for (size_t k = 0; k < c->hinst.size(); ++k) {
+        t[k] = c->hinst[k].t;
+        dt[k] = c->hinst[k].dt;
+    }
+    return 0;
+}
+
+int sphmhd_set_clock(sphmhd_ctx* c, const double* t, const double* dt) {
+    if (!c || !t || !dt) return fail(SPHMHD_EINVAL, "null argument");
+    int rc = pull_inst(c);
+    if (rc) return rc;
+    for (size_t k = 0; k < c->hinst.size(); ++k) {
+        c->hinst[k].t = t[k];
+        c->hinst[k].dt = dt[k];
+    }
+    return push_inst(c);
+}
+
+int sphmhd_get_pending_dt(sphmhd_ctx* c, double* dt) {
+    if (!c || !dt) return fail(SPHMHD_EINVAL, "null argument");
+    int rc = pull_inst(c);
+    if (rc) return rc;
+    for (size_t k = 0; k < c->hinst.size(); ++k) {
+        unsigned long long b = c->hinst[k].dtp_bits;
+        std::memcpy(&dt[k], &b, sizeof(double));
+    }
+    return 0;
+}
+
+int sphmhd_totals(sphmhd_ctx* c, double* out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return fail(SPHMHD_EINVAL, "null argument");
+    launch_totals(c->d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->d.totals, sizeof(double) * c->d.ninst * SPHMHD_NTOTALS, hipMemcpyDeviceToHost,
+                          c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+int sphmhd_cells(sphmhd_ctx* c, uint32_t* keys, int32_t* perm) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!keys || !perm) return fail(SPHMHD_EINVAL, "null argument");
+    if ((rc = nnps_rebuild(c, false))) return rc;
+    HIPCHK(hipMemcpyAsync(keys, c->d.keys, sizeof(uint32_t) * c->d.n, hipMemcpyDeviceToHost, c->d.stream));
+    HIPCHK(hipMemcpyAsync(perm, c->d.perm, sizeof(int32_t) * c->d.n, hipMemcpyDeviceToHost, c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+int sphmhd_neighbours(sphmhd_ctx* c, int64_t* offs, int32_t* idx, int64_t cap, int64_t* total) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!offs || !total || (cap > 0 && !idx)) return fail(SPHMHD_EINVAL, "null argument");
+    try {
+        if ((rc = nnps_rebuild(c, false))) return rc;
+        launch_pack_xh(c->d);
+        if ((rc = build_lists(c))) return rc;
+        const int64_t n = c->d.n;
+        const int stride = c->d.nb_stride;
+        std::vector<int32_t> nb((size_t)stride * n), cnt(n), perm(n);
+        HIPCHK(hipMemcpyAsync(nb.data(), c->d.nb, sizeof(int32_t) * nb.size(), hipMemcpyDeviceToHost, c->d.stream));
+        HIPCHK(hipMemcpyAsync(cnt.data(), c->d.nbcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->d.stream));
+        HIPCHK(hipMemcpyAsync(perm.data(), c->d.perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->d.stream));
+        HIPCHK(hipStreamSynchronize(c->d.stream));
+        std::vector<std::vector<int32_t>> lists(n);
+        for (int64_t t = 0; t < n; ++t) {
+            std::vector<int32_t>& l = lists[perm[t]];
+            l.resize(cnt[t]);
+            for (int q = 0; q < cnt[t]; ++q) l[q] = perm[nb[(size_t)q * n + t]];
+            std::sort(l.begin(), l.end());
+        }
+        int64_t pos = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            offs[i] = pos;
+            for (int32_t j : lists[i]) {
+                if (pos < cap) idx[pos] = j;
+                ++pos;
+            }
+        }
+        offs[n] = pos;
+        *total = pos;
+        return pos > cap ? fail(SPHMHD_ECAP, "idx buffer too small") : 0;
+    } catch (const std::bad_alloc&) {
+        return fail(SPHMHD_ECAP, "host allocation failed");
+    }
+}
+
+int sphmhd_get_stats(sphmhd_ctx* c, sphmhd_stats* out) {
+    if (!c || !out) return fail(SPHMHD_EINVAL, "null argument");
+    *out = c->stats;
+    return 0;
+}
+
+int sphmhd_synchronize(sphmhd_ctx* c) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+// sph_internal.h -- shared definitions of the gfx950 SPH-MHD library.
+//
+// Layout in HBM
+//   * Bound particle properties stay in the caller's storage order (one FP64
+//     array per property, the PySPH ParticleArray layout).
+//   * Each evaluation bins particles into cells, radix-sorts the cell keys and
+//     then writes cell-ordered AoS "packs" of exactly what the pair loops read
+//     from a neighbour: a 32-byte record for the density pass and a 256-byte
+//     (two cache lines) record for the force pass.  Pair kernels run one
+//     thread per destination in cell order, so a wave's neighbours sit in a
+//     handful of contiguous pack runs.
+//   * Per-instance scalars (grid, clock, reductions) live in one device array
+//     of Inst, so an ensemble of uneven instances runs in the same launches as
+//     a single instance.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sphmhd.h"
+
+namespace sph {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+// Force-pack record slots (FP64 each, 32 per particle).
+enum FSlot {
+    FX = 0, FY, FZ, FH,
+    FM, FRHO, FP, FE,
+    FU, FV, FW, FOMEGA,
+    FBX, FBY, FBZ, FALPHA,
+    FAU, FAV, FAW, FPSICH,     // a of the previous evaluation; psipred * c_h
+    FVWAVE, FPBRHO2, FRHO2INV, FHALFINVRHO,
+    FBRHOX, FBRHOY, FBRHOZ, FBRO2,
+    FCS, FPSI, FH1, FSPARE,
+    FREC = 32
+};
+
+struct Inst {
+    // static parameters (filled at create)
+    int64_t off0, off1;
+    int32_t dim, kernel, density_iterate, min_iter;
+    int32_t max_iter, periodic, adaptive, pad0;
+    double lo[3], hi[3], L[3], halfL[3];
+    double gamma, gamma1, mu0, hfact;
+    double av, ac, ar, chf;
+    double sigma, tsf, alpha_min, alpha_max;
+    double tf, cleaning_active, c_cour, c_force;
+    // grid of the current evaluation
+    int32_t n[3];
+    int32_t srad[3];
+    double ceff[3];
+    int64_t cell_base, ncell;
+    // clock
+    double t, dt;
+    // reduction targets (bit patterns of positive doubles order like integers)
+    unsigned long long hmax_bits;       // max h for the current pass
+    unsigned long long hmax_next_bits;  // max h after the running density pass
+    unsigned long long dtp_bits;        // min dt_adapt of the last force pass
+    int32_t iters, unconv, done, status;
+};
+
+struct Ctl {
+    int64_t total_cells;
+    int32_t all_done;
+    int32_t status;
+    int32_t max_nb;
+    int32_t iters_max;
+    int64_t pairs;
+    int64_t count;
+};
+
+struct Fields {
+    double* f[SPHMHD_NFIELDS];
+    int32_t* itype;
+};
+
+// ---------------------------------------------------------------------------
+// exact helpers (must round exactly like the CPU oracle)
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline double kernel_radius(int k) {
+    return (k == SPHMHD_QUINTIC || k == SPHMHD_GAUSSIAN || k == SPHMHD_SUPER_GAUSSIAN) ? 3.0 : 2.0;
+}
+
+__device__ __forceinline__ double min_image(double d, double L, double halfL, int periodic) {
+    if (periodic) {
+        if (d > halfL) d -= L;
+        else if (d < -halfL) d += L;
+    }
+    return d;
+}
+
+__device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    double s = dx * dx + dy * dy;
+    return s + dz * dz;
+}
+
+__device__ __forceinline__ int cell_coord(double x, double lo, double ceff, int n) {
+    double f = floor((x - lo) / ceff);
+    if (f < 0.0) return 0;
+    if (f > (double)(n - 1)) return n - 1;
+    return (int)f;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// stencil half-width per axis from the instance's current h_max
+__device__ inline void stencil_from_hmax(Inst& s) {
+    double R = kernel_radius(s.kernel);
+    double hm = bitsd(s.hmax_bits);
+    for (int a = 0; a < 3; ++a) {
+        if (a < s.dim) {
+            double r = ceil(R * hm / s.ceff[a]);
+            s.srad[a] = !(r < 1.0e6) ? 1000000 : (r < 1.0 ? 1 : (int)r);
+        } else {
+            s.srad[a] = 0;
+        }
+    }
+}
+
+// time step after a step: the pending min dt_adapt (adaptive) or the current
+// dt, clipped to tf - t (solver.py:61)
+__host__ __device__ inline double next_dt(const Inst& s, double t, double dt_now, double pending) {
+    double dt = s.adaptive ? pending : dt_now;
+    if (t + dt > s.tf) dt = s.tf - t;
+    return dt;
+}
+
+// ---------------------------------------------------------------------------
+// SPH kernel shapes w(q), w'(q) (normalisation applied by the caller)
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline double kernel_sigma(int k, int dim) {
+    const double pi = 3.14159265358979323846;
+    switch (k) {
+    case SPHMHD_CUBIC: return dim == 2 ? 10.0 / (7.0 * pi) : 1.0 / pi;
+    case SPHMHD_QUINTIC: return dim == 2 ? 7.0 / (478.0 * pi) : 1.0 / (120.0 * pi);
+    case SPHMHD_WENDLAND_C2: return dim == 2 ? 7.0 / (4.0 * pi) : 21.0 / (16.0 * pi);
+    case SPHMHD_WENDLAND_C4: return dim == 2 ? 9.0 / (4.0 * pi) : 495.0 / (256.0 * pi);
+    case SPHMHD_WENDLAND_C6: return dim == 2 ? 78.0 / (28.0 * pi) : 1365.0 / (512.0 * pi);
+    case SPHMHD_GAUSSIAN:
+    case SPHMHD_SUPER_GAUSSIAN: return dim == 2 ? 1.0 / pi : 1.0 / (pi * 1.7724538509055160273);
+    }
+    return 0.0;
+}
+
+template <int K, int DIM>
+__device__ __forceinline__ void shape(double q, double& w, double& dw) {
+    if constexpr (K == SPHMHD_CUBIC) {
+        double t = fmax(2.0 - q, 0.0);
+        if (q > 1.0) {
+            w = 0.25 * t * t * t;
+            dw = -0.75 * t * t;
+        } else {
+            w = 1.0 - 1.5 * q * q * (1.0 - 0.5 * q);
+            dw = -3.0 * q * (1.0 - 0.75 * q);
+        }
+    } else if constexpr (K == SPHMHD_QUINTIC) {
+        double t3 = fmax(3.0 - q, 0.0), t2 = fmax(2.0 - q, 0.0), t1 = fmax(1.0 - q, 0.0);
+        double a3 = t3 * t3, a2 = t2 * t2, a1 = t1 * t1;
+        double b3 = a3 * a3, b2 = a2 * a2, b1 = a1 * a1;
+        w = b3 * t3 - 6.0 * (b2 * t2) + 15.0 * (b1 * t1);
+        dw = -5.0 * b3 + 30.0 * b2 - 75.0 * b1;
+    } else if constexpr (K == SPHMHD_WENDLAND_C2) {
+        double t = fmax(1.0 - 0.5 * q, 0.0);
+        double t3 = t * t * t;
+        w = t3 * t * (2.0 * q + 1.0);
+        dw = -5.0 * q * t3;
+    } else if constexpr (K == SPHMHD_WENDLAND_C4) {
+        double t = fmax(1.0 - 0.5 * q, 0.0);
+        double t2 = t * t;
+        double t5 = t2 * t2 * t;
+        w = t5 * t * ((35.0 / 12.0) * q * q + 3.0 * q + 1.0);
+        dw = -(14.0 / 3.0) * q * (1.0 + 2.5 * q) * t5;
+    } else if constexpr (K == SPHMHD_WENDLAND_C6) {
+        double t = fmax(1.0 - 0.5 * q, 0.0);
+        double t2 = t * t;
+        double t4 = t2 * t2;
+        double t7 = t4 * t2 * t;
+        w = t7 * t * (4.0 * q * q * q + 6.25 * q * q + 4.0 * q + 1.0);
+        dw = -5.5 * q * t7 * (1.0 + 3.5 * q + 4.0 * q * q);
+    } else if constexpr (K == SPHMHD_GAUSSIAN) {
+        double g = q < 3.0 ? exp(-q * q) : 0.0;
+        w = g;
+        dw = -2.0 * q * g;
+    } else {
+        double g = q < 3.0 ? exp(-q * q) : 0.0;
+        const double d2 = 0.5 * DIM;
+        w = g * (d2 + 1.0 - q * q);
+        dw = g * (-2.0 * q * (d2 + 1.0 - q * q) - 2.0 * q);
+    }
+}
+
+template <int K, int DIM>
+__device__ __forceinline__ double shape_dw(double q) {
+    double w, dw;
+    shape<K, DIM>(q, w, dw);
+    return dw;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+
+struct Dev {
+    // sizes
+    int64_t n;
+    int32_t ninst;
+    int32_t dim, kernel;
+    int64_t cell_cap;
+    int32_t key_bits;
+    int32_t nb_stride;      // neighbour-list capacity per particle
+    hipStream_t stream;
+    // device arrays
+    Inst* inst;
+    Ctl* ctl;
+    int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
+    uint32_t *keys, *keys_alt;
+    int32_t *perm, *perm_alt;
+    uint32_t* rs_hist;      // 256 x tiles
+    uint32_t* rs_base;      // 256
+    int32_t *cstart, *cend;
+    double* dpack;          // 4 doubles per sorted particle
+    double* fpack;          // FREC doubles per sorted particle
+    int32_t* nb;            // nb_stride x n, column major
+    int32_t* nbcnt;
+    double* scratch3;       // 3 x n (warm-start B copy)
+    double* totals;         // ninst x SPHMHD_NTOTALS
+    Fields fields;
+};
+
+void launch_eval_begin(Dev& d, bool reset_dt);
+void launch_pack_xh(Dev& d);
+void launch_reset_nb_stats(Dev& d);
+void launch_count_unconverged(Dev& d);
+void launch_hmax(Dev& d, bool next);
+void launch_grid(Dev& d);
+void launch_keys_sort_ranges(Dev& d);
+void launch_pack_density(Dev& d);
+void launch_density(Dev& d);
+void launch_g0_control(Dev& d);
+void launch_prep_force(Dev& d);
+void launch_nb_build(Dev& d);
+void launch_force(Dev& d);
+void launch_eval_end(Dev& d);
+
+void launch_initialize(Dev& d);
+void launch_stage1(Dev& d);
+void launch_stage2(Dev& d);
+void launch_stage2_init_stage1(Dev& d);
+void launch_clock(Dev& d);
+void launch_sync_initial(Dev& d);
+void launch_copy(Dev& d, double* dst, const double* src, int64_t n);
+void launch_totals(Dev& d);
+void launch_warm_dt(Dev& d);
+void launch_fill_i32(Dev& d, int32_t* p, int32_t v, int64_t n);
+
+}  // namespace sph

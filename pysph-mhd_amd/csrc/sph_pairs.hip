@@ -1,0 +1,586 @@
+// sph_pairs.hip -- the pair loops of one force evaluation.
+//
+//   k_density   G0  DensityIteration loop + post_loop (equations.py:38-88)
+//   k_g0_control    Group(iterate=True, min 2, max 250) control (scheme.py:71)
+//   k_prep_force G1 + G2 + G4 (equations.py:104-130, 476-485) and the cell-ordered
+//                   force pack with per-particle hoisted terms
+//   k_force     G3 + G5 + G7 + G8 pair loops (equations.py:143-567) fused into one
+//               neighbour traversal, with the G3/G5/G8 post_loops and G6 + G9
+//               (equations.py:133-140, 570-624) as the epilogue.
+//
+// Fusion is legal because G3, G5, G7 and G8 read only neighbour values fixed by
+// G0..G4; the one hazard (G3 reads the neighbours' previous a while G5 writes
+// a) is removed by packing the previous a into the force pack before the loop.
+// One thread per destination in cell order; FP64 throughout; no atomics on
+// particle data (gather form), so results are deterministic.
+#include "sph_internal.h"
+
+namespace sph {
+
+__device__ __forceinline__ double ipow_dim(double x, int dim) { return dim == 2 ? x * x : x * x * x; }
+__device__ __forceinline__ double iroot_dim(double x, int dim) { return dim == 2 ? sqrt(x) : cbrt(x); }
+
+// ---------------------------------------------------------------------------
+// G0: density sum + Newton step for h
+// ---------------------------------------------------------------------------
+
+template <int K, int DIM>
+__global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __restrict__ dpack,
+                                                    const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
+                                                    const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                                                    Inst* inst, const int32_t* __restrict__ inst_of, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = t < n;
+    int ki = valid ? inst_of[t] : 0;
+    const Inst& s = inst[ki];
+    bool live = valid && !s.done;
+    int32_t i = valid ? perm[t] : 0;
+    double hi = live ? F.f[SPHMHD_F_H][i] : 0.0;
+    bool active = live && F.f[SPHMHD_F_CONVERGED][i] != 1.0;
+    double hout = hi;
+    bool unconv = false;
+
+    if (active) {
+        const double R = kernel_radius(K);
+        const double4 me = dpack[t];
+        const double h1 = 1.0 / hi;
+        const double Rh = R * hi;
+        const double Rh2 = Rh * Rh;
+        int64_t local = (int64_t)skey[t] - s.cell_base;
+        int c[3];
+        c[0] = (int)(local % s.n[0]);
+        c[1] = (int)((local / s.n[0]) % s.n[1]);
+        c[2] = DIM == 3 ? (int)(local / ((int64_t)s.n[0] * s.n[1])) : 0;
+        int lo[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
+        for (int a = 0; a < DIM; ++a) {
+            int sr = s.srad[a];
+            if (s.periodic && 2 * sr + 1 >= s.n[a]) { lo[a] = 0; hi_[a] = s.n[a] - 1; }
+            else if (s.periodic) { lo[a] = c[a] - sr; hi_[a] = c[a] + sr; }
+            else { lo[a] = c[a] - sr < 0 ? 0 : c[a] - sr; hi_[a] = c[a] + sr > s.n[a] - 1 ? s.n[a] - 1 : c[a] + sr; }
+        }
+        double sw = 0.0, sg = 0.0;
+        for (int qz = lo[2]; qz <= hi_[2]; ++qz) {
+            int wz = DIM == 3 ? ((qz % s.n[2]) + s.n[2]) % s.n[2] : 0;
+            for (int qy = lo[1]; qy <= hi_[1]; ++qy) {
+                int wy = ((qy % s.n[1]) + s.n[1]) % s.n[1];
+                for (int qx = lo[0]; qx <= hi_[0]; ++qx) {
+                    int wx = ((qx % s.n[0]) + s.n[0]) % s.n[0];
+                    int64_t cell = s.cell_base + wx + (int64_t)s.n[0] * (wy + (int64_t)s.n[1] * wz);
+                    int j1 = cend[cell];
+                    for (int j = cstart[cell]; j < j1; ++j) {
+                        double4 o = dpack[j];
+                        double dx = min_image(me.x - o.x, s.L[0], s.halfL[0], s.periodic);
+                        double dy = min_image(me.y - o.y, s.L[1], s.halfL[1], s.periodic);
+                        double dz = DIM == 3 ? min_image(me.z - o.z, s.L[2], s.halfL[2], s.periodic) : 0.0;
+                        double r2 = r2_exact(dx, dy, dz);
+                        if (r2 < Rh2) {
+                            double q = sqrt(r2) * h1;
+                            double w, dw;
+                            shape<K, DIM>(q, w, dw);
+                            sw += o.w * w;
+                            sg += o.w * ((double)DIM * w + q * dw);
+                        }
+                    }
+                }
+            }
+        }
+        const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
+        const double rho_sum = sw * fac;
+        const double dwdh = -h1 * (fac * sg);
+
+        // DensityIteration.post_loop (equations.py:54-86)
+        const double mi = me.w;
+        const double hi0 = F.f[SPHMHD_F_H0][i];
+        const double rhoi = mi / ipow_dim(hi / s.hfact, DIM);
+        const double dhdrhoi = -hi / (DIM * rhoi);
+        const double omegai = 1.0 - dhdrhoi * dwdh;
+        F.f[SPHMHD_F_OMEGA][i] = 1.0 / omegai;
+        const double func = rhoi - rho_sum;
+        const double dfdh1 = omegai > 0.0 ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + 2.220446049250313e-16);
+        double hnew = hi - func * dfdh1;
+        if (hnew > 1.2 * hi) hnew = 1.2 * hi;
+        else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
+        const double diff = fabs(hnew - hi) / hi0;
+        const bool fin = ((diff < 1.0e-4) && (omegai > 0.0) && (hi > 0.0)) || !s.density_iterate;
+        if (!fin) {
+            hout = hnew;
+            unconv = true;
+            F.f[SPHMHD_F_CONVERGED][i] = 0.0;
+        } else {
+            hout = s.hfact * iroot_dim(mi / rho_sum, DIM);
+            F.f[SPHMHD_F_CONVERGED][i] = 1.0;
+        }
+        F.f[SPHMHD_F_H][i] = hout;
+    }
+
+    // reductions: max h after this pass and the unconverged count, per instance
+    unsigned long long hb = live ? dbits(hout) : 0ull;
+    int uc = unconv ? 1 : 0;
+    int k0 = __shfl(ki, 0);
+    bool uniform = __all(ki == k0 || !valid);
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(hb, off);
+            hb = o > hb ? o : hb;
+            uc += __shfl_xor(uc, off);
+        }
+        if ((threadIdx.x & 63) == 0 && !inst[k0].done) {
+            atomicMax(&inst[k0].hmax_next_bits, hb);
+            if (uc) atomicAdd(&inst[k0].unconv, uc);
+        }
+    } else if (live) {
+        atomicMax(&inst[ki].hmax_next_bits, hb);
+        if (uc) atomicAdd(&inst[ki].unconv, uc);
+    }
+}
+
+// Group iteration control: iterations >= min and converged, or == max.
+__global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl) {
+    __shared__ int alldone;
+    if (threadIdx.x == 0) alldone = 1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+        Inst& s = inst[k];
+        if (!s.done) {
+            s.iters += 1;
+            bool conv = s.unconv == 0;
+            bool stop = !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
+            if (s.density_iterate && !(s.iters >= s.min_iter && conv) && s.iters >= s.max_iter) s.status |= 1;
+            s.unconv = 0;
+            s.hmax_bits = s.hmax_next_bits;
+            s.hmax_next_bits = 0ull;
+            stencil_from_hmax(s);
+            if (stop) s.done = 1;
+            atomicMax(&ctl->iters_max, s.iters);
+        }
+        if (!s.done) atomicAnd(&alldone, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ctl->all_done = alldone;
+}
+
+// ---------------------------------------------------------------------------
+// G1 + G2 + G4 and the force pack
+// ---------------------------------------------------------------------------
+
+__global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
+                             const int32_t* __restrict__ inst_of, double* __restrict__ fpack, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Inst& s = inst[inst_of[t]];
+    const int32_t i = perm[t];
+    const int dim = s.dim;
+    double** f = F.f;
+    const double h = f[SPHMHD_F_H][i];
+    const double m = f[SPHMHD_F_M][i];
+    // DensityFromSmoothingLength (equations.py:118)
+    const double rho = m / ipow_dim(h / s.hfact, dim);
+    // IdealGasEOS (equations.py:105-106)
+    const double epred = f[SPHMHD_F_EPRED][i];
+    const double p = s.gamma1 * rho * epred;
+    const double cs = sqrt(s.gamma * p / rho);
+    // MagneticStateReconstruction (equations.py:477-485)
+    const double bx = f[SPHMHD_F_BEVOLXPRED][i] * rho;
+    const double by = f[SPHMHD_F_BEVOLYPRED][i] * rho;
+    const double bz = f[SPHMHD_F_BEVOLZPRED][i] * rho;
+    f[SPHMHD_F_RHO][i] = rho;
+    f[SPHMHD_F_P][i] = p;
+    f[SPHMHD_F_CS][i] = cs;
+    f[SPHMHD_F_BX][i] = bx;
+    f[SPHMHD_F_BY][i] = by;
+    f[SPHMHD_F_BZ][i] = bz;
+    f[SPHMHD_F_BXPRED][i] = bx;
+    f[SPHMHD_F_BYPRED][i] = by;
+    f[SPHMHD_F_BZPRED][i] = bz;
+
+    const double b2 = bx * bx + by * by + bz * bz;
+    const double vwave = sqrt(cs * cs + b2 / (s.mu0 * rho));
+    const double inv_rho = 1.0 / rho;
+    const double brx = bx * inv_rho, bry = by * inv_rho, brz = bz * inv_rho;
+    const double psipred = f[SPHMHD_F_PSIPRED][i];
+    double r[FREC];
+    r[FX] = f[SPHMHD_F_X][i];
+    r[FY] = f[SPHMHD_F_Y][i];
+    r[FZ] = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
+    r[FH] = h;
+    r[FM] = m;
+    r[FRHO] = rho;
+    r[FP] = p;
+    r[FE] = epred;
+    r[FU] = f[SPHMHD_F_UPRED][i];
+    r[FV] = f[SPHMHD_F_VPRED][i];
+    r[FW] = f[SPHMHD_F_WPRED][i];
+    r[FOMEGA] = f[SPHMHD_F_OMEGA][i];
+    r[FBX] = bx;
+    r[FBY] = by;
+    r[FBZ] = bz;
+    r[FALPHA] = f[SPHMHD_F_ALPHA1][i];
+    r[FAU] = f[SPHMHD_F_AU][i];
+    r[FAV] = f[SPHMHD_F_AV][i];
+    r[FAW] = f[SPHMHD_F_AW][i];
+    r[FPSICH] = psipred * (s.chf * vwave);
+    r[FVWAVE] = vwave;
+    r[FPBRHO2] = p / (rho * rho);
+    r[FRHO2INV] = 1.0 / (rho * rho);
+    r[FHALFINVRHO] = 0.5 * inv_rho;
+    r[FBRHOX] = brx;
+    r[FBRHOY] = bry;
+    r[FBRHOZ] = brz;
+    r[FBRO2] = (brx * brx + bry * bry + brz * brz) / s.mu0;
+    r[FCS] = cs;
+    r[FPSI] = psipred;
+    r[FH1] = 1.0 / h;
+    r[FSPARE] = 0.0;
+    double4* out = (double4*)(fpack + t * FREC);
+#pragma unroll
+    for (int q = 0; q < FREC / 4; ++q) out[q] = make_double4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// fused force pass
+// ---------------------------------------------------------------------------
+
+template <int DIM>
+struct Acc {
+    double ax, ay, az, ae, div, dtcfl;
+    double bx, by, bz, divbs, divbd;
+    // Cullen-Dehnen sums (equations.py:329-354); in 2-D only the entries that
+    // can be non-zero are kept
+    double rxx, rxy, rxz, ryy, ryz, rzz;
+    double vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz;
+    double axx, axy, axz, ayx, ayy, ayz, azx, azy, azz;
+};
+
+template <int K, int DIM>
+__global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
+                                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
+                                                  const int32_t* __restrict__ nbcnt, Inst* inst,
+                                                  const int32_t* __restrict__ inst_of, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = t < n;
+    int ki = valid ? inst_of[t] : 0;
+    const Inst& s = inst[ki];
+    double dt_adapt = 1.0e300;
+    bool bad = false;
+    if (valid) {
+        const double* me = fpack + t * FREC;
+        const double xi = me[FX], yi = me[FY], zi = DIM == 3 ? me[FZ] : 0.0;
+        const double hi = me[FH], h1i = me[FH1];
+        const double rhoi = me[FRHO], pi_ = me[FP], ei = me[FE];
+        const double ui = me[FU], vi = me[FV], wi = me[FW];
+        const double omi = me[FOMEGA], ali = me[FALPHA];
+        const double bxi = me[FBX], byi = me[FBY], bzi = me[FBZ];
+        const double aui = me[FAU], avi = me[FAV], awi = me[FAW];
+        const double psichi = me[FPSICH], vwi = me[FVWAVE], pbi = me[FPBRHO2];
+        const double r2ii = me[FRHO2INV], hiri = me[FHALFINVRHO];
+        const double brxi = me[FBRHOX], bryi = me[FBRHOY], brzi = me[FBRHOZ], bro2i = me[FBRO2];
+        const double fac_i = (DIM == 2 ? kernel_sigma(K, 2) * h1i * h1i : kernel_sigma(K, 3) * h1i * h1i * h1i) * h1i;
+        const double inv_mu0 = 1.0 / s.mu0;
+        const double av = s.av, ac = s.ac, ar = s.ar;
+        const double sig2 = DIM == 2 ? kernel_sigma(K, 2) : kernel_sigma(K, 3);
+
+        Acc<DIM> A = {};
+        const int cnt = nbcnt[t];
+        for (int q = 0; q < cnt; ++q) {
+            const int j = nb[(int64_t)q * n + t];
+            const double* o = fpack + (int64_t)j * FREC;
+            const double4 o0 = *(const double4*)(o + 0);   // x y z h
+            const double4 o1 = *(const double4*)(o + 4);   // m rho p e
+            const double4 o2 = *(const double4*)(o + 8);   // u v w omega
+            const double4 o3 = *(const double4*)(o + 12);  // bx by bz alpha
+            const double4 o4 = *(const double4*)(o + 16);  // au av aw psich
+            const double4 o5 = *(const double4*)(o + 20);  // vwave pbrho2 rho2inv halfinvrho
+            const double4 o6 = *(const double4*)(o + 24);  // brho(3) bro2
+            const double h1j = o[FH1];
+
+            double dx = min_image(xi - o0.x, s.L[0], s.halfL[0], s.periodic);
+            double dy = min_image(yi - o0.y, s.L[1], s.halfL[1], s.periodic);
+            double dz = DIM == 3 ? min_image(zi - o0.z, s.L[2], s.halfL[2], s.periodic) : 0.0;
+            const double r2 = r2_exact(dx, dy, dz);
+            const double r = sqrt(r2);
+            const double mj = o1.x;
+
+            // kernel gradients DWI, DWJ (0 for r <= 1e-12)
+            double gi = 0.0, gj = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+            const bool far12 = r > 1.0e-12;
+            if (far12) {
+                const double ir = 1.0 / r;
+                gi = shape_dw<K, DIM>(r * h1i) * fac_i * ir;
+                const double facj = (DIM == 2 ? sig2 * h1j * h1j : sig2 * h1j * h1j * h1j) * h1j;
+                gj = shape_dw<K, DIM>(r * h1j) * facj * ir;
+                ex = dx * ir;
+                ey = dy * ir;
+                ez = dz * ir;
+            }
+            const double dwix = gi * dx, dwiy = gi * dy, dwiz = gi * dz;
+            const double dwjx = gj * dx, dwjy = gj * dy, dwjz = gj * dz;
+
+            // relative velocity, projections
+            const double dvx = ui - o2.x, dvy = vi - o2.y, dvz = wi - o2.z;
+            const double gri = omi * (ex * dwix + ey * dwiy + ez * dwiz);
+            const double grj = o2.w * (ex * dwjx + ey * dwjy + ez * dwjz);
+            const double dot = dvx * ex + dvy * ey + dvz * ez;
+
+            // ---- MHDAccelerations (equations.py:217-275); unit vector is 0 below 1e-8
+            {
+                const bool near8 = r < 1.0e-8;
+                const double dotA = near8 ? 0.0 : dot;
+                const double griA = near8 ? 0.0 : gri;
+                const double grjA = near8 ? 0.0 : grj;
+                A.div += mj * dotA * griA;
+                const double vsig2 = sqrt(fabs(pi_ - o1.z) / (0.5 * (rhoi + o1.y)));
+                double c = fmax(vwi - 2.0 * dotA, o5.x - 2.0 * dotA);
+                c = fmax(c, 0.0);
+                A.dtcfl = fmax(A.dtcfl, c);
+                double sx = pbi * omi, sjx = o5.y * o2.w;  // P/rho^2 * omega
+                if (dotA < 0.0) {
+                    const double vsi = fmax(ali * vwi - 2.0 * dotA, 0.0);
+                    const double vsj = fmax(o3.w * o5.x - 2.0 * dotA, 0.0);
+                    const double qi = -hiri * vsi * dotA * av;
+                    const double qj = -o5.w * vsj * dotA * av;
+                    sx += qi * omi;
+                    sjx += qj * o2.w;
+                    A.ae += mj * qi * dotA * griA;
+                }
+                A.ax -= mj * (sx * dwix + sjx * dwjx);
+                A.ay -= mj * (sx * dwiy + sjx * dwjy);
+                A.az -= mj * (sx * dwiz + sjx * dwjz);
+                A.ae += mj * pbi * omi * (dvx * dwix + dvy * dwiy + dvz * dwiz);
+                A.ae += ac * vsig2 * (ei - o1.w) * mj * (hiri * griA + o5.w * grjA);
+            }
+            // ---- MagneticStressAccelerations (equations.py:149-195)
+            {
+                const double gradp = mj * (0.5 * bro2i * gri + 0.5 * o6.w * grj);
+                const double pi_b = (brxi * ex + bryi * ey + brzi * ez) * gri;
+                const double pj_b = (o6.x * ex + o6.y * ey + o6.z * ez) * grj;
+                const double sm = mj * inv_mu0;
+                A.ax += -ex * gradp + sm * (brxi * pi_b + o6.x * pj_b);
+                A.ay += -ey * gradp + sm * (bryi * pi_b + o6.y * pj_b);
+                A.az += -ez * gradp + sm * (brzi * pi_b + o6.z * pj_b);
+            }
+            // ---- MagneticStateRates (equations.py:507-559)
+            {
+                const double pbi_ = bxi * ex + byi * ey + bzi * ez;
+                const double pbj_ = o3.x * ex + o3.y * ey + o3.z * ez;
+                const double dbx = bxi - o3.x, dby = byi - o3.y, dbz = bzi - o3.z;
+                const double pdb = dbx * ex + dby * ey + dbz * ez;
+                const double db2 = dbx * dbx + dby * dby + dbz * dbz;
+                const double pmi = mj * r2ii * gri;
+                const double pmj = mj * o5.z * grj;
+                const double brt = -(pmi * pbi_);
+                const double tx = dvx - dot * ex, ty = dvy - dot * ey, tz = dvz - dot * ez;
+                const double vsb = sqrt(tx * tx + ty * ty + tz * tz);
+                const double diss = 0.5 * (pmi + pmj) * vsb * ar;
+                const double dpsi = pmi * psichi + pmj * o4.w;
+                A.bx += brt * dvx + diss * dbx - dpsi * ex;
+                A.by += brt * dvy + diss * dby - dpsi * ey;
+                A.bz += brt * dvz + diss * dbz - dpsi * ez;
+                A.ae += -0.5 * db2 * diss * inv_mu0;
+                A.divbs += pmi * pbi_ + pmj * pbj_;
+                A.divbd += -mj * pdb * gri;
+            }
+            // ---- CullenDehnenAlphaDiagnostics loop (equations.py:319-354)
+            {
+                const double rx = mj * dwix, ry = mj * dwiy, rz = mj * dwiz;
+                const double dax = aui - o4.x, day = avi - o4.y;
+                A.vxx += dvx * rx; A.vxy += dvx * ry;
+                A.vyx += dvy * rx; A.vyy += dvy * ry;
+                A.vzx += dvz * rx; A.vzy += dvz * ry;
+                A.axx += dax * rx; A.ayy += day * ry;
+                if constexpr (DIM == 3) {
+                    const double daz = awi - o4.z;
+                    A.rxx += -dx * rx; A.rxy += -dx * ry; A.rxz += -dx * rz;
+                    A.ryy += -dy * ry; A.ryz += -dy * rz; A.rzz += -dz * rz;
+                    A.vxz += dvx * rz; A.vyz += dvy * rz; A.vzz += dvz * rz;
+                    A.axy += dax * ry; A.axz += dax * rz;
+                    A.ayx += day * rx; A.ayz += day * rz;
+                    A.azx += daz * rx; A.azy += daz * ry; A.azz += daz * rz;
+                }
+            }
+        }
+
+        // ---- epilogue ------------------------------------------------------
+        const int32_t i = perm[t];
+        double** f = F.f;
+        const double cs = me[FCS], psipred = me[FPSI];
+        // MHDAccelerations.post_loop (equations.py:279-280)
+        const double div = -A.div / rhoi;
+        // SmoothingLengthRateFromForceDivergence (equations.py:140)
+        const double ah = hi * div * (1.0 / (double)DIM);
+
+        // CullenDehnenAlphaDiagnostics.post_loop (equations.py:358-429)
+        double div_a, dvxdx, dvxdy, dvxdz, dvydx, dvydy, dvydz, dvzdx, dvzdy, dvzdz;
+        const double termnorm = omi / rhoi;
+        bool inverted = false;
+        if constexpr (DIM == 3) {
+            const double rxx = A.rxx, rxy = A.rxy, rxz = A.rxz, ryy = A.ryy, ryz = A.ryz, rzz = A.rzz;
+            const double denom = rxx * ryy * rzz + 2.0 * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
+            if (fabs(denom) > 2.2250738585072014e-308) {
+                inverted = true;
+                const double dd = 1.0 / denom;
+                const double txx = ryy * rzz - ryz * ryz;
+                const double txy = rxz * ryz - rzz * rxy;
+                const double txz = rxy * ryz - rxz * ryy;
+                const double tyy = rzz * rxx - rxz * rxz;
+                const double tyz = rxy * rxz - rxx * ryz;
+                const double tzz = rxx * ryy - rxy * rxy;
+                const double gax = (A.axx * txx + A.axy * txy + A.axz * txz) * dd;
+                const double gay = (A.ayx * txy + A.ayy * tyy + A.ayz * tyz) * dd;
+                const double gaz = (A.azx * txz + A.azy * tyz + A.azz * tzz) * dd;
+                div_a = -(gax + gay + gaz);
+                dvxdx = -(A.vxx * txx + A.vxy * txy + A.vxz * txz) * dd;
+                dvxdy = -(A.vxx * txy + A.vxy * tyy + A.vxz * tyz) * dd;
+                dvxdz = -(A.vxx * txz + A.vxy * tyz + A.vxz * tzz) * dd;
+                dvydx = -(A.vyx * txx + A.vyy * txy + A.vyz * txz) * dd;
+                dvydy = -(A.vyx * txy + A.vyy * tyy + A.vyz * tyz) * dd;
+                dvydz = -(A.vyx * txz + A.vyy * tyz + A.vyz * tzz) * dd;
+                dvzdx = -(A.vzx * txx + A.vzy * txy + A.vzz * txz) * dd;
+                dvzdy = -(A.vzx * txy + A.vzy * tyy + A.vzz * tyz) * dd;
+                dvzdz = -(A.vzx * txz + A.vzy * tyz + A.vzz * tzz) * dd;
+            }
+        }
+        if (!inverted) {
+            // 2-D: the 3x3 determinant of the reference is identically zero, so
+            // its fallback branch (equations.py:400-410) always applies.
+            div_a = -termnorm * (A.axx + A.ayy + A.azz);
+            dvxdx = -termnorm * A.vxx; dvxdy = -termnorm * A.vxy; dvxdz = -termnorm * A.vxz;
+            dvydx = -termnorm * A.vyx; dvydy = -termnorm * A.vyy; dvydz = -termnorm * A.vyz;
+            dvzdx = -termnorm * A.vzx; dvzdy = -termnorm * A.vzy; dvzdz = -termnorm * A.vzz;
+        }
+        const double divvdt = div_a - (dvxdx * dvxdx + dvydy * dvydy + dvzdz * dvzdz +
+                                       2.0 * (dvxdy * dvydx + dvxdz * dvzdx + dvydz * dvzdy));
+        const double divv = dvxdx + dvydy + dvzdz;
+        const double cvx = dvzdy - dvydz, cvy = dvxdz - dvzdx, cvz = dvydx - dvxdy;
+        const double comp = fmax(-divv, 0.0);
+        const double fc = comp * comp;
+        const double den = fc + (cvx * cvx + cvy * cvy + cvz * cvz);
+        const double xil = den > 0.0 ? fc / den : 1.0;
+        const double source = 10.0 * hi * hi * xil * fmax(-divvdt, 0.0);
+        const double alphaloc = fmax(s.alpha_min, s.alpha_max * fmax(fmin(source / (cs * cs), 1.0), 0.0));
+
+        // MagneticStateRates.post_loop (equations.py:563-567)
+        const double ch = s.chf * vwi;
+        const double apsi = -ch * A.divbd / rhoi - s.sigma * psipred * ch / hi -
+                            0.5 * s.cleaning_active * psipred * div;
+        double dtcfl = fmax(A.dtcfl, vwi);
+        dtcfl = fmax(dtcfl, ch);
+
+        // EnergyLimiter (equations.py:595-598)
+        double ae = A.ae;
+        if (dtcfl > 0.0) {
+            const double dtc = s.c_cour * hi / dtcfl;
+            if (ei + dtc * ae < 0.0 && ei > 0.0) ae /= 1.0 - dtc * ae / ei;
+        }
+        // DivBCorrection (equations.py:608-624)
+        const double b2 = bxi * bxi + byi * byi + bzi * bzi;
+        const double divb = A.divbs;
+        double frac = 0.0;
+        if (b2 > 0.0) {
+            const double beta = 2.0 * s.mu0 * pi_ / b2;
+            frac = fmax(0.0, fmin(1.0, (10.0 - beta) * 0.125));
+        }
+        const double corr = divb * frac * inv_mu0;
+        const double ax = A.ax - bxi * corr, ay = A.ay - byi * corr, az = A.az - bzi * corr;
+        // AdaptiveTimestep (equations.py:579-585)
+        dt_adapt = 1.0e29;
+        if (dtcfl > 0.0) dt_adapt = s.c_cour * hi / dtcfl;
+        const double f2 = ax * ax + ay * ay + az * az;
+        if (f2 > 0.0) dt_adapt = fmin(dt_adapt, s.c_force * sqrt(hi / sqrt(f2)));
+        bad = !isfinite(f2) || !isfinite(ae);
+
+        f[SPHMHD_F_AU][i] = ax;
+        f[SPHMHD_F_AV][i] = ay;
+        f[SPHMHD_F_AW][i] = az;
+        f[SPHMHD_F_AE][i] = ae;
+        f[SPHMHD_F_AH][i] = ah;
+        f[SPHMHD_F_DIV][i] = div;
+        f[SPHMHD_F_DIV_FOR_PSI][i] = div;
+        f[SPHMHD_F_DT_CFL][i] = dtcfl;
+        f[SPHMHD_F_DT_ADAPT][i] = dt_adapt;
+        f[SPHMHD_F_ABEVOLX][i] = A.bx;
+        f[SPHMHD_F_ABEVOLY][i] = A.by;
+        f[SPHMHD_F_ABEVOLZ][i] = A.bz;
+        f[SPHMHD_F_APSI][i] = apsi;
+        f[SPHMHD_F_DIVBSYMM][i] = rhoi * divb;
+        f[SPHMHD_F_DIVBDIFF][i] = A.divbd;
+        f[SPHMHD_F_ALPHALOC][i] = alphaloc;
+        f[SPHMHD_F_XILIMITER][i] = xil;
+        f[SPHMHD_F_DIVVDT][i] = divvdt;
+    }
+
+    // min dt_adapt per instance; NaN flag
+    unsigned long long db = dbits(dt_adapt);
+    int k0 = __shfl(ki, 0);
+    bool uniform = __all(ki == k0 || !valid);
+    bool anybad = __any(bad);
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(db, off);
+            db = o < db ? o : db;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&inst[k0].dtp_bits, db);
+            if (anybad) atomicOr(&inst[k0].status, 2);
+        }
+    } else if (valid) {
+        atomicMin(&inst[ki].dtp_bits, db);
+        if (bad) atomicOr(&inst[ki].status, 2);
+    }
+}
+
+// collect per-instance warning bits (1: density hit max_iterations, 2: NaN)
+__global__ void k_eval_end(const Inst* __restrict__ inst, int ninst, Ctl* ctl) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ninst && inst[k].status) atomicOr(&ctl->status, inst[k].status);
+}
+
+// ---------------------------------------------------------------------------
+// launchers (kernel x dim dispatch)
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+#define SPH_DISPATCH(KID, DIMV, CALL)                                                    \
+    switch (KID) {                                                                       \
+    case SPHMHD_CUBIC: if (DIMV == 2) CALL(SPHMHD_CUBIC, 2) else CALL(SPHMHD_CUBIC, 3) break; \
+    case SPHMHD_QUINTIC: if (DIMV == 2) CALL(SPHMHD_QUINTIC, 2) else CALL(SPHMHD_QUINTIC, 3) break; \
+    case SPHMHD_WENDLAND_C2: if (DIMV == 2) CALL(SPHMHD_WENDLAND_C2, 2) else CALL(SPHMHD_WENDLAND_C2, 3) break; \
+    case SPHMHD_WENDLAND_C4: if (DIMV == 2) CALL(SPHMHD_WENDLAND_C4, 2) else CALL(SPHMHD_WENDLAND_C4, 3) break; \
+    case SPHMHD_WENDLAND_C6: if (DIMV == 2) CALL(SPHMHD_WENDLAND_C6, 2) else CALL(SPHMHD_WENDLAND_C6, 3) break; \
+    case SPHMHD_GAUSSIAN: if (DIMV == 2) CALL(SPHMHD_GAUSSIAN, 2) else CALL(SPHMHD_GAUSSIAN, 3) break; \
+    default: if (DIMV == 2) CALL(SPHMHD_SUPER_GAUSSIAN, 2) else CALL(SPHMHD_SUPER_GAUSSIAN, 3) break; \
+    }
+
+void launch_density(Dev& d) {
+    if (d.n == 0) return;
+#define CALL_DENS(KK, DD)                                                                                   \
+    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields,        \
+                         (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n); }
+    SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
+#undef CALL_DENS
+}
+
+void launch_g0_control(Dev& d) {
+    hipLaunchKernelGGL(k_g0_control, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+}
+
+void launch_prep_force(Dev& d) {
+    if (d.n == 0) return;
+    hipLaunchKernelGGL(k_prep_force, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst, d.inst_of,
+                       d.fpack, d.n);
+}
+
+void launch_eval_end(Dev& d) {
+    hipLaunchKernelGGL(k_eval_end, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+}
+
+void launch_force(Dev& d) {
+    if (d.n == 0) return;
+#define CALL_FORCE(KK, DD)                                                                                  \
+    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
+                         d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n); }
+    SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)
+#undef CALL_FORCE
+}
+
+}  // namespace sph

@@ -1,0 +1,283 @@
+// sph_stage.hip -- per-particle (HBM-bound) kernels around the pair loops.
+//
+//   MHDStep.initialize / stage1 / stage2 (integrator.py:12-82) with the
+//   periodic wrap of PySPH's update_domain, the fused stage2(n) + initialize
+//   + stage1(n+1) kernel used inside multi-step runs, the solver clock
+//   (solver.py:57-61), the warm-start state sync (application.py:100-108) and
+//   initial dt (application.py:134-147), and the conserved totals.
+#include "sph_internal.h"
+
+namespace sph {
+
+#define FLD(name) F.f[SPHMHD_F_##name][i]
+
+__device__ __forceinline__ void do_initialize(const Fields& F, int64_t i) {
+    FLD(X0) = FLD(X);
+    FLD(Y0) = FLD(Y);
+    if (F.f[SPHMHD_F_Z0]) FLD(Z0) = F.f[SPHMHD_F_Z] ? FLD(Z) : 0.0;
+    FLD(U0) = FLD(U);
+    FLD(V0) = FLD(V);
+    FLD(W0) = FLD(W);
+    FLD(UPRED) = FLD(U);
+    FLD(VPRED) = FLD(V);
+    FLD(WPRED) = FLD(W);
+    FLD(E0) = FLD(E);
+    FLD(EPRED) = FLD(E);
+    FLD(H0) = FLD(H);
+    FLD(CONVERGED) = 0.0;
+    FLD(OMEGA) = 1.0;
+    FLD(ALPHA10) = FLD(ALPHA1);
+    FLD(BEVOLX0) = FLD(BEVOLX);
+    FLD(BEVOLY0) = FLD(BEVOLY);
+    FLD(BEVOLZ0) = FLD(BEVOLZ);
+    FLD(PSI0) = FLD(PSI);
+    FLD(BEVOLXPRED) = FLD(BEVOLX);
+    FLD(BEVOLYPRED) = FLD(BEVOLY);
+    FLD(BEVOLZPRED) = FLD(BEVOLZ);
+    FLD(PSIPRED) = FLD(PSI);
+}
+
+__device__ __forceinline__ double wrap1(double x, double lo, double hi, double L) {
+    if (x < lo) x += L;
+    else if (x >= hi) x -= L;
+    return x;
+}
+
+__device__ __forceinline__ void do_wrap(const Fields& F, const Inst& s, int64_t i) {
+    if (!s.periodic) return;
+    FLD(X) = wrap1(FLD(X), s.lo[0], s.hi[0], s.L[0]);
+    FLD(Y) = wrap1(FLD(Y), s.lo[1], s.hi[1], s.L[1]);
+    if (s.dim == 3) FLD(Z) = wrap1(FLD(Z), s.lo[2], s.hi[2], s.L[2]);
+}
+
+__device__ __forceinline__ void do_stage1(const Fields& F, const Inst& s, int64_t i, double dt) {
+    if (F.itype[i] != 1) return;
+    const double dtb2 = 0.5 * dt;
+    const double u = FLD(U0) + dtb2 * FLD(AU);
+    const double v = FLD(V0) + dtb2 * FLD(AV);
+    const double w = FLD(W0) + dtb2 * FLD(AW);
+    FLD(U) = u;
+    FLD(V) = v;
+    FLD(W) = w;
+    FLD(UPRED) = FLD(U0) + dt * FLD(AU);
+    FLD(VPRED) = FLD(V0) + dt * FLD(AV);
+    FLD(WPRED) = FLD(W0) + dt * FLD(AW);
+    FLD(X) = FLD(X0) + dt * u;
+    FLD(Y) = FLD(Y0) + dt * v;
+    if (s.dim == 3) FLD(Z) = FLD(Z0) + dt * w;
+    FLD(E) = FLD(E0) + dtb2 * FLD(AE);
+    FLD(EPRED) = FLD(E0) + dt * FLD(AE);
+    const double h = FLD(H0) + dt * FLD(AH);
+    FLD(H) = h;
+    const double decay = 0.1 * FLD(CS) / h;
+    const double a0 = FLD(ALPHA10), al = FLD(ALPHALOC);
+    FLD(ALPHA1) = a0 < al ? al : (a0 + dt * al * decay) / (1.0 + dt * decay);
+    FLD(BEVOLX) = FLD(BEVOLX0) + dtb2 * FLD(ABEVOLX);
+    FLD(BEVOLY) = FLD(BEVOLY0) + dtb2 * FLD(ABEVOLY);
+    FLD(BEVOLZ) = FLD(BEVOLZ0) + dtb2 * FLD(ABEVOLZ);
+    FLD(PSI) = FLD(PSI0) + dtb2 * FLD(APSI);
+    FLD(BEVOLXPRED) = FLD(BEVOLX0) + dt * FLD(ABEVOLX);
+    FLD(BEVOLYPRED) = FLD(BEVOLY0) + dt * FLD(ABEVOLY);
+    FLD(BEVOLZPRED) = FLD(BEVOLZ0) + dt * FLD(ABEVOLZ);
+    FLD(PSIPRED) = FLD(PSI0) + dt * FLD(APSI);
+}
+
+__device__ __forceinline__ void do_stage2(const Fields& F, const Inst& s, int64_t i, double dt) {
+    if (F.itype[i] != 1) return;
+    const double dtb2 = 0.5 * dt;
+    FLD(U) += dtb2 * FLD(AU);
+    FLD(V) += dtb2 * FLD(AV);
+    FLD(W) += dtb2 * FLD(AW);
+    const double e = FLD(E) + dtb2 * FLD(AE);
+    FLD(E) = e;
+    const double rho = FLD(RHO);
+    FLD(P) = s.gamma1 * rho * e;
+    const double bx = FLD(BEVOLX) + dtb2 * FLD(ABEVOLX);
+    const double by = FLD(BEVOLY) + dtb2 * FLD(ABEVOLY);
+    const double bz = FLD(BEVOLZ) + dtb2 * FLD(ABEVOLZ);
+    FLD(BEVOLX) = bx;
+    FLD(BEVOLY) = by;
+    FLD(BEVOLZ) = bz;
+    FLD(PSI) += dtb2 * FLD(APSI);
+    FLD(BX) = bx * rho;
+    FLD(BY) = by * rho;
+    FLD(BZ) = bz * rho;
+}
+
+__global__ void k_initialize(Fields F, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) do_initialize(F, i);
+}
+
+__global__ void k_stage1(Fields F, const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Inst& s = inst[inst_of[i]];
+    do_stage1(F, s, i, s.dt);
+    do_wrap(F, s, i);
+}
+
+__global__ void k_stage2(Fields F, const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Inst& s = inst[inst_of[i]];
+    do_stage2(F, s, i, s.dt);
+    do_wrap(F, s, i);
+}
+
+// stage2 of step n, then initialize and stage1 of step n+1 with the step the
+// clock kernel is about to commit
+__global__ void k_stage2_init_stage1(Fields F, const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
+                                     int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Inst& s = inst[inst_of[i]];
+    do_stage2(F, s, i, s.dt);
+    const double dt_new = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
+    do_initialize(F, i);
+    do_stage1(F, s, i, dt_new);
+    do_wrap(F, s, i);
+}
+
+__global__ void k_clock(Inst* inst, int ninst) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    Inst& s = inst[k];
+    const double t = s.t + s.dt;
+    s.dt = next_dt(s, t, s.dt, bitsd(s.dtp_bits));
+    s.t = t;
+}
+
+// application.py:134-147 followed by the solver's first _get_timestep
+// (solver.py:40): equal to min dt_adapt when every dt_cfl > 0.
+__global__ void k_warm_dt(Fields F, Inst* inst, const int32_t* __restrict__ inst_of, int64_t n, Ctl* ctl) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(F.f[SPHMHD_F_DT_CFL][i] > 0.0)) atomicOr(&ctl->status, 4);
+}
+
+__global__ void k_warm_dt_commit(Inst* inst, int ninst) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    Inst& s = inst[k];
+    const double pending = bitsd(s.dtp_bits);
+    double dt = pending;
+    if (s.t + dt > s.tf) dt = s.tf - s.t;
+    s.dt = dt;
+}
+
+__global__ void k_sync_initial(Fields F, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rho = FLD(RHO);
+    const double bx = FLD(BX) / rho, by = FLD(BY) / rho, bz = FLD(BZ) / rho;
+    FLD(BEVOLX) = bx;
+    FLD(BEVOLY) = by;
+    FLD(BEVOLZ) = bz;
+    FLD(PSI) = 0.0;
+    FLD(BEVOLXPRED) = bx;
+    FLD(BEVOLYPRED) = by;
+    FLD(BEVOLZPRED) = bz;
+    FLD(PSIPRED) = 0.0;
+    FLD(UPRED) = FLD(U);
+    FLD(VPRED) = FLD(V);
+    FLD(WPRED) = FLD(W);
+    FLD(EPRED) = FLD(E);
+}
+
+__global__ void k_count_unconverged(Fields F, int64_t n, Ctl* ctl) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = i < n && F.f[SPHMHD_F_CONVERGED][i] != 1.0;
+    unsigned long long m = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd((unsigned long long*)&ctl->count, (unsigned long long)__popcll(m));
+}
+
+__global__ void k_zero_count(Ctl* ctl) { ctl->count = 0; }
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// conserved totals: one block per instance, fixed-order tree reduction
+__global__ void k_totals(Fields F, const Inst* __restrict__ inst, double* __restrict__ out) {
+    const Inst& s = inst[blockIdx.x];
+    double acc[SPHMHD_NTOTALS];
+#pragma unroll
+    for (int q = 0; q < SPHMHD_NTOTALS; ++q) acc[q] = 0.0;
+    for (int64_t i = s.off0 + threadIdx.x; i < s.off1; i += blockDim.x) {
+        const double m = FLD(M), rho = FLD(RHO);
+        const double u = FLD(U), v = FLD(V), w = FLD(W);
+        const double bx = FLD(BX), by = FLD(BY), bz = FLD(BZ);
+        const double v2 = u * u + v * v + w * w;
+        const double b2 = bx * bx + by * by + bz * bz;
+        acc[SPHMHD_T_MASS] += m;
+        acc[SPHMHD_T_MOMX] += m * u;
+        acc[SPHMHD_T_MOMY] += m * v;
+        acc[SPHMHD_T_MOMZ] += m * w;
+        acc[SPHMHD_T_EKIN] += 0.5 * m * v2;
+        acc[SPHMHD_T_ETHERM] += m * FLD(E);
+        acc[SPHMHD_T_EMAG] += 0.5 * m * b2 / (s.mu0 * rho);
+        if (b2 > 0.0) acc[SPHMHD_T_DIVB] += (m / rho) * FLD(H) * fabs(FLD(DIVBSYMM)) / sqrt(b2);
+        acc[SPHMHD_T_VOLUME] += m / rho;
+    }
+    __shared__ double red[kBlock];
+    for (int q = 0; q < SPHMHD_NTOTALS; ++q) {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int off = kBlock / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(int64_t)blockIdx.x * SPHMHD_NTOTALS + q] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void launch_initialize(Dev& d) {
+    if (d.n) hipLaunchKernelGGL(k_initialize, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.n);
+}
+void launch_stage1(Dev& d) {
+    if (d.n) hipLaunchKernelGGL(k_stage1, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.inst_of, d.n);
+}
+void launch_stage2(Dev& d) {
+    if (d.n) hipLaunchKernelGGL(k_stage2, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.inst_of, d.n);
+}
+void launch_stage2_init_stage1(Dev& d) {
+    if (d.n)
+        hipLaunchKernelGGL(k_stage2_init_stage1, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
+                           d.inst_of, d.n);
+}
+void launch_clock(Dev& d) {
+    hipLaunchKernelGGL(k_clock, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);
+}
+void launch_sync_initial(Dev& d) {
+    if (d.n) hipLaunchKernelGGL(k_sync_initial, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.n);
+}
+void launch_copy(Dev& d, double* dst, const double* src, int64_t n) {
+    if (n) hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(kBlock), 0, d.stream, dst, src, n);
+}
+void launch_fill_i32(Dev& d, int32_t* p, int32_t v, int64_t n) {
+    if (n) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(kBlock), 0, d.stream, p, v, n);
+}
+void launch_warm_dt(Dev& d) {
+    if (d.n) hipLaunchKernelGGL(k_warm_dt, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.inst_of, d.n, d.ctl);
+    hipLaunchKernelGGL(k_warm_dt_commit, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);
+}
+void launch_count_unconverged(Dev& d) {
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, d.stream, d.ctl);
+    if (d.n) hipLaunchKernelGGL(k_count_unconverged, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.n, d.ctl);
+}
+void launch_totals(Dev& d) {
+    hipLaunchKernelGGL(k_totals, dim3(d.ninst), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.totals);
+}
+
+}  // namespace sph

@@ -1,0 +1,217 @@
+"""ctypes view of include/sphmhd.h and the loader of libsphmhd.so.
+
+The library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+ABI_VERSION = 1
+
+# sphmhd_field, in header order (names are the reference's property names,
+# particles.py:1-162)
+FIELDS = (
+    "x", "y", "z",
+    "u", "v", "w",
+    "rho", "h", "m", "cs", "p", "e",
+    "au", "av", "aw", "ae", "ah",
+    "x0", "y0", "z0",
+    "u0", "v0", "w0",
+    "upred", "vpred", "wpred",
+    "epred", "e0", "h0",
+    "div", "dt_cfl", "dt_adapt",
+    "omega", "converged",
+    "alpha1", "alpha10",
+    "Bx", "By", "Bz",
+    "Bevolx", "Bevoly", "Bevolz", "psi",
+    "Bevolxpred", "Bevolypred", "Bevolzpred", "psipred",
+    "Bxpred", "Bypred", "Bzpred",
+    "aBevolx", "aBevoly", "aBevolz", "apsi",
+    "Bevolx0", "Bevoly0", "Bevolz0", "psi0",
+    "divBsymm", "divBdiff", "div_for_psi",
+    "alphaloc", "xilimiter", "divvdt",
+)
+FIELD_INDEX = {name: k for k, name in enumerate(FIELDS)}
+NFIELDS = len(FIELDS)
+
+TOTALS = ("mass", "momx", "momy", "momz", "ekin", "etherm", "emag", "divb", "volume")
+NTOTALS = len(TOTALS)
+
+KERNELS = {
+    "cubic": 0,
+    "quintic": 1,
+    "wendland_c2": 2,
+    "wendland_c4": 3,
+    "wendland_c6": 4,
+    "gaussian": 5,
+    "super_gaussian": 6,
+}
+
+OK, EINVAL, EHIP, ECAP, ESTATE, ENOTCONV = 0, -1, -2, -3, -4, -5
+WNOCONV, WNAN = 1, 2
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("kernel", ctypes.c_int32),
+        ("density_iterate", ctypes.c_int32),
+        ("min_iterations", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("periodic", ctypes.c_int32),
+        ("adaptive_timestep", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double),
+        ("mu0", ctypes.c_double),
+        ("hfact", ctypes.c_double),
+        ("artificial_viscosity", ctypes.c_double),
+        ("artificial_thermal_conductivity", ctypes.c_double),
+        ("artificial_magnetic_dissipation", ctypes.c_double),
+        ("cleaning_speed_factor", ctypes.c_double),
+        ("cleaning_damping_factor", ctypes.c_double),
+        ("timestep_factor", ctypes.c_double),
+        ("alpha_min", ctypes.c_double),
+        ("alpha_max", ctypes.c_double),
+        ("tf", ctypes.c_double),
+    ]
+
+    def copy(self) -> "Params":
+        p = Params()
+        ctypes.pointer(p)[0] = self
+        return p
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("evals", ctypes.c_int64),
+        ("density_iterations", ctypes.c_int64),
+        ("pairs_force", ctypes.c_int64),
+        ("cells", ctypes.c_int64),
+        ("last_iterations_max", ctypes.c_int32),
+        ("max_neighbours", ctypes.c_int32),
+    ]
+
+
+def make_params(
+    *,
+    dim: int,
+    kernel: str | int,
+    lo,
+    hi,
+    gamma: float,
+    mu0: float = 1.0,
+    hfact: float,
+    artificial_viscosity: float = 1.0,
+    artificial_thermal_conductivity: float = 1.0,
+    artificial_magnetic_dissipation: float = 1.0,
+    cleaning_speed_factor: float = 1.0,
+    cleaning_damping_factor: float = 1.0,
+    timestep_factor: float = 1.0,
+    alpha_min: float = 0.0,
+    alpha_max: float = 1.0,
+    tf: float = 1.0e30,
+    density: str = "iterate",
+    periodic: bool = True,
+    adaptive_timestep: bool = True,
+    max_iterations: int = 250,
+) -> Params:
+    """Parameters of one instance; defaults follow scheme.py:28,71."""
+    iterate = density == "iterate"
+    p = Params()
+    p.dim = int(dim)
+    p.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    p.density_iterate = 1 if iterate else 0
+    p.min_iterations = 2 if iterate else 0
+    p.max_iterations = int(max_iterations) if iterate else 1
+    p.periodic = 1 if periodic else 0
+    p.adaptive_timestep = 1 if adaptive_timestep else 0
+    for a in range(3):
+        p.lo[a] = float(lo[a]) if a < len(lo) else 0.0
+        p.hi[a] = float(hi[a]) if a < len(hi) else 0.0
+    p.gamma = gamma
+    p.mu0 = mu0
+    p.hfact = hfact
+    p.artificial_viscosity = artificial_viscosity
+    p.artificial_thermal_conductivity = artificial_thermal_conductivity
+    p.artificial_magnetic_dissipation = artificial_magnetic_dissipation
+    p.cleaning_speed_factor = cleaning_speed_factor
+    p.cleaning_damping_factor = cleaning_damping_factor
+    p.timestep_factor = timestep_factor
+    p.alpha_min = alpha_min
+    p.alpha_max = alpha_max
+    p.tf = tf
+    return p
+
+
+LIB_PATH = Path(__file__).resolve().parent / "libsphmhd.so"
+_lib = None
+
+
+class SPHMHDError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"sphmhd error {code}: {message}")
+        self.code = code
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("SPHMHD_LIBRARY", LIB_PATH))
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libsphmhd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build it with `python pysph-mhd_amd/build.py` "
+            "(there is no non-HIP fallback)"
+        )
+    lib = ctypes.CDLL(str(path))
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    dp = ctypes.POINTER(ctypes.c_double)
+    sigs = {
+        "sphmhd_abi_version": ([], ctypes.c_int),
+        "sphmhd_last_error": ([], ctypes.c_char_p),
+        "sphmhd_create": ([ctypes.POINTER(Params), i32, ctypes.POINTER(ctypes.c_int64), vp, ctypes.POINTER(vp)], ctypes.c_int),
+        "sphmhd_destroy": ([vp], ctypes.c_int),
+        "sphmhd_bind": ([vp, ctypes.POINTER(ctypes.c_void_p), vp], ctypes.c_int),
+        "sphmhd_eval": ([vp], ctypes.c_int),
+        "sphmhd_pec_initialize": ([vp], ctypes.c_int),
+        "sphmhd_pec_stage1": ([vp], ctypes.c_int),
+        "sphmhd_pec_stage2": ([vp], ctypes.c_int),
+        "sphmhd_advance_clock": ([vp], ctypes.c_int),
+        "sphmhd_step": ([vp, i64], ctypes.c_int),
+        "sphmhd_sync_initial_state": ([vp], ctypes.c_int),
+        "sphmhd_warm_start": ([vp], ctypes.c_int),
+        "sphmhd_get_clock": ([vp, dp, dp], ctypes.c_int),
+        "sphmhd_set_clock": ([vp, dp, dp], ctypes.c_int),
+        "sphmhd_get_pending_dt": ([vp, dp], ctypes.c_int),
+        "sphmhd_totals": ([vp, dp], ctypes.c_int),
+        "sphmhd_cells": ([vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "sphmhd_neighbours": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "sphmhd_get_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
+        "sphmhd_synchronize": ([vp], ctypes.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.sphmhd_abi_version() != ABI_VERSION:
+        raise RuntimeError("libsphmhd.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, *, allow_warnings: bool = True) -> int:
+    """Raise on negative status codes; warnings (>0) are returned."""
+    if rc < 0 or (rc > 0 and not allow_warnings):
+        msg = load_library().sphmhd_last_error()
+        raise SPHMHDError(rc, msg.decode() if msg else "")
+    return rc

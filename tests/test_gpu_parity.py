@@ -1,0 +1,219 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Standards (from the project's contract for this path):
+  * cell keys, the sort permutation and neighbour sets: bit-exact;
+  * per-step fields: relative L2 <= 1e-10 in FP64;
+  * conserved totals after 100 steps: <= 1e-8 relative.
+The HIP kernels fuse the pair loops and hoist per-particle terms, so FP64
+sums differ from the oracle's group-by-group order only by rounding.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import INIT, RATES, STATE, compare, init_fields
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-10
+
+
+def _pair(setup, threads=None):
+    from pyoracle import OracleSystem
+    from sphmhd.device import DeviceSystem
+
+    dev = DeviceSystem(setup.params, setup.offsets)
+    orc = OracleSystem(setup.params, setup.offsets, threads=threads)
+    f = init_fields(setup)
+    dev.set_fields(f)
+    orc.set_fields(f)
+    return dev, orc
+
+
+def _ot(nx, jitter=True, **kw):
+    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+
+    s = orszag_tang_2d(nx, **kw)
+    return jittered(s, 0.1, seed=nx) if jitter else s
+
+
+def _csr_equal(a, b):
+    oa, ia = a
+    ob, ib = b
+    assert np.array_equal(oa, ob), "neighbour counts differ"
+    assert np.array_equal(ia, ib), "neighbour indices differ"
+
+
+def test_cells_bit_exact(gpu):
+    s = _ot(48)
+    dev, orc = _pair(s)
+    kd, pd = dev.cells()
+    ko, po = orc.cells()
+    assert np.array_equal(kd, ko)
+    assert np.array_equal(pd, po)
+
+
+@pytest.mark.parametrize("which", ["ot2d", "blast3d", "ensemble"])
+def test_neighbours_bit_exact(gpu, which):
+    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered
+
+    if which == "ot2d":
+        s = _ot(40)
+    elif which == "blast3d":
+        s = jittered(blast_3d(14), 0.1, seed=3)
+    else:
+        s = jittered(ensemble_ot(3, sizes=[24, 31, 40]), 0.1, seed=5)
+    dev, orc = _pair(s)
+    _csr_equal(dev.neighbours(), orc.neighbours())
+
+
+def test_single_evaluation(gpu):
+    s = _ot(64)
+    dev, orc = _pair(s)
+    for sysm in (dev, orc):
+        sysm.set_fields({"h0": s.fields["h"]})
+        sysm.sync_initial_state()
+        sysm.eval()
+    compare(dev, orc, STATE + RATES + ("converged", "xilimiter", "divvdt"), TOL, "eval")
+    assert np.allclose(dev.pending_dt(), orc.pending_dt(), rtol=1e-12, atol=0)
+
+
+def test_warm_start(gpu):
+    s = _ot(64)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    compare(dev, orc, STATE + RATES, TOL, "warm start")
+    td, dtd = dev.clock()
+    to, dto = orc.clock()
+    assert abs(dtd[0] - dto[0]) <= 1e-12 * dto[0]
+
+
+def test_steps_lockstep(gpu):
+    s = _ot(64)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    for k in range(10):
+        dev.step(1)
+        orc.step(1)
+        compare(dev, orc, STATE + RATES, TOL, f"step {k + 1}")
+
+
+def test_fused_multistep_equals_single_steps(gpu):
+    """sphmhd_step(n) (fused stage kernels) against n calls of sphmhd_step(1)."""
+    from sphmhd.device import DeviceSystem
+
+    s = _ot(40)
+    a = DeviceSystem(s.params, s.offsets)
+    b = DeviceSystem(s.params, s.offsets)
+    for d in (a, b):
+        d.set_fields(init_fields(s))
+        d.warm_start()
+    a.step(6)
+    for _ in range(6):
+        b.step(1)
+    for nm in STATE + RATES:
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+    assert np.array_equal(a.clock()[0], b.clock()[0])
+
+
+@pytest.mark.parametrize("kernel", ["cubic", "wendland_c2", "wendland_c4", "wendland_c6", "gaussian",
+                                    "super_gaussian"])
+def test_kernels_steps(gpu, kernel):
+    hf = 1.2 if kernel in ("cubic", "quintic", "gaussian", "super_gaussian") else 1.6
+    s = _ot(32, kernel=kernel, hfact=hf)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    dev.step(3)
+    orc.step(3)
+    compare(dev, orc, STATE + RATES, TOL, kernel)
+
+
+def test_blast_3d_steps(gpu):
+    from sphmhd.cases.baseline import blast_3d
+
+    s = blast_3d(16)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    compare(dev, orc, STATE + RATES, TOL, "3d warm")
+    for k in range(3):
+        dev.step(1)
+        orc.step(1)
+        compare(dev, orc, STATE + RATES + ("z",), TOL, f"3d step {k + 1}")
+
+
+def test_rotor_variable_mass(gpu):
+    from sphmhd.cases.baseline import rotor_2d
+
+    s = rotor_2d(48)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    for k in range(4):
+        dev.step(1)
+        orc.step(1)
+    compare(dev, orc, STATE + RATES, TOL, "rotor")
+
+
+def test_frozen_particles(gpu):
+    """itype != 1 particles are not integrated (integrator.py:39,69)."""
+    s = _ot(32)
+    itype = np.ones(s.n, dtype=np.int32)
+    itype[::7] = 3
+    s.itype = itype
+    dev, orc = _pair(s)
+    x0 = s.fields["x"][itype == 3]
+    dev.warm_start()
+    orc.warm_start()
+    dev.step(3)
+    orc.step(3)
+    compare(dev, orc, STATE + RATES, TOL, "frozen")
+    assert np.array_equal(dev.get("x")[itype == 3], x0)
+
+
+def test_ensemble_per_instance(gpu):
+    from sphmhd.cases.baseline import ensemble_ot
+
+    s = ensemble_ot(5, sizes=[24, 40, 32, 48, 28])
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    for k in range(4):
+        dev.step(1)
+        orc.step(1)
+    compare(dev, orc, STATE + RATES, TOL, "ensemble")
+    td, dtd = dev.clock()
+    to, dto = orc.clock()
+    assert np.allclose(td, to, rtol=1e-12, atol=0)
+    assert len(set(np.round(dto, 12))) > 1  # instances really take different steps
+
+
+def test_orszag_tang_128_100_steps(gpu):
+    """The CPU-runnable workload: 128^2 Orszag-Tang, 100 PEC steps, compared
+    every 10 steps; conserved totals at step 100 to 1e-8."""
+    from sphmhd.cases.baseline import orszag_tang_2d
+
+    s = orszag_tang_2d(128)
+    dev, orc = _pair(s, threads=16)
+    dev.warm_start()
+    orc.warm_start()
+    t_d0 = dev.totals()[0]
+    t_o0 = orc.totals()[0]
+    assert rel_l2(t_d0, t_o0) <= 1e-12
+    for k in range(10):
+        dev.step(10)
+        orc.step(10)
+        compare(dev, orc, STATE, TOL, f"C1 step {10 * (k + 1)}")
+    td = dev.totals()[0]
+    to = orc.totals()[0]
+    for q, name in enumerate(("mass", "momx", "momy", "momz", "ekin", "etherm", "emag", "divb", "volume")):
+        scale = max(abs(to[q]), abs(to[0]) * 1e-3 if name.startswith("mom") else 0.0, 1e-300)
+        assert abs(td[q] - to[q]) <= 1e-8 * scale, (name, td[q], to[q])
+    e_d = td[4] + td[5] + td[6]
+    e_o = to[4] + to[5] + to[6]
+    assert abs(e_d - e_o) <= 1e-8 * abs(e_o)
