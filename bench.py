@@ -1,0 +1,237 @@
+"""Throughput of the SPH-MHD step on one MI355X.
+
+One step = one full predictor-evaluate-corrector step (integrator.py:12-82
+around the group plan of scheme.py:70-81, all density iterations included)
+over every particle of the workload.  Inputs are device-resident before the
+timed region; the timed region is exactly --steps steps between two stream
+synchronisations, after the warm start (application.py:110-147) and
+--warmup untimed steps.
+
+Headline workload: the 2-D MHD rotor on a 512^2 lattice, FP64 (the
+single-GPU configuration the metric is quoted on).  The JSON line also
+carries the other single-GPU workloads (the 1M-particle Orszag-Tang vortex,
+the 128^3 blast and the 64-instance uneven ensemble), the roofline of the
+dominant kernel (the fused force pass) and the CPU oracle timed on a bounded
+sample of the headline workload on this host.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "pysph-mhd_amd"))
+
+METRIC = "particle-steps/s & HBM roofline % (plain vs batched vs fwd+adjoint), 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic bytes of one force launch per particle (compulsory traffic: each
+# live input read once, each output written once, FP64): reads x, y, h, m,
+# rho, p, cs, epred, upred(3), Bpred(3), psipred, omega, alpha1, a_prev(3)
+# (20 fields, 160 B in 2-D; 21 in 3-D) and writes a(3), ae, ah, aBevol(3),
+# apsi, alphaloc, dt_adapt (11 fields, 88 B).
+FORCE_BYTES_PER_PARTICLE = {2: 248.0, 3: 256.0}
+
+
+def _flops_per_pair(dim: int) -> float:
+    """Algorithmic FP64 flops of one neighbour entry of the fused force pass,
+    counting add/sub/mul/div/sqrt as 1 and fma as 2, from the operation
+    counts of the reference bodies as fused (kernel gradients for h_i and
+    h_j, MHDAccelerations, MagneticStressAccelerations, MagneticStateRates,
+    Cullen-Dehnen sums)."""
+    return 330.0 if dim == 2 else 390.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="rotor512",
+                    choices=["rotor512", "ot1m", "blast128", "ensemble64", "ot128"])
+    ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def make_setup(name: str):
+    from sphmhd.cases import baseline as B
+
+    if name == "rotor512":
+        return B.rotor_2d(512), "rotor_2d 512x512 (262,144 particles), cubic kernel, h=1.2dx"
+    if name == "ot1m":
+        return B.orszag_tang_2d(1024), "orszag_tang_2d 1024x1024 (1,048,576 particles), quintic, h=1.2dx"
+    if name == "ot128":
+        return B.orszag_tang_2d(128), "orszag_tang_2d 128x128 (16,384 particles), quintic, h=1.2dx"
+    if name == "blast128":
+        return B.blast_3d(128), "blast_3d 128^3 (2,097,152 particles), quintic, h=0.81dx"
+    if name == "ensemble64":
+        s = B.ensemble_ot(64)
+        return s, f"ensemble_ot 64 instances, N_k = n_k^2 with n_k~U{{64..256}} seed 0 ({s.n:,} particles)"
+    raise ValueError(name)
+
+
+def run_gpu(name: str, steps: int, warmup: int):
+    import torch
+
+    from sphmhd.device import DeviceSystem
+
+    setup, desc = make_setup(name)
+    dev = DeviceSystem(setup.params, setup.offsets, device="cuda:0")
+    init = {k: v for k, v in setup.fields.items()}
+    dev.set_fields(init)
+    dev.warm_start()
+    if warmup:
+        dev.step(warmup)
+    dev.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps)
+    dev.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = dev.timing()
+    st = dev.stats()
+    res = {
+        "workload": name,
+        "desc": desc,
+        "particles": setup.n,
+        "instances": len(setup.params),
+        "dim": int(setup.params[0].dim),
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "particle_steps_per_s": setup.n * steps / wall,
+        "timing": tm,
+        "stats": st,
+    }
+    dev.close()
+    del dev
+    torch.cuda.empty_cache()
+    return res
+
+
+def roofline(res):
+    tm = res["timing"]
+    n = res["particles"]
+    dim = res["dim"]
+    launches = max(1, tm["force_launches"])
+    t_force = tm["force_ms"] / launches * 1e-3
+    bytes_alg = FORCE_BYTES_PER_PARTICLE[dim] * n
+    pairs = res["stats"]["pairs_force"]
+    flops = _flops_per_pair(dim) * pairs
+    gbs = bytes_alg / t_force / 1e9
+    tflops = flops / t_force / 1e12
+    return {
+        "bound": "hbm",
+        "kernel": "k_force (fused G3+G5+G7+G8 pair loop, G3/G5/G6/G8/G9 epilogue)",
+        "achieved": round(gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "avg_launch_ms": round(tm["force_ms"] / launches, 5),
+        "launches": tm["force_launches"],
+        "bytes_per_launch": bytes_alg,
+        "fp64_vector": {
+            "achieved": round(tflops, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+            "flops_per_pair": _flops_per_pair(dim),
+            "pairs_per_launch": pairs,
+        },
+        "binding_frac": round(max(gbs / HBM_PEAK_GBS, tflops / FP64_PEAK_TFLOPS), 4),
+    }
+
+
+def cpu_baseline(name: str, steps: int):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyoracle import OracleSystem
+
+    cores = len(os.sched_getaffinity(0))
+    threads = min(cores, 16)
+    setup, desc = make_setup(name)
+    orc = OracleSystem(setup.params, setup.offsets, threads=threads)
+    orc.set_fields({k: v for k, v in setup.fields.items()})
+    orc.warm_start()
+    t0 = time.perf_counter()
+    orc.step(steps)
+    wall = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    orc.close()
+    return {
+        "value": round(setup.n * steps / wall, 1),
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{desc}: warm start untimed, then {steps} PEC steps timed ({wall:.1f} s); "
+                  f"FP64 C oracle, OpenMP {threads} threads on {model or platform.processor()}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is a design (see DESIGN.md), "
+                                   "so --gpus must be 1"}))
+        sys.exit(2)
+    head = run_gpu(args.workload, args.steps, args.warmup)
+    extras = {}
+    if not args.no_extras:
+        for name, steps in (("ot1m", 20), ("blast128", 5), ("ensemble64", 20)):
+            if name == args.workload:
+                continue
+            r = run_gpu(name, steps, 2)
+            extras[name] = {
+                "desc": r["desc"],
+                "particles": r["particles"],
+                "instances": r["instances"],
+                "steps": steps,
+                "particle_steps_per_s": round(r["particle_steps_per_s"], 1),
+                "ms_per_step": round(r["ms_per_step"], 4),
+                "pair_interactions_per_s": round(r["stats"]["pairs_force"] * steps / r["wall_s"], 1),
+                "roofline": roofline(r),
+            }
+    line = {
+        "metric": METRIC,
+        "value": round(head["particle_steps_per_s"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (lattice initial conditions of the named workload)",
+        "config": {"workload": head["desc"], "particles": head["particles"], "steps_timed": args.steps,
+                   "step": "PEC step incl. all density iterations", "parallelism": "single GPU"},
+        "pair_interactions_per_s": round(head["stats"]["pairs_force"] * args.steps / head["wall_s"], 1),
+        "timing_ms": {k: round(v, 4) if isinstance(v, float) else v for k, v in head["timing"].items()},
+        "roofline": roofline(head),
+        "extra_workloads": extras,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,21 @@ typedef struct sphmhd_stats {
     int32_t max_neighbours;      /* largest neighbour count in the last force pass */
 } sphmhd_stats;
 
+/* Device time of the hot kernels, measured with HIP events recorded on the
+ * context's stream around each launch (enabled by sphmhd_set_timing). */
+typedef struct sphmhd_timing {
+    double force_ms;          /* fused force pass (G3+G5..G9) */
+    double density_ms;        /* density + Newton passes (G0) */
+    double nnps_ms;           /* cell keys + radix sort + cell table */
+    double lists_ms;          /* G1/G2/G4 finalize + force pack + neighbour lists */
+    double stage_ms;          /* integrator stage kernels */
+    int64_t force_launches;
+    int64_t density_launches;
+    int64_t nnps_builds;
+    int64_t list_builds;
+    int64_t stage_launches;
+} sphmhd_timing;
+
 typedef struct sphmhd_ctx sphmhd_ctx;
 
 int sphmhd_abi_version(void);
@@ -200,6 +215,10 @@ int sphmhd_neighbours(sphmhd_ctx* ctx, int64_t* offsets, int32_t* idx,
                       int64_t cap, int64_t* total);
 
 int sphmhd_get_stats(sphmhd_ctx* ctx, sphmhd_stats* out);
+
+/* Enable (1) or disable (0) event timing; enabling resets the sums. */
+int sphmhd_set_timing(sphmhd_ctx* ctx, int32_t enable);
+int sphmhd_get_timing(sphmhd_ctx* ctx, sphmhd_timing* out);
 
 /* Block until the context's stream is idle. */
 int sphmhd_synchronize(sphmhd_ctx* ctx);

@@ -47,8 +47,19 @@ int dalloc(T** p, int64_t count) {
 
 }  // namespace
 
+// HIP-event timer of one phase: events are recorded around launches and
+// read after the next synchronisation of the stream.
+struct PhaseTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool pending = false;
+    double ms = 0.0;
+    int64_t count = 0;
+};
+
 struct sphmhd_ctx {
     Dev d{};
+    bool timing = false;
+    PhaseTimer tm[5];  // force, density, nnps, lists, stage
     std::vector<Inst> hinst;
     std::vector<int64_t> offsets;
     bool bound = false;
@@ -58,9 +69,37 @@ struct sphmhd_ctx {
     int64_t tiles = 0;
 };
 
+enum { T_FORCE = 0, T_DENSITY, T_NNPS, T_LISTS, T_STAGE };
+
+static void tbegin(sphmhd_ctx* c, int k) {
+    if (c->timing && !c->tm[k].pending) (void)hipEventRecord(c->tm[k].a, c->d.stream);
+}
+
+static void tend(sphmhd_ctx* c, int k) {
+    if (c->timing && !c->tm[k].pending) {
+        (void)hipEventRecord(c->tm[k].b, c->d.stream);
+        c->tm[k].pending = true;
+    }
+}
+
+// fold finished event pairs into the sums (call after a stream sync)
+static void tcollect(sphmhd_ctx* c) {
+    if (!c->timing) return;
+    for (PhaseTimer& t : c->tm) {
+        if (!t.pending) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            t.ms += ms;
+            t.count += 1;
+        }
+        t.pending = false;
+    }
+}
+
 static int read_ctl(sphmhd_ctx* c) {
     HIPCHK(hipMemcpyAsync(c->hctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->d.stream));
     HIPCHK(hipStreamSynchronize(c->d.stream));
+    tcollect(c);
     return 0;
 }
 
@@ -93,6 +132,7 @@ static int grow_nb(sphmhd_ctx* c, int need) {
 static int build_lists(sphmhd_ctx* c) {
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_nb_build(c->d);
+        tend(c, T_LISTS);
         int rc = read_ctl(c);
         if (rc) return rc;
         if (c->hctl->max_nb <= c->d.nb_stride) return 0;
@@ -114,12 +154,16 @@ static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions) {
 
 static int do_eval(sphmhd_ctx* c) {
     Dev& d = c->d;
+    tbegin(c, T_NNPS);
     int rc = nnps_rebuild(c, true);
     if (rc) return rc;
     launch_pack_density(d);
+    tend(c, T_NNPS);
     int passes = 0;
     for (;;) {
+        tbegin(c, T_DENSITY);
         launch_density(d);
+        tend(c, T_DENSITY);
         launch_g0_control(d);
         HIPCHK(hipGetLastError());
         rc = read_ctl(c);
@@ -130,13 +174,16 @@ static int do_eval(sphmhd_ctx* c) {
     }
     c->stats.density_iterations += passes;
     c->stats.last_iterations_max = c->hctl->iters_max;
+    tbegin(c, T_LISTS);
     launch_prep_force(d);
     rc = build_lists(c);
     if (rc) return rc;
     c->stats.max_neighbours = c->hctl->max_nb;
     c->stats.pairs_force = c->hctl->pairs;
     c->stats.cells = c->hctl->total_cells;
+    tbegin(c, T_FORCE);
     launch_force(d);
+    tend(c, T_FORCE);
     launch_eval_end(d);
     HIPCHK(hipGetLastError());
     rc = read_ctl(c);
@@ -302,6 +349,10 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);
+    for (PhaseTimer& t : c->tm) {
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
+    }
     delete c;
     return 0;
 }
@@ -367,6 +418,7 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
     int worst = 0;
     try {
         for (int64_t s = 0; s < nsteps; ++s) {
+            tbegin(c, T_STAGE);
             if (s == 0) {
                 launch_initialize(c->d);
                 launch_stage1(c->d);
@@ -374,6 +426,7 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
                 launch_stage2_init_stage1(c->d);
                 launch_clock(c->d);
             }
+            tend(c, T_STAGE);
             HIPCHK(hipGetLastError());
             rc = do_eval(c);
             if (rc < 0) return rc;
@@ -534,6 +587,40 @@ int sphmhd_get_stats(sphmhd_ctx* c, sphmhd_stats* out) {
 int sphmhd_synchronize(sphmhd_ctx* c) {
     if (!c) return fail(SPHMHD_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->d.stream));
+    tcollect(c);
+    return 0;
+}
+
+int sphmhd_set_timing(sphmhd_ctx* c, int32_t enable) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    for (PhaseTimer& t : c->tm) {
+        if (enable && !t.a) {
+            HIPCHK(hipEventCreate(&t.a));
+            HIPCHK(hipEventCreate(&t.b));
+        }
+        t.ms = 0.0;
+        t.count = 0;
+        t.pending = false;
+    }
+    c->timing = enable != 0;
+    return 0;
+}
+
+int sphmhd_get_timing(sphmhd_ctx* c, sphmhd_timing* out) {
+    if (!c || !out) return fail(SPHMHD_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    tcollect(c);
+    out->force_ms = c->tm[T_FORCE].ms;
+    out->density_ms = c->tm[T_DENSITY].ms;
+    out->nnps_ms = c->tm[T_NNPS].ms;
+    out->lists_ms = c->tm[T_LISTS].ms;
+    out->stage_ms = c->tm[T_STAGE].ms;
+    out->force_launches = c->tm[T_FORCE].count;
+    out->density_launches = c->tm[T_DENSITY].count;
+    out->nnps_builds = c->tm[T_NNPS].count;
+    out->list_builds = c->tm[T_LISTS].count;
+    out->stage_launches = c->tm[T_STAGE].count;
     return 0;
 }
 

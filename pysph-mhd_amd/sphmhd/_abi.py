@@ -97,6 +97,21 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("force_ms", ctypes.c_double),
+        ("density_ms", ctypes.c_double),
+        ("nnps_ms", ctypes.c_double),
+        ("lists_ms", ctypes.c_double),
+        ("stage_ms", ctypes.c_double),
+        ("force_launches", ctypes.c_int64),
+        ("density_launches", ctypes.c_int64),
+        ("nnps_builds", ctypes.c_int64),
+        ("list_builds", ctypes.c_int64),
+        ("stage_launches", ctypes.c_int64),
+    ]
+
+
 def make_params(
     *,
     dim: int,
@@ -198,6 +213,8 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_neighbours": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "sphmhd_get_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
         "sphmhd_synchronize": ([vp], ctypes.c_int),
+        "sphmhd_set_timing": ([vp, i32], ctypes.c_int),
+        "sphmhd_get_timing": ([vp, ctypes.POINTER(Timing)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

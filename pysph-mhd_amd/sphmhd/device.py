@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from ._abi import FIELD_INDEX, FIELDS, NFIELDS, NTOTALS, TOTALS, Params, Stats, check, load_library
+from ._abi import FIELD_INDEX, FIELDS, NFIELDS, NTOTALS, TOTALS, Params, Stats, Timing, check, load_library
 
 
 def _as_params_array(params: Params | Sequence[Params]):
@@ -177,3 +177,12 @@ class DeviceSystem:
         s = Stats()
         check(self.lib.sphmhd_get_stats(self._ctx, ctypes.byref(s)))
         return {name: getattr(s, name) for name, _ in Stats._fields_}
+
+    def set_timing(self, enable: bool = True) -> None:
+        """Enable HIP-event timing of the hot kernels (resets the sums)."""
+        check(self.lib.sphmhd_set_timing(self._ctx, 1 if enable else 0))
+
+    def timing(self) -> dict[str, float]:
+        t = Timing()
+        check(self.lib.sphmhd_get_timing(self._ctx, ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in Timing._fields_}

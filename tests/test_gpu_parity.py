@@ -6,6 +6,13 @@ Standards (from the project's contract for this path):
   * conserved totals after 100 steps: <= 1e-8 relative.
 The HIP kernels fuse the pair loops and hoist per-particle terms, so FP64
 sums differ from the oracle's group-by-group order only by rounding.
+
+Small exact lattices are avoided: there the reference algorithm itself is
+ill-conditioned (the Cullen-Dehnen limiter, equations.py:418-422, is 0/0
+where lattice symmetry zeroes both div v and curl v, so a one-ulp change of
+the input moves alpha by ~1e-2 within a few steps in the oracle alone).
+Small cases use lattices jittered by 5-10% of dx; the 128^2 workload is an
+exact lattice and is well-conditioned.
 """
 from __future__ import annotations
 
@@ -134,9 +141,9 @@ def test_kernels_steps(gpu, kernel):
 
 
 def test_blast_3d_steps(gpu):
-    from sphmhd.cases.baseline import blast_3d
+    from sphmhd.cases.baseline import blast_3d, jittered
 
-    s = blast_3d(16)
+    s = jittered(blast_3d(16), 0.05, seed=2)
     dev, orc = _pair(s)
     dev.warm_start()
     orc.warm_start()
@@ -148,9 +155,9 @@ def test_blast_3d_steps(gpu):
 
 
 def test_rotor_variable_mass(gpu):
-    from sphmhd.cases.baseline import rotor_2d
+    from sphmhd.cases.baseline import jittered, rotor_2d
 
-    s = rotor_2d(48)
+    s = jittered(rotor_2d(48), 0.1, seed=4)
     dev, orc = _pair(s)
     dev.warm_start()
     orc.warm_start()
@@ -179,7 +186,9 @@ def test_frozen_particles(gpu):
 def test_ensemble_per_instance(gpu):
     from sphmhd.cases.baseline import ensemble_ot
 
-    s = ensemble_ot(5, sizes=[24, 40, 32, 48, 28])
+    from sphmhd.cases.baseline import jittered
+
+    s = jittered(ensemble_ot(5, sizes=[24, 40, 32, 48, 28]), 0.1, seed=11)
     dev, orc = _pair(s)
     dev.warm_start()
     orc.warm_start()
