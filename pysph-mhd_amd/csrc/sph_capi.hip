@@ -64,6 +64,8 @@ struct sphmhd_ctx {
     std::vector<int64_t> offsets;
     bool bound = false;
     int max_iter_all = 1;
+    int min_iter_all = 1;
+    int spec_passes = 2;  // density passes launched before the first wait
     Ctl* hctl = nullptr;  // pinned readback
     sphmhd_stats stats{};
     int64_t tiles = 0;
@@ -128,11 +130,11 @@ static int grow_nb(sphmhd_ctx* c, int need) {
     return 0;
 }
 
-// neighbour lists for the current pack (x, h); grows the slots if needed
+// neighbour lists for the current {x,y,z,h} pack; grows the slots and
+// rebuilds when a particle has more neighbours than slots
 static int build_lists(sphmhd_ctx* c) {
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_nb_build(c->d);
-        tend(c, T_LISTS);
         int rc = read_ctl(c);
         if (rc) return rc;
         if (c->hctl->max_nb <= c->d.nb_stride) return 0;
@@ -152,6 +154,12 @@ static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions) {
     return 0;
 }
 
+// One evaluation.  The density passes are launched speculatively (as many as
+// the previous evaluation needed, at least min_iterations): a pass launched
+// after an instance has finished is a no-op for it, so the host only waits
+// once for the iteration flags.  The list overflow check is read together
+// with the end-of-evaluation status; an overflowing list is grown and only
+// the list build and force pass are re-run (their inputs are unchanged).
 static int do_eval(sphmhd_ctx* c) {
     Dev& d = c->d;
     tbegin(c, T_NNPS);
@@ -160,34 +168,48 @@ static int do_eval(sphmhd_ctx* c) {
     launch_pack_density(d);
     tend(c, T_NNPS);
     int passes = 0;
+    int spec = std::max(c->spec_passes, 1);
     for (;;) {
-        tbegin(c, T_DENSITY);
-        launch_density(d);
-        tend(c, T_DENSITY);
-        launch_g0_control(d);
+        for (int q = 0; q < spec; ++q) {
+            d.pass = passes;
+            if (q == 0) tbegin(c, T_DENSITY);
+            launch_density(d);
+            if (q == spec - 1) tend(c, T_DENSITY);
+            launch_g0_control(d);
+            ++passes;
+        }
         HIPCHK(hipGetLastError());
         rc = read_ctl(c);
         if (rc) return rc;
-        ++passes;
         if (c->hctl->all_done) break;
         if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
+        spec = 1;
     }
-    c->stats.density_iterations += passes;
+    c->spec_passes = std::max(c->hctl->iters_max, c->min_iter_all);
+    c->stats.density_iterations += c->hctl->iters_max;
     c->stats.last_iterations_max = c->hctl->iters_max;
-    tbegin(c, T_LISTS);
-    launch_prep_force(d);
-    rc = build_lists(c);
-    if (rc) return rc;
+    for (int attempt = 0;; ++attempt) {
+        tbegin(c, T_LISTS);
+        if (attempt == 0) launch_prep_force(d);
+        launch_nb_build(d);
+        tend(c, T_LISTS);
+        tbegin(c, T_FORCE);
+        launch_force(d);
+        tend(c, T_FORCE);
+        launch_eval_end(d);
+        HIPCHK(hipGetLastError());
+        rc = read_ctl(c);
+        if (rc) return rc;
+        if (!c->hctl->overflow) break;
+        if (attempt > 0) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
+        rc = grow_nb(c, c->hctl->max_nb);
+        if (rc) return rc;
+        launch_reset_nb_stats(d);
+        launch_reset_force_outputs(d);
+    }
     c->stats.max_neighbours = c->hctl->max_nb;
     c->stats.pairs_force = c->hctl->pairs;
     c->stats.cells = c->hctl->total_cells;
-    tbegin(c, T_FORCE);
-    launch_force(d);
-    tend(c, T_FORCE);
-    launch_eval_end(d);
-    HIPCHK(hipGetLastError());
-    rc = read_ctl(c);
-    if (rc) return rc;
     c->stats.evals += 1;
     int st = c->hctl->status;
     if (st & 2) return SPHMHD_WNAN;
@@ -257,7 +279,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         while (bits < 32 && ((int64_t)1 << bits) < cap) ++bits;
         d.key_bits = bits;
         d.nb_stride = p0.dim == 2 ? 64 : 96;
-        c->tiles = (n + 1023) / 1024;
+        c->tiles = (n + kRadixTile - 1) / kRadixTile;
 
         c->hinst.resize(n_instances);
         for (int k = 0; k < n_instances; ++k) {
@@ -297,6 +319,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
             s.c_force = p.timestep_factor * 0.25;
             s.dtp_bits = 0x7FF0000000000000ull;
             c->max_iter_all = std::max(c->max_iter_all, p.density_iterate ? p.max_iterations : 1);
+            c->min_iter_all = std::max(c->min_iter_all, p.density_iterate ? p.min_iterations : 1);
         }
 
         std::vector<int32_t> inst_of(n);
@@ -323,6 +346,8 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.fpack, (int64_t)FREC * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nb, (int64_t)d.nb_stride * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nbcnt, n))) return cleanup(rc);
+        if ((rc = dalloc(&d.act[0], n))) return cleanup(rc);
+        if ((rc = dalloc(&d.act[1], n))) return cleanup(rc);
         if ((rc = dalloc(&d.scratch3, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
         hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocDefault);
@@ -345,7 +370,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     Dev& d = c->d;
     (void)hipStreamSynchronize(d.stream);
     void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
-                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.totals};
+                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.totals, d.act[0], d.act[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);

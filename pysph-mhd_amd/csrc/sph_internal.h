@@ -23,6 +23,7 @@ namespace sph {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kRadixTile = 256;  // keys per wave tile of the radix sort
 
 // Force-pack record slots (FP64 each, 32 per particle).
 enum FSlot {
@@ -56,7 +57,8 @@ struct Inst {
     double t, dt;
     // reduction targets (bit patterns of positive doubles order like integers)
     unsigned long long hmax_bits;       // max h for the current pass
-    unsigned long long hmax_next_bits;  // max h after the running density pass
+    unsigned long long hmax_next_bits;  // max h of still-unconverged particles after the running pass
+    unsigned long long hmax_conv_bits;  // max h of converged particles (frozen for the evaluation)
     unsigned long long dtp_bits;        // min dt_adapt of the last force pass
     int32_t iters, unconv, done, status;
 };
@@ -69,6 +71,9 @@ struct Ctl {
     int32_t iters_max;
     int64_t pairs;
     int64_t count;
+    int32_t act_cnt[2];  // lengths of the two active-particle lists of the density passes
+    int32_t overflow;    // a neighbour list did not fit its slots
+    int32_t pad;
 };
 
 struct Fields {
@@ -120,6 +125,60 @@ __device__ inline void stencil_from_hmax(Inst& s) {
             s.srad[a] = 0;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// stencil traversal
+// ---------------------------------------------------------------------------
+// cstart[c] / cend[c] hold, for every cell c (empty cells included), the
+// sorted positions [first key >= c, first key > c), so the x-adjacent cells of
+// one stencil row form ONE contiguous range [cstart[first], cend[last]).
+// for_each_row_range calls f(a, b) for each such range in cell order (z, y
+// outer, x inner), splitting a row in two where it wraps a periodic boundary.
+template <int DIM, class F>
+__device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
+                                                   const int32_t* __restrict__ cend, F&& f) {
+    const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
+    const int64_t local = key - s.cell_base;
+    const int cx = (int)(local % n0);
+    const int cy = (int)((local / n0) % n1);
+    const int cz = DIM == 3 ? (int)(local / ((int64_t)n0 * n1)) : 0;
+    int lo[3], hi[3];
+    const int c[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) {
+        const int na = s.n[a], sr = s.srad[a];
+        if (a >= DIM) { lo[a] = 0; hi[a] = 0; }
+        else if (s.periodic && 2 * sr + 1 >= na) { lo[a] = 0; hi[a] = na - 1; }
+        else if (s.periodic) { lo[a] = c[a] - sr; hi[a] = c[a] + sr; }
+        else { lo[a] = max(c[a] - sr, 0); hi[a] = min(c[a] + sr, na - 1); }
+    }
+    // x segments (at most two after wrapping)
+    int xa[2], xb[2], nseg;
+    if (lo[0] < 0) { xa[0] = lo[0] + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0]; nseg = 2; }
+    else if (hi[0] >= n0) { xa[0] = lo[0]; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0] - n0; nseg = 2; }
+    else { xa[0] = lo[0]; xb[0] = hi[0]; nseg = 1; }
+    for (int qz = lo[2]; qz <= hi[2]; ++qz) {
+        const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+        for (int qy = lo[1]; qy <= hi[1]; ++qy) {
+            const int wy = ((qy % n1) + n1) % n1;
+            const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+            for (int g = 0; g < nseg; ++g) f(cstart[row + xa[g]], cend[row + xb[g]]);
+        }
+    }
+}
+
+// f(j, rec) for j in [a, b), four independent loads in flight per thread
+template <class T, class F>
+__device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, int b, F&& f) {
+    int j = a;
+    for (; j + 4 <= b; j += 4) {
+        const T o0 = rec[j], o1 = rec[j + 1], o2 = rec[j + 2], o3 = rec[j + 3];
+        f(j, o0);
+        f(j + 1, o1);
+        f(j + 2, o2);
+        f(j + 3, o3);
+    }
+    for (; j < b; ++j) f(j, rec[j]);
 }
 
 // time step after a step: the pending min dt_adapt (adaptive) or the current
@@ -214,6 +273,7 @@ struct Dev {
     int64_t cell_cap;
     int32_t key_bits;
     int32_t nb_stride;      // neighbour-list capacity per particle
+    int32_t pass;           // density pass index within the evaluation
     hipStream_t stream;
     // device arrays
     Inst* inst;
@@ -224,7 +284,9 @@ struct Dev {
     uint32_t* rs_hist;      // 256 x tiles
     uint32_t* rs_base;      // 256
     int32_t *cstart, *cend;
-    double* dpack;          // 4 doubles per sorted particle
+    double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
+                            // passes, then {x,y,z,h} for the neighbour-list build
+    int32_t* act[2];        // active-particle lists of the density passes
     double* fpack;          // FREC doubles per sorted particle
     int32_t* nb;            // nb_stride x n, column major
     int32_t* nbcnt;
@@ -236,6 +298,7 @@ struct Dev {
 void launch_eval_begin(Dev& d, bool reset_dt);
 void launch_pack_xh(Dev& d);
 void launch_reset_nb_stats(Dev& d);
+void launch_reset_force_outputs(Dev& d);
 void launch_count_unconverged(Dev& d);
 void launch_hmax(Dev& d, bool next);
 void launch_grid(Dev& d);

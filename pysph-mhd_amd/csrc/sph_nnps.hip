@@ -27,11 +27,15 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
         ctl->iters_max = 0;
         ctl->pairs = 0;
         ctl->count = 0;
+        ctl->act_cnt[0] = 0;
+        ctl->act_cnt[1] = 0;
+        ctl->overflow = 0;
     }
     if (k >= ninst) return;
     Inst& s = inst[k];
     s.hmax_bits = 0ull;
     s.hmax_next_bits = 0ull;
+    s.hmax_conv_bits = 0ull;
     if (reset_dt) s.dtp_bits = 0x7FF0000000000000ull;  // +inf
     s.iters = 0;
     s.unconv = 0;
@@ -39,23 +43,46 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
     s.status = 0;
 }
 
-// max h per instance (wave-aggregated atomics; instances are contiguous)
+// max h per instance: wave then block reduction, one atomic per block when
+// the block lies inside one instance (instances are contiguous)
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
                        int64_t n, int next) {
+    __shared__ unsigned long long wmax[kBlock / kWave];
+    __shared__ int wk[kBlock / kWave];
+    __shared__ int wuni[kBlock / kWave];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     bool valid = i < n;
     int k = valid ? inst_of[i] : -1;
     unsigned long long b = valid ? dbits(h[i]) : 0ull;
     int k0 = __shfl(k, 0);
     bool uniform = __all(k == k0 || !valid);
-    if (uniform && k0 >= 0) {
+    if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
             unsigned long long o = __shfl_xor(b, off);
             b = o > b ? o : b;
         }
-        if ((threadIdx.x & 63) == 0) atomicMax(next ? &inst[k0].hmax_next_bits : &inst[k0].hmax_bits, b);
     } else if (valid) {
         atomicMax(next ? &inst[k].hmax_next_bits : &inst[k].hmax_bits, b);
+    }
+    if (lane == 0) {
+        wmax[wave] = uniform ? b : 0ull;
+        wk[wave] = uniform ? k0 : -2;
+        wuni[wave] = uniform;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int kb = wk[0];
+        bool same = true;
+        for (int w = 0; w < kBlock / kWave; ++w) same = same && wuni[w] && (wk[w] == kb || wk[w] == -1);
+        if (same && kb >= 0) {
+            unsigned long long m = 0ull;
+            for (int w = 0; w < kBlock / kWave; ++w) m = wmax[w] > m ? wmax[w] : m;
+            atomicMax(next ? &inst[kb].hmax_next_bits : &inst[kb].hmax_bits, m);
+        } else {
+            for (int w = 0; w < kBlock / kWave; ++w)
+                if (wuni[w] && wk[w] >= 0) atomicMax(next ? &inst[wk[w]].hmax_next_bits : &inst[wk[w]].hmax_bits, wmax[w]);
+        }
     }
 }
 
@@ -119,12 +146,22 @@ __global__ void k_keys(Fields F, const int32_t* __restrict__ inst_of, const Inst
 // stable LSD radix sort, 8-bit digits, wave-sized tiles
 // ---------------------------------------------------------------------------
 
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = kWave * RS_ITEMS;  // keys per wave tile
+constexpr int RS_TILE = kRadixTile;          // keys per wave tile
+constexpr int RS_ITEMS = RS_TILE / kWave;
 constexpr int RS_WAVES = kBlock / kWave;
 
+// A pass whose digit lies above the highest bit of the largest key (the cell
+// count is only known on the device) is an identity: hist/scan return at
+// once and the scatter becomes a copy, so the result always ends in the same
+// buffer whatever the number of live passes.
+__device__ __forceinline__ bool rs_pass_dead(const Ctl* ctl, int shift) {
+    const int64_t top = ctl->total_cells - 1;
+    return shift > 0 && top >= 0 && (top >> shift) == 0;
+}
+
 __global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                          uint32_t* __restrict__ hist, int64_t tiles) {
+                          uint32_t* __restrict__ hist, int64_t tiles, const Ctl* ctl) {
+    if (rs_pass_dead(ctl, shift)) return;
     __shared__ uint32_t h[RS_WAVES][256];
     int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     for (int d = lane; d < 256; d += kWave) h[wave][d] = 0;
@@ -143,7 +180,9 @@ __global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shif
 }
 
 // exclusive scan of one digit's row of tile counts; row totals to tot[d]
-__global__ void k_rs_scan_rows(uint32_t* __restrict__ hist, int64_t tiles, uint32_t* __restrict__ tot) {
+__global__ void k_rs_scan_rows(uint32_t* __restrict__ hist, int64_t tiles, uint32_t* __restrict__ tot,
+                               int shift, const Ctl* ctl) {
+    if (rs_pass_dead(ctl, shift)) return;
     __shared__ uint32_t part[kBlock];
     uint32_t* row = hist + (int64_t)blockIdx.x * tiles;
     int64_t per = (tiles + kBlock - 1) / kBlock;
@@ -167,7 +206,8 @@ __global__ void k_rs_scan_rows(uint32_t* __restrict__ hist, int64_t tiles, uint3
     if (threadIdx.x == kBlock - 1) tot[blockIdx.x] = part[kBlock - 1];
 }
 
-__global__ void k_rs_scan_digits(uint32_t* __restrict__ tot) {
+__global__ void k_rs_scan_digits(uint32_t* __restrict__ tot, int shift, const Ctl* ctl) {
+    if (rs_pass_dead(ctl, shift)) return;
     __shared__ uint32_t part[256];
     uint32_t v = tot[threadIdx.x];
     part[threadIdx.x] = v;
@@ -183,10 +223,22 @@ __global__ void k_rs_scan_digits(uint32_t* __restrict__ tot) {
 
 __global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                              uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
-                             const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dbase, int64_t tiles) {
+                             const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dbase, int64_t tiles,
+                             const Ctl* ctl) {
     __shared__ uint32_t run[RS_WAVES][256];
     int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     int64_t tile = (int64_t)blockIdx.x * RS_WAVES + wave;
+    if (rs_pass_dead(ctl, shift)) {
+        if (tile < tiles)
+            for (int it = 0; it < RS_ITEMS; ++it) {
+                int64_t idx = tile * RS_TILE + it * kWave + lane;
+                if (idx < n) {
+                    kout[idx] = kin[idx];
+                    vout[idx] = vin[idx];
+                }
+            }
+        return;
+    }
     for (int d = lane; d < 256; d += kWave)
         run[wave][d] = tile < tiles ? dbase[d] + hist[(int64_t)d * tiles + tile] : 0u;
     __syncthreads();
@@ -225,20 +277,25 @@ __global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __
 // cell table and packs
 // ---------------------------------------------------------------------------
 
-__global__ void k_cell_clear(int32_t* __restrict__ cstart, int32_t* __restrict__ cend, const Ctl* ctl, int64_t cap) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap || c >= ctl->total_cells) return;
-    cstart[c] = 0;
-    cend[c] = 0;
+// cell table: cstart[c] = first sorted position with key >= c and cend[c] =
+// first with key > c, for every cell (binary search of the sorted keys), so
+// empty cells still bound their neighbours' ranges
+__device__ __forceinline__ int32_t lower_bound_key(const uint32_t* __restrict__ skey, int64_t n, uint32_t k) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skey[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return (int32_t)lo;
 }
 
-__global__ void k_cell_ranges(const uint32_t* __restrict__ skey, int32_t* __restrict__ cstart,
-                              int32_t* __restrict__ cend, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    uint32_t k = skey[t];
-    if (t == 0 || skey[t - 1] != k) cstart[k] = (int32_t)t;
-    if (t == n - 1 || skey[t + 1] != k) cend[k] = (int32_t)(t + 1);
+__global__ void k_cell_bounds(const uint32_t* __restrict__ skey, int32_t* __restrict__ cstart,
+                              int32_t* __restrict__ cend, int64_t n, const Ctl* ctl, int64_t cap) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cap || c >= ctl->total_cells) return;
+    cstart[c] = lower_bound_key(skey, n, (uint32_t)c);
+    cend[c] = lower_bound_key(skey, n, (uint32_t)(c + 1));
 }
 
 __global__ void k_pack_density(Fields F, const int32_t* __restrict__ perm, double4* __restrict__ dpack, int64_t n,
@@ -255,7 +312,7 @@ __global__ void k_pack_density(Fields F, const int32_t* __restrict__ perm, doubl
 }
 
 // positions and h only, for neighbour exports outside an evaluation
-__global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double* __restrict__ fpack, int64_t n, int dim) {
+__global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double4* __restrict__ xh, int64_t n, int dim) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int32_t i = perm[t];
@@ -264,69 +321,64 @@ __global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double* __
     r.y = F.f[SPHMHD_F_Y][i];
     r.z = dim == 3 ? F.f[SPHMHD_F_Z][i] : 0.0;
     r.w = F.f[SPHMHD_F_H][i];
-    *(double4*)(fpack + t * FREC) = r;
+    xh[t] = r;
+}
+
+// before re-running a force pass: forget the partial pass's dt minimum and
+// warning bits (density warnings are kept)
+__global__ void k_reset_force_outputs(Inst* inst, int ninst, Ctl* ctl) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) ctl->status = 0;
+    if (k < ninst) {
+        inst[k].dtp_bits = 0x7FF0000000000000ull;
+        inst[k].status &= ~2;
+    }
 }
 
 __global__ void k_reset_nb_stats(Ctl* ctl) {
     ctl->max_nb = 0;
     ctl->pairs = 0;
+    ctl->overflow = 0;
 }
 
 // ---------------------------------------------------------------------------
-// neighbour lists for the force pass (column-major, nb[k * n + t])
+// neighbour lists for the force pass, column-major (nb[k * n + t]): the force
+// kernel's wave then reads one contiguous 256-byte row per neighbour slot
+// (measured faster than per-thread rows, whose lines do not stay in L1)
 // ---------------------------------------------------------------------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_nb_build(const double* __restrict__ fpack, const uint32_t* __restrict__ skey,
+__global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
                                                      const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const Inst& s = inst[inst_of[t]];
-    const double R = kernel_radius(s.kernel);
-    const double* me = fpack + t * FREC;
-    const double xi = me[FX], yi = me[FY], zi = DIM == 3 ? me[FZ] : 0.0, hi = me[FH];
-    int64_t local = (int64_t)skey[t] - s.cell_base;
-    int cx = (int)(local % s.n[0]);
-    int cy = (int)((local / s.n[0]) % s.n[1]);
-    int cz = DIM == 3 ? (int)(local / ((int64_t)s.n[0] * s.n[1])) : 0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
     int cnt = 0;
-    int ax[3], bx[3];
-    for (int a = 0; a < DIM; ++a) {
-        int c = a == 0 ? cx : (a == 1 ? cy : cz);
-        int sr = s.srad[a];
-        if (s.periodic && 2 * sr + 1 >= s.n[a]) { ax[a] = 0; bx[a] = s.n[a] - 1; }
-        else if (s.periodic) { ax[a] = c - sr; bx[a] = c + sr; }
-        else { ax[a] = c - sr < 0 ? 0 : c - sr; bx[a] = c + sr > s.n[a] - 1 ? s.n[a] - 1 : c + sr; }
-    }
-    if (DIM == 2) { ax[2] = 0; bx[2] = 0; }
-    for (int qz = ax[2]; qz <= bx[2]; ++qz) {
-        int wz = DIM == 3 ? ((qz % s.n[2]) + s.n[2]) % s.n[2] : 0;
-        for (int qy = ax[1]; qy <= bx[1]; ++qy) {
-            int wy = ((qy % s.n[1]) + s.n[1]) % s.n[1];
-            for (int qx = ax[0]; qx <= bx[0]; ++qx) {
-                int wx = ((qx % s.n[0]) + s.n[0]) % s.n[0];
-                int64_t cell = s.cell_base + wx + (int64_t)s.n[0] * (wy + (int64_t)s.n[1] * wz);
-                int j0 = cstart[cell], j1 = cend[cell];
-                for (int j = j0; j < j1; ++j) {
-                    const double* o = fpack + (int64_t)j * FREC;
-                    double dx = min_image(xi - o[FX], s.L[0], s.halfL[0], s.periodic);
-                    double dy = min_image(yi - o[FY], s.L[1], s.halfL[1], s.periodic);
-                    double dz = DIM == 3 ? min_image(zi - o[FZ], s.L[2], s.halfL[2], s.periodic) : 0.0;
-                    double r2 = r2_exact(dx, dy, dz);
-                    double hm = fmax(hi, o[FH]);
-                    double rad = R * hm;
-                    if (r2 < rad * rad) {
-                        if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
-                        ++cnt;
-                    }
+    if (valid) {
+        const Inst& s = inst[inst_of[t]];
+        const double R = kernel_radius(s.kernel);
+        const double4 me = xh[t];
+        const double xi = me.x, yi = me.y, zi = me.z, hi = me.w;
+        const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+        const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
+        const int per = s.periodic;
+        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b) {
+            scan_range4(xh, a, b, [&](int j, const double4& o) {
+                const double dx = min_image(xi - o.x, L0, H0, per);
+                const double dy = min_image(yi - o.y, L1, H1, per);
+                const double dz = DIM == 3 ? min_image(zi - o.z, L2, H2, per) : 0.0;
+                const double r2 = r2_exact(dx, dy, dz);
+                const double rad = R * fmax(hi, o.w);
+                if (r2 < rad * rad) {
+                    if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
+                    ++cnt;
                 }
-            }
-        }
+            });
+        });
+        nbcnt[t] = cnt;
     }
-    nbcnt[t] = cnt;
     // wave-aggregated max and pair count
     int m = cnt;
     long long pc = cnt;
@@ -338,6 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double* __restrict__ 
     if ((threadIdx.x & 63) == 0) {
         atomicMax(&ctl->max_nb, m);
         atomicAdd((unsigned long long*)&ctl->pairs, (unsigned long long)pc);
+        if (m > stride) atomicOr(&ctl->overflow, 1);
     }
 }
 
@@ -354,7 +407,12 @@ void launch_eval_begin(Dev& d, bool reset_dt) {
 
 void launch_pack_xh(Dev& d) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(k_pack_xh, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.fpack, d.n, d.dim);
+    hipLaunchKernelGGL(k_pack_xh, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm, (double4*)d.dpack,
+                       d.n, d.dim);
+}
+
+void launch_reset_force_outputs(Dev& d) {
+    hipLaunchKernelGGL(k_reset_force_outputs, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
 }
 
 void launch_reset_nb_stats(Dev& d) {
@@ -380,11 +438,13 @@ void launch_keys_sort_ranges(Dev& d) {
     uint32_t *ka = d.keys, *kb = d.keys_alt;
     int32_t *va = d.perm, *vb = d.perm_alt;
     for (int shift = 0; shift < d.key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(blocks), dim3(kBlock), 0, d.stream, ka, d.n, shift, d.rs_hist, tiles);
-        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(kBlock), 0, d.stream, d.rs_hist, tiles, d.rs_base);
-        hipLaunchKernelGGL(k_rs_scan_digits, dim3(1), dim3(256), 0, d.stream, d.rs_base);
+        hipLaunchKernelGGL(k_rs_hist, dim3(blocks), dim3(kBlock), 0, d.stream, ka, d.n, shift, d.rs_hist, tiles,
+                           d.ctl);
+        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(kBlock), 0, d.stream, d.rs_hist, tiles, d.rs_base, shift,
+                           d.ctl);
+        hipLaunchKernelGGL(k_rs_scan_digits, dim3(1), dim3(256), 0, d.stream, d.rs_base, shift, d.ctl);
         hipLaunchKernelGGL(k_rs_scatter, dim3(blocks), dim3(kBlock), 0, d.stream, ka, va, kb, vb, d.n, shift,
-                           d.rs_hist, d.rs_base, tiles);
+                           d.rs_hist, d.rs_base, tiles, d.ctl);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
     }
@@ -392,9 +452,8 @@ void launch_keys_sort_ranges(Dev& d) {
         uint32_t* tk = d.keys; d.keys = d.keys_alt; d.keys_alt = tk;
         int32_t* tv = d.perm; d.perm = d.perm_alt; d.perm_alt = tv;
     }
-    hipLaunchKernelGGL(k_cell_clear, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.ctl,
-                       d.cell_cap);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.keys, d.cstart, d.cend, d.n);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.keys, d.cstart, d.cend, d.n,
+                       d.ctl, d.cell_cap);
 }
 
 void launch_pack_density(Dev& d) {
@@ -406,10 +465,10 @@ void launch_pack_density(Dev& d) {
 void launch_nb_build(Dev& d) {
     if (d.n == 0) return;
     if (d.dim == 2)
-        hipLaunchKernelGGL(k_nb_build<2>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fpack, d.keys, d.cstart,
+        hipLaunchKernelGGL(k_nb_build<2>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
                            d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
     else
-        hipLaunchKernelGGL(k_nb_build<3>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fpack, d.keys, d.cstart,
+        hipLaunchKernelGGL(k_nb_build<3>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
                            d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
 }
 

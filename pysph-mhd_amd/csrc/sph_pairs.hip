@@ -24,13 +24,25 @@ __device__ __forceinline__ double iroot_dim(double x, int dim) { return dim == 2
 // G0: density sum + Newton step for h
 // ---------------------------------------------------------------------------
 
+// A pass either scans every particle (act_in == nullptr: the first pass of an
+// evaluation) or only the sorted positions listed in act_in (count at
+// *cnt_in): the particles still unconverged after the previous pass.  A
+// converged particle's h, omega and density sum cannot change again within
+// the evaluation (its own h is frozen and the sum only involves its h, x and
+// the neighbours' m, x), so skipping it is exact.  Particles that stay
+// unconverged are appended to act_out.
 template <int K, int DIM>
 __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __restrict__ dpack,
                                                     const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                                                    Inst* inst, const int32_t* __restrict__ inst_of, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = t < n;
+                                                    Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
+                                                    const int32_t* __restrict__ act_in, const int32_t* cnt_in,
+                                                    int32_t* __restrict__ act_out, int32_t* cnt_out) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t items = act_in ? (int64_t)*cnt_in : n;
+    bool valid = idx < items;
+    if (__all(!valid)) return;
+    int64_t t = valid ? (act_in ? (int64_t)act_in[idx] : idx) : 0;
     int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
     bool live = valid && !s.done;
@@ -46,44 +58,25 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
         const double h1 = 1.0 / hi;
         const double Rh = R * hi;
         const double Rh2 = Rh * Rh;
-        int64_t local = (int64_t)skey[t] - s.cell_base;
-        int c[3];
-        c[0] = (int)(local % s.n[0]);
-        c[1] = (int)((local / s.n[0]) % s.n[1]);
-        c[2] = DIM == 3 ? (int)(local / ((int64_t)s.n[0] * s.n[1])) : 0;
-        int lo[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
-        for (int a = 0; a < DIM; ++a) {
-            int sr = s.srad[a];
-            if (s.periodic && 2 * sr + 1 >= s.n[a]) { lo[a] = 0; hi_[a] = s.n[a] - 1; }
-            else if (s.periodic) { lo[a] = c[a] - sr; hi_[a] = c[a] + sr; }
-            else { lo[a] = c[a] - sr < 0 ? 0 : c[a] - sr; hi_[a] = c[a] + sr > s.n[a] - 1 ? s.n[a] - 1 : c[a] + sr; }
-        }
+        const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+        const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
+        const int per = s.periodic;
         double sw = 0.0, sg = 0.0;
-        for (int qz = lo[2]; qz <= hi_[2]; ++qz) {
-            int wz = DIM == 3 ? ((qz % s.n[2]) + s.n[2]) % s.n[2] : 0;
-            for (int qy = lo[1]; qy <= hi_[1]; ++qy) {
-                int wy = ((qy % s.n[1]) + s.n[1]) % s.n[1];
-                for (int qx = lo[0]; qx <= hi_[0]; ++qx) {
-                    int wx = ((qx % s.n[0]) + s.n[0]) % s.n[0];
-                    int64_t cell = s.cell_base + wx + (int64_t)s.n[0] * (wy + (int64_t)s.n[1] * wz);
-                    int j1 = cend[cell];
-                    for (int j = cstart[cell]; j < j1; ++j) {
-                        double4 o = dpack[j];
-                        double dx = min_image(me.x - o.x, s.L[0], s.halfL[0], s.periodic);
-                        double dy = min_image(me.y - o.y, s.L[1], s.halfL[1], s.periodic);
-                        double dz = DIM == 3 ? min_image(me.z - o.z, s.L[2], s.halfL[2], s.periodic) : 0.0;
-                        double r2 = r2_exact(dx, dy, dz);
-                        if (r2 < Rh2) {
-                            double q = sqrt(r2) * h1;
-                            double w, dw;
-                            shape<K, DIM>(q, w, dw);
-                            sw += o.w * w;
-                            sg += o.w * ((double)DIM * w + q * dw);
-                        }
-                    }
+        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b) {
+            scan_range4(dpack, a, b, [&](int j, const double4& o) {
+                const double dx = min_image(me.x - o.x, L0, H0, per);
+                const double dy = min_image(me.y - o.y, L1, H1, per);
+                const double dz = DIM == 3 ? min_image(me.z - o.z, L2, H2, per) : 0.0;
+                const double r2 = r2_exact(dx, dy, dz);
+                if (r2 < Rh2) {
+                    const double q = sqrt(r2) * h1;
+                    double w, dw;
+                    shape<K, DIM>(q, w, dw);
+                    sw += o.w * w;
+                    sg += o.w * ((double)DIM * w + q * dw);
                 }
-            }
-        }
+            });
+        });
         const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
         const double rho_sum = sw * fac;
         const double dwdh = -h1 * (fac * sg);
@@ -113,31 +106,55 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
         F.f[SPHMHD_F_H][i] = hout;
     }
 
-    // reductions: max h after this pass and the unconverged count, per instance
-    unsigned long long hb = live ? dbits(hout) : 0ull;
+    // still-unconverged particles go to the next pass's list
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long um = __ballot(unconv);
+    if (um) {
+        const int first = __ffsll((unsigned long long)um) - 1;
+        int base = 0;
+        if (lane == first) base = atomicAdd(cnt_out, __popcll(um));
+        base = __shfl(base, first);
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if (unconv) act_out[base + __popcll(um & lt)] = (int32_t)t;
+    }
+
+    // per instance: max h over converged particles (frozen from now on), max h
+    // over the still-unconverged ones, and their count
+    const bool conv_now = live && !unconv;
+    unsigned long long hc = conv_now ? dbits(hout) : 0ull;
+    unsigned long long hu = unconv ? dbits(hout) : 0ull;
     int uc = unconv ? 1 : 0;
     int k0 = __shfl(ki, 0);
     bool uniform = __all(ki == k0 || !valid);
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(hb, off);
-            hb = o > hb ? o : hb;
+            unsigned long long o = __shfl_xor(hc, off);
+            hc = o > hc ? o : hc;
+            o = __shfl_xor(hu, off);
+            hu = o > hu ? o : hu;
             uc += __shfl_xor(uc, off);
         }
-        if ((threadIdx.x & 63) == 0 && !inst[k0].done) {
-            atomicMax(&inst[k0].hmax_next_bits, hb);
+        if (lane == 0 && !inst[k0].done) {
+            if (hc) atomicMax(&inst[k0].hmax_conv_bits, hc);
+            if (hu) atomicMax(&inst[k0].hmax_next_bits, hu);
             if (uc) atomicAdd(&inst[k0].unconv, uc);
         }
     } else if (live) {
-        atomicMax(&inst[ki].hmax_next_bits, hb);
+        if (hc) atomicMax(&inst[ki].hmax_conv_bits, hc);
+        if (hu) atomicMax(&inst[ki].hmax_next_bits, hu);
         if (uc) atomicAdd(&inst[ki].unconv, uc);
     }
 }
 
 // Group iteration control: iterations >= min and converged, or == max.
-__global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl) {
+// Also clears the counter of the active list the next pass will write
+// (next_list), so passes can be launched back to back without the host.
+__global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
     __shared__ int alldone;
-    if (threadIdx.x == 0) alldone = 1;
+    if (threadIdx.x == 0) {
+        alldone = 1;
+        ctl->act_cnt[next_list] = 0;
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
@@ -147,7 +164,7 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl) {
             bool stop = !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
             if (s.density_iterate && !(s.iters >= s.min_iter && conv) && s.iters >= s.max_iter) s.status |= 1;
             s.unconv = 0;
-            s.hmax_bits = s.hmax_next_bits;
+            s.hmax_bits = s.hmax_conv_bits > s.hmax_next_bits ? s.hmax_conv_bits : s.hmax_next_bits;
             s.hmax_next_bits = 0ull;
             stencil_from_hmax(s);
             if (stop) s.done = 1;
@@ -164,7 +181,8 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl) {
 // ---------------------------------------------------------------------------
 
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
-                             const int32_t* __restrict__ inst_of, double* __restrict__ fpack, int64_t n) {
+                             const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
+                             double4* __restrict__ xh, int64_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
@@ -231,6 +249,7 @@ __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const I
     r[FPSI] = psipred;
     r[FH1] = 1.0 / h;
     r[FSPARE] = 0.0;
+    xh[t] = make_double4(r[FX], r[FY], r[FZ], h);
     double4* out = (double4*)(fpack + t * FREC);
 #pragma unroll
     for (int q = 0; q < FREC / 4; ++q) out[q] = make_double4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
@@ -255,7 +274,7 @@ template <int K, int DIM>
 __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
-                                                  const int32_t* __restrict__ inst_of, int64_t n) {
+                                                  const int32_t* __restrict__ inst_of, int64_t n, int stride) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = t < n;
     int ki = valid ? inst_of[t] : 0;
@@ -280,7 +299,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
         const double sig2 = DIM == 2 ? kernel_sigma(K, 2) : kernel_sigma(K, 3);
 
         Acc<DIM> A = {};
-        const int cnt = nbcnt[t];
+        const int cnt = min(nbcnt[t], stride);  // a longer list is re-run after growing
         for (int q = 0; q < cnt; ++q) {
             const int j = nb[(int64_t)q * n + t];
             const double* o = fpack + (int64_t)j * FREC;
@@ -551,23 +570,32 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kB
     default: if (DIMV == 2) CALL(SPHMHD_SUPER_GAUSSIAN, 2) else CALL(SPHMHD_SUPER_GAUSSIAN, 3) break; \
     }
 
+// pass 0 scans every particle and writes list 0; pass p > 0 reads list
+// (p - 1) % 2 and writes list p % 2
 void launch_density(Dev& d) {
     if (d.n == 0) return;
+    const int p = d.pass;
+    const int32_t* ain = p == 0 ? nullptr : d.act[(p - 1) & 1];
+    const int32_t* cin = p == 0 ? nullptr : &d.ctl->act_cnt[(p - 1) & 1];
+    int32_t* aout = d.act[p & 1];
+    int32_t* cout = &d.ctl->act_cnt[p & 1];
 #define CALL_DENS(KK, DD)                                                                                   \
     { hipLaunchKernelGGL((k_density<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields,        \
-                         (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n); }
+                         (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
+                         ain, cin, aout, cout); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS
 }
 
 void launch_g0_control(Dev& d) {
-    hipLaunchKernelGGL(k_g0_control, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+    // the next pass (d.pass + 1) writes list (d.pass + 1) % 2
+    hipLaunchKernelGGL(k_g0_control, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl, (d.pass + 1) & 1);
 }
 
 void launch_prep_force(Dev& d) {
     if (d.n == 0) return;
     hipLaunchKernelGGL(k_prep_force, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst, d.inst_of,
-                       d.fpack, d.n);
+                       d.fpack, (double4*)d.dpack, d.n);
 }
 
 void launch_eval_end(Dev& d) {
@@ -578,7 +606,7 @@ void launch_force(Dev& d) {
     if (d.n == 0) return;
 #define CALL_FORCE(KK, DD)                                                                                  \
     { hipLaunchKernelGGL((k_force<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
-                         d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n); }
+                         d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)
 #undef CALL_FORCE
 }

@@ -1,0 +1,139 @@
+"""The equation-group evaluation on the GPU (replaces PySPH's AccelerationEval
+and its compiled group kernels for the plan of scheme.py:67-81).
+
+HIPAccelerationEval checks that the plan handed to it is the reference's
+10-group plan (same equation classes, order, iteration settings and mutually
+consistent constants) and rejects anything else, then builds the library's
+parameter block from the plan constants, the domain and the solver, and
+binds the particle array's device mirror to a DeviceSystem.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import equations as E
+from ._abi import FIELDS, make_params
+
+PLAN = [
+    [E.DensityIteration],
+    [E.SmoothingLengthRateFromDensityRelation, E.DensityFromSmoothingLength],
+    [E.IdealGasEOS],
+    [E.CullenDehnenAlphaDiagnostics],
+    [E.StrainDiagnostics, E.MagneticStateReconstruction],
+    [E.MHDAccelerations],
+    [E.SmoothingLengthRateFromForceDivergence],
+    [E.MagneticStressAccelerations],
+    [E.MagneticStateRates],
+    [E.EnergyLimiter, E.DivBCorrection, E.AdaptiveTimestep],
+]
+
+
+class PlanError(ValueError):
+    pass
+
+
+def _same(values, what):
+    vals = [v for v in values if v is not None]
+    if vals and any(abs(v - vals[0]) > 1e-15 * max(1.0, abs(vals[0])) for v in vals):
+        raise PlanError(f"inconsistent {what} across the plan: {vals}")
+    return vals[0] if vals else None
+
+
+def read_plan(groups) -> dict:
+    """Validate the group plan and return its constants."""
+    if len(groups) != len(PLAN):
+        raise PlanError(f"expected {len(PLAN)} groups, got {len(groups)}")
+    for gi, (grp, want) in enumerate(zip(groups, PLAN)):
+        got = [type(e) for e in grp.equations]
+        if got != want:
+            raise PlanError(f"group {gi}: expected {[w.__name__ for w in want]}, got {[g.__name__ for g in got]}")
+        for e in grp.equations:
+            if e.dest != "fluid":
+                raise PlanError("only the single 'fluid' array of the reference is supported")
+    g0 = groups[0]
+    di = g0.equations[0]
+    if not g0.update_nnps:
+        raise PlanError("group 0 must update the neighbour search")
+    iterate = bool(g0.iterate)
+    if iterate == bool(di.iterate_only_once):
+        raise PlanError("DensityIteration.iterate_only_once must be the negation of the group's iterate flag")
+    eqs = {type(e): e for grp in groups for e in grp.equations}
+    k = _same([di.k, eqs[E.DensityFromSmoothingLength].k, eqs[E.SmoothingLengthRateFromDensityRelation].k], "hfact")
+    dim = int(_same([di.dim, eqs[E.DensityFromSmoothingLength].dim, eqs[E.SmoothingLengthRateFromDensityRelation].dim,
+                     1.0 / eqs[E.SmoothingLengthRateFromForceDivergence].dim1], "dim"))
+    mu0 = _same([eqs[E.MHDAccelerations].mu0, eqs[E.MagneticStressAccelerations].mu0,
+                 eqs[E.MagneticStateRates].mu0, eqs[E.DivBCorrection].mu0], "mu0")
+    tsf_a = eqs[E.AdaptiveTimestep].c_cour / E.COURANT_FACTOR
+    tsf_e = eqs[E.EnergyLimiter].c_cour / E.COURANT_FACTOR
+    tsf = _same([tsf_a, tsf_e], "timestep_factor")
+    mr = eqs[E.MagneticStateRates]
+    return dict(
+        iterate=iterate,
+        min_iterations=int(g0.min_iterations),
+        max_iterations=int(g0.max_iterations),
+        hfact=float(k),
+        dim=dim,
+        gamma=float(eqs[E.IdealGasEOS].gamma),
+        mu0=float(mu0),
+        artificial_viscosity=float(eqs[E.MHDAccelerations].artificial_viscosity),
+        artificial_thermal_conductivity=float(eqs[E.MHDAccelerations].artificial_thermal_conductivity),
+        artificial_magnetic_dissipation=float(mr.artificial_magnetic_dissipation),
+        cleaning_speed_factor=float(mr.cleaning_speed_factor),
+        cleaning_damping_factor=float(mr.cleaning_damping_factor),
+        timestep_factor=float(tsf),
+    )
+
+
+class HIPAccelerationEval:
+    def __init__(self, pa, groups, kernel, domain, scheme=None, solver=None):
+        from .device import DeviceSystem
+
+        self.pa = pa
+        self.groups = groups
+        self.plan = read_plan(groups)
+        self.density_eq = groups[0].equations[0]
+        dim = self.plan["dim"]
+        if kernel is not None and kernel.dim != dim:
+            raise PlanError("kernel dimension differs from the plan's")
+        if domain is None or not getattr(domain, "is_periodic", False):
+            raise PlanError("the device path needs a periodic domain (DomainManager with periodic bounds)")
+        b = domain.bounds
+        lo = (b[0], b[2], b[4])
+        hi = (b[1], b[3], b[5])
+        pl = self.plan
+        p = make_params(
+            dim=dim, kernel=kernel.kernel_id, lo=lo, hi=hi, gamma=pl["gamma"], mu0=pl["mu0"], hfact=pl["hfact"],
+            artificial_viscosity=pl["artificial_viscosity"],
+            artificial_thermal_conductivity=pl["artificial_thermal_conductivity"],
+            artificial_magnetic_dissipation=pl["artificial_magnetic_dissipation"],
+            cleaning_speed_factor=pl["cleaning_speed_factor"], cleaning_damping_factor=pl["cleaning_damping_factor"],
+            timestep_factor=pl["timestep_factor"],
+            alpha_min=getattr(scheme, "alpha_min", 0.0), alpha_max=getattr(scheme, "alpha_max", 1.0),
+            tf=float(getattr(solver, "tf", 1.0e30)), density="iterate" if pl["iterate"] else "single",
+            periodic=True, adaptive_timestep=bool(getattr(solver, "adaptive_timestep", True)),
+            max_iterations=pl["max_iterations"])
+        p.min_iterations = pl["min_iterations"]
+        self.params = p
+        self.system = DeviceSystem(p, n=len(pa))
+        pa.gpu.attach(self.system)
+        pa.gpu.push(*[n for n in FIELDS if n in pa.properties], "itype")
+
+    def initial_acceleration(self) -> int:
+        rc = self.system.eval()
+        conv = self.system.get("converged")
+        self.density_eq.equation_has_converged = 1 if np.all(conv == 1.0) else -1
+        return rc
+
+    def step(self, nsteps: int = 1) -> int:
+        return self.system.step(nsteps)
+
+    def run(self, nsteps: int, t: float, dt: float):
+        """nsteps PEC steps starting from the host clock (t, dt); returns the
+        new (t, dt) from the device clock."""
+        self.system.set_clock(t, dt)
+        self.system.step(nsteps)
+        tt, dtt = self.system.clock()
+        return float(tt[0]), float(dtt[0])
+
+    def pending_dt(self):
+        return self.system.pending_dt()

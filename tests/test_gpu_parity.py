@@ -226,3 +226,17 @@ def test_orszag_tang_128_100_steps(gpu):
     e_d = td[4] + td[5] + td[6]
     e_o = to[4] + to[5] + to[6]
     assert abs(e_d - e_o) <= 1e-8 * abs(e_o)
+
+
+def test_neighbour_list_growth(gpu):
+    """More neighbours than the initial list slots: the evaluation grows the
+    lists and re-runs only the list build and the force pass."""
+    s = _ot(32, hfact=1.9)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    assert dev.stats()["max_neighbours"] > 64
+    dev.step(2)
+    orc.step(2)
+    compare(dev, orc, STATE + RATES, TOL, "grown lists")
+    assert np.allclose(dev.clock()[1], orc.clock()[1], rtol=1e-12, atol=0)
