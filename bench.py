@@ -60,7 +60,27 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 512 MiB FP64 buffer 5 times (scales HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain device copy of known size: the FETCH_SIZE / WRITE_SIZE counters
+    of its dispatches give the factor between counted and moved bytes."""
+    import torch
+
+    n = 512 * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                      "copy_GBps": round(2 * n * 8 / dt / 1e9, 1)}))
 
 
 def make_setup(name: str):
@@ -187,6 +207,9 @@ def cpu_baseline(name: str, steps: int):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is a design (see DESIGN.md), "
                                    "so --gpus must be 1"}))

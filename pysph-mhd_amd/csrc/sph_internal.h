@@ -128,6 +128,26 @@ __device__ inline void stencil_from_hmax(Inst& s) {
 }
 
 // ---------------------------------------------------------------------------
+// XCD-aware tiles
+// ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Gather kernels launch 8 * ceil(tiles / 8) blocks and map block b to tile
+// (b % 8) * ceil(tiles / 8) + b / 8, so every XCD walks one contiguous range of
+// cell-ordered particles and its L2 keeps the neighbours they share.  Blocks
+// mapped beyond the last tile have nothing to do.
+constexpr int kXcds = 8;
+
+__host__ __device__ inline unsigned xcd_grid(int64_t tiles) {
+    return (unsigned)(((tiles + kXcds - 1) / kXcds) * kXcds);
+}
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t tiles) {
+    const int64_t per = (tiles + kXcds - 1) / kXcds;
+    const int64_t b = blockIdx.x;
+    return (b % kXcds) * per + b / kXcds;
+}
+
+// ---------------------------------------------------------------------------
 // stencil traversal
 // ---------------------------------------------------------------------------
 // cstart[c] / cend[c] hold, for every cell c (empty cells included), the

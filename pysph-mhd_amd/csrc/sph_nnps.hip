@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
     const bool valid = t < n;
     int cnt = 0;
     if (valid) {
@@ -465,10 +465,12 @@ void launch_pack_density(Dev& d) {
 void launch_nb_build(Dev& d) {
     if (d.n == 0) return;
     if (d.dim == 2)
-        hipLaunchKernelGGL(k_nb_build<2>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
+        hipLaunchKernelGGL(k_nb_build<2>, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
+                           d.cstart,
                            d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
     else
-        hipLaunchKernelGGL(k_nb_build<3>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
+        hipLaunchKernelGGL(k_nb_build<3>, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
+                           d.cstart,
                            d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
 }
 

@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
                                                     Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                                     const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                     int32_t* __restrict__ act_out, int32_t* cnt_out) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t idx = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
     bool valid = idx < items;
     if (__all(!valid)) return;
@@ -183,7 +183,7 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
                              double4* __restrict__ xh, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
     const int32_t i = perm[t];
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
     bool valid = t < n;
     int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
@@ -580,7 +580,7 @@ void launch_density(Dev& d) {
     int32_t* aout = d.act[p & 1];
     int32_t* cout = &d.ctl->act_cnt[p & 1];
 #define CALL_DENS(KK, DD)                                                                                   \
-    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields,        \
+    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
                          ain, cin, aout, cout); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
@@ -594,7 +594,8 @@ void launch_g0_control(Dev& d) {
 
 void launch_prep_force(Dev& d) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(k_prep_force, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst, d.inst_of,
+    hipLaunchKernelGGL(k_prep_force, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst,
+                       d.inst_of,
                        d.fpack, (double4*)d.dpack, d.n);
 }
 
@@ -605,7 +606,7 @@ void launch_eval_end(Dev& d) {
 void launch_force(Dev& d) {
     if (d.n == 0) return;
 #define CALL_FORCE(KK, DD)                                                                                  \
-    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
+    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
                          d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)
 #undef CALL_FORCE
