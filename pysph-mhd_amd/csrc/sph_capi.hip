@@ -343,7 +343,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
-        if ((rc = dalloc(&d.fpack, (int64_t)FREC * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.fpack, (int64_t)kFRecMax * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nb, (int64_t)d.nb_stride * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nbcnt, n))) return cleanup(rc);
         if ((rc = dalloc(&d.act[0], n))) return cleanup(rc);

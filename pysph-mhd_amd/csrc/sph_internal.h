@@ -5,10 +5,10 @@
 //     array per property, the PySPH ParticleArray layout).
 //   * Each evaluation bins particles into cells, radix-sorts the cell keys and
 //     then writes cell-ordered AoS "packs" of exactly what the pair loops read
-//     from a neighbour: a 32-byte record for the density pass and a 256-byte
-//     (two cache lines) record for the force pass.  Pair kernels run one
-//     thread per destination in cell order, so a wave's neighbours sit in a
-//     handful of contiguous pack runs.
+//     from a neighbour: a 32-byte record for the density pass and a 144-byte
+//     (2-D) or 160-byte (3-D) record for the force pass.  Pair kernels run
+//     one thread per destination in cell order, so a wave's neighbours sit in
+//     a handful of contiguous pack runs.
 //   * Per-instance scalars (grid, clock, reductions) live in one device array
 //     of Inst, so an ensemble of uneven instances runs in the same launches as
 //     a single instance.
@@ -25,18 +25,28 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kRadixTile = 256;  // keys per wave tile of the radix sort
 
-// Force-pack record slots (FP64 each, 32 per particle).
-enum FSlot {
-    FX = 0, FY, FZ, FH,
-    FM, FRHO, FP, FE,
-    FU, FV, FW, FOMEGA,
-    FBX, FBY, FBZ, FALPHA,
-    FAU, FAV, FAW, FPSICH,     // a of the previous evaluation; psipred * c_h
-    FVWAVE, FPBRHO2, FRHO2INV, FHALFINVRHO,
-    FBRHOX, FBRHOY, FBRHOZ, FBRO2,
-    FCS, FPSI, FH1, FSPARE,
-    FREC = 32
+// Force-pack record: the FP64 values the force pair loop reads from a
+// neighbour, 18 in 2-D (144 B) and 20 in 3-D (160 B), packed densely so that
+// a record is a whole number of 16-byte units.  The pair loop is bound by the
+// bytes it gathers, so whatever the loop can rebuild exactly from the record
+// is left out: p = (gamma - 1) rho e and 1/rho, P/rho^2, B/rho are recomputed
+// per pair with the same operations the prologue uses.  au, av, aw are the
+// accelerations of the previous evaluation; PSICH is psipred * c_h.
+template <int DIM>
+struct FRec {
+    static constexpr int X = 0, Y = 1, Z = 2;
+    static constexpr int H1 = DIM == 3 ? 3 : 2;  // 1 / h
+    static constexpr int M = H1 + 1, RHO = M + 1, E = RHO + 1;
+    static constexpr int U = E + 1, V = U + 1, W = V + 1, OM = W + 1;
+    static constexpr int BX = OM + 1, BY = BX + 1, BZ = BY + 1, AL = BZ + 1;
+    static constexpr int AU = AL + 1, AV = AU + 1, AW = AV + 1;  // AW only in 3-D
+    static constexpr int PSICH = DIM == 3 ? AW + 1 : AV + 1;
+    static constexpr int VW = PSICH + 1;  // wave speed
+    static constexpr int N = VW + 1;
+    static constexpr int UNITS = N / 2;  // 16-byte units per record
 };
+static_assert(FRec<2>::N == 18 && FRec<3>::N == 20, "force record sizes");
+constexpr int kFRecMax = 20;  // doubles allocated per particle
 
 struct Inst {
     // static parameters (filled at create)
@@ -307,7 +317,7 @@ struct Dev {
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build
     int32_t* act[2];        // active-particle lists of the density passes
-    double* fpack;          // FREC doubles per sorted particle
+    double* fpack;          // FRec<dim>::N doubles per sorted particle
     int32_t* nb;            // nb_stride x n, column major
     int32_t* nbcnt;
     double* scratch3;       // 3 x n (warm-start B copy)

@@ -10,6 +10,8 @@
 //   pairs  = r^2 < (R max(h_i, h_j))^2 with the minimum image
 // All of it is HBM-bound integer work: coalesced key writes, one LDS
 // histogram per wave tile and a stable multisplit scatter per 8-bit digit.
+#include <algorithm>
+
 #include "sph_internal.h"
 
 namespace sph {
@@ -43,45 +45,56 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
     s.status = 0;
 }
 
-// max h per instance: wave then block reduction, one atomic per block when
-// the block lies inside one instance (instances are contiguous)
+// max h per instance.  Same-address atomics serialise, so the grid is small
+// (kHmaxBlocks blocks, each over one contiguous chunk): a thread keeps a
+// running max for the instance it is in and flushes it when the instance
+// changes (instances are contiguous), then waves and the block combine what
+// is left into one atomic per block for a single instance.
+constexpr int kHmaxBlocks = 128;
+
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
-                       int64_t n, int next) {
+                       int64_t n, int next, int64_t chunk) {
     __shared__ unsigned long long wmax[kBlock / kWave];
     __shared__ int wk[kBlock / kWave];
-    __shared__ int wuni[kBlock / kWave];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    bool valid = i < n;
-    int k = valid ? inst_of[i] : -1;
-    unsigned long long b = valid ? dbits(h[i]) : 0ull;
-    int k0 = __shfl(k, 0);
-    bool uniform = __all(k == k0 || !valid);
+    const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = min(c0 + chunk, n);
+    int kcur = -1;
+    unsigned long long b = 0ull;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        const int k = inst_of[i];
+        if (k != kcur) {
+            if (kcur >= 0) atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
+            kcur = k;
+            b = 0ull;
+        }
+        const unsigned long long v = dbits(h[i]);
+        b = v > b ? v : b;
+    }
+    // threads with no element carry kcur = -1 and b = 0
+    int k0 = kcur;
+    for (int off = 32; off > 0; off >>= 1) k0 = max(k0, __shfl_xor(k0, off));
+    const bool uniform = __all(kcur == k0 || kcur < 0);
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(b, off);
+            const unsigned long long o = __shfl_xor(b, off);
             b = o > b ? o : b;
         }
-    } else if (valid) {
-        atomicMax(next ? &inst[k].hmax_next_bits : &inst[k].hmax_bits, b);
+    } else if (kcur >= 0) {
+        atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
     }
     if (lane == 0) {
-        wmax[wave] = uniform ? b : 0ull;
-        wk[wave] = uniform ? k0 : -2;
-        wuni[wave] = uniform;
+        wmax[wave] = b;
+        wk[wave] = uniform ? k0 : -1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int kb = wk[0];
-        bool same = true;
-        for (int w = 0; w < kBlock / kWave; ++w) same = same && wuni[w] && (wk[w] == kb || wk[w] == -1);
-        if (same && kb >= 0) {
-            unsigned long long m = 0ull;
-            for (int w = 0; w < kBlock / kWave; ++w) m = wmax[w] > m ? wmax[w] : m;
-            atomicMax(next ? &inst[kb].hmax_next_bits : &inst[kb].hmax_bits, m);
-        } else {
-            for (int w = 0; w < kBlock / kWave; ++w)
-                if (wuni[w] && wk[w] >= 0) atomicMax(next ? &inst[wk[w]].hmax_next_bits : &inst[wk[w]].hmax_bits, wmax[w]);
+        for (int q = 0; q < kBlock / kWave;) {
+            const int k = wk[q];
+            unsigned long long m = wmax[q];
+            int r = q + 1;
+            for (; r < kBlock / kWave && wk[r] == k; ++r) m = wmax[r] > m ? wmax[r] : m;
+            if (k >= 0) atomicMax(next ? &inst[k].hmax_next_bits : &inst[k].hmax_bits, m);
+            q = r;
         }
     }
 }
@@ -379,18 +392,31 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         });
         nbcnt[t] = cnt;
     }
-    // wave-aggregated max and pair count
+    // longest list and pair count: wave, then block, then one atomic each
+    // (same-address atomics serialise, so one per block rather than per wave)
     int m = cnt;
     long long pc = cnt;
     for (int off = 32; off > 0; off >>= 1) {
-        int o = __shfl_xor(m, off);
-        m = o > m ? o : m;
+        m = max(m, __shfl_xor(m, off));
         pc += __shfl_xor(pc, off);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&ctl->max_nb, m);
-        atomicAdd((unsigned long long*)&ctl->pairs, (unsigned long long)pc);
-        if (m > stride) atomicOr(&ctl->overflow, 1);
+    __shared__ int bm[kBlock / kWave];
+    __shared__ long long bpc[kBlock / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        bm[threadIdx.x / kWave] = m;
+        bpc[threadIdx.x / kWave] = pc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            m = max(m, bm[w]);
+            pc += bpc[w];
+        }
+        if (pc) {
+            atomicMax(&ctl->max_nb, m);
+            atomicAdd((unsigned long long*)&ctl->pairs, (unsigned long long)pc);
+            if (m > stride) atomicOr(&ctl->overflow, 1);
+        }
     }
 }
 
@@ -421,8 +447,10 @@ void launch_reset_nb_stats(Dev& d) {
 
 void launch_hmax(Dev& d, bool next) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(k_hmax, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields.f[SPHMHD_F_H], d.inst_of, d.inst,
-                       d.n, next ? 1 : 0);
+    const int64_t blocks = std::min<int64_t>(kHmaxBlocks, nblk(d.n));
+    const int64_t chunk = (d.n + blocks - 1) / blocks;
+    hipLaunchKernelGGL(k_hmax, dim3((unsigned)blocks), dim3(kBlock), 0, d.stream, d.fields.f[SPHMHD_F_H], d.inst_of,
+                       d.inst, d.n, next ? 1 : 0, chunk);
 }
 
 void launch_grid(Dev& d) {

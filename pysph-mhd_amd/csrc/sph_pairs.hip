@@ -38,10 +38,11 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
                                                     Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                                     const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                     int32_t* __restrict__ act_out, int32_t* cnt_out) {
-    int64_t idx = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
+    const int64_t idx0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
-    bool valid = idx < items;
-    if (__all(!valid)) return;
+    if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
+    const int64_t idx = idx0 + threadIdx.x;
+    const bool valid = idx < items;
     int64_t t = valid ? (act_in ? (int64_t)act_in[idx] : idx) : 0;
     int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
@@ -106,26 +107,23 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
         F.f[SPHMHD_F_H][i] = hout;
     }
 
-    // still-unconverged particles go to the next pass's list
-    const int lane = threadIdx.x & (kWave - 1);
+    // Still-unconverged particles go to the next pass's list, and per instance
+    // the max h over converged particles (frozen from now on), the max h over
+    // the still-unconverged ones and their count are reduced.  Same-address
+    // atomics serialise, so waves combine in LDS and thread 0 issues one
+    // atomic per quantity and block (per instance run when a block straddles
+    // instances; a wave that straddles instances falls back to lane atomics).
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    constexpr int NW = kBlock / kWave;
+    __shared__ int wcnt[NW], wbase[NW], wk[NW], wuc[NW];
+    __shared__ unsigned long long whc[NW], whu[NW];
     const unsigned long long um = __ballot(unconv);
-    if (um) {
-        const int first = __ffsll((unsigned long long)um) - 1;
-        int base = 0;
-        if (lane == first) base = atomicAdd(cnt_out, __popcll(um));
-        base = __shfl(base, first);
-        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        if (unconv) act_out[base + __popcll(um & lt)] = (int32_t)t;
-    }
-
-    // per instance: max h over converged particles (frozen from now on), max h
-    // over the still-unconverged ones, and their count
     const bool conv_now = live && !unconv;
     unsigned long long hc = conv_now ? dbits(hout) : 0ull;
     unsigned long long hu = unconv ? dbits(hout) : 0ull;
     int uc = unconv ? 1 : 0;
-    int k0 = __shfl(ki, 0);
-    bool uniform = __all(ki == k0 || !valid);
+    const int k0 = __shfl(ki, 0);  // lane 0 is valid whenever any lane is
+    const bool uniform = __all(ki == k0 || !valid);
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
             unsigned long long o = __shfl_xor(hc, off);
@@ -134,15 +132,51 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
             hu = o > hu ? o : hu;
             uc += __shfl_xor(uc, off);
         }
-        if (lane == 0 && !inst[k0].done) {
-            if (hc) atomicMax(&inst[k0].hmax_conv_bits, hc);
-            if (hu) atomicMax(&inst[k0].hmax_next_bits, hu);
-            if (uc) atomicAdd(&inst[k0].unconv, uc);
-        }
     } else if (live) {
         if (hc) atomicMax(&inst[ki].hmax_conv_bits, hc);
         if (hu) atomicMax(&inst[ki].hmax_next_bits, hu);
         if (uc) atomicAdd(&inst[ki].unconv, uc);
+    }
+    if (lane == 0) {
+        wcnt[w] = __popcll(um);
+        wk[w] = uniform && valid ? k0 : -1;
+        whc[w] = hc;
+        whu[w] = hu;
+        wuc[w] = uc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int q = 0; q < NW; ++q) {
+            wbase[q] = tot;
+            tot += wcnt[q];
+        }
+        if (tot) {
+            const int base = atomicAdd(cnt_out, tot);
+            for (int q = 0; q < NW; ++q) wbase[q] += base;
+        }
+        for (int q = 0; q < NW;) {
+            const int k = wk[q];
+            unsigned long long c = whc[q], u = whu[q];
+            int cu = wuc[q];
+            int r = q + 1;
+            for (; r < NW && wk[r] == k; ++r) {
+                c = whc[r] > c ? whc[r] : c;
+                u = whu[r] > u ? whu[r] : u;
+                cu += wuc[r];
+            }
+            if (k >= 0 && !inst[k].done) {
+                if (c) atomicMax(&inst[k].hmax_conv_bits, c);
+                if (u) atomicMax(&inst[k].hmax_next_bits, u);
+                if (cu) atomicAdd(&inst[k].unconv, cu);
+            }
+            q = r;
+        }
+    }
+    __syncthreads();
+    if (unconv) {
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        act_out[wbase[w] + __popcll(um & lt)] = (int32_t)t;
     }
 }
 
@@ -180,6 +214,39 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
 // G1 + G2 + G4 and the force pack
 // ---------------------------------------------------------------------------
 
+template <int DIM>
+__device__ __forceinline__ void write_force_record(double* __restrict__ fpack, int64_t t, const Fields& F, int32_t i,
+                                                   double x, double y, double z, double h, double m, double rho,
+                                                   double p, double e, double bx, double by, double bz, double vwave,
+                                                   double chf) {
+    using R = FRec<DIM>;
+    double* const* f = F.f;
+    double r[R::N];
+    r[R::X] = x;
+    r[R::Y] = y;
+    if constexpr (DIM == 3) r[R::Z] = z;
+    r[R::H1] = 1.0 / h;
+    r[R::M] = m;
+    r[R::RHO] = rho;
+    r[R::E] = e;
+    r[R::U] = f[SPHMHD_F_UPRED][i];
+    r[R::V] = f[SPHMHD_F_VPRED][i];
+    r[R::W] = f[SPHMHD_F_WPRED][i];
+    r[R::OM] = f[SPHMHD_F_OMEGA][i];
+    r[R::BX] = bx;
+    r[R::BY] = by;
+    r[R::BZ] = bz;
+    r[R::AL] = f[SPHMHD_F_ALPHA1][i];
+    r[R::AU] = f[SPHMHD_F_AU][i];
+    r[R::AV] = f[SPHMHD_F_AV][i];
+    if constexpr (DIM == 3) r[R::AW] = f[SPHMHD_F_AW][i];
+    r[R::PSICH] = f[SPHMHD_F_PSIPRED][i] * (chf * vwave);
+    r[R::VW] = vwave;
+    double2* out = (double2*)(fpack + t * R::N);
+#pragma unroll
+    for (int q = 0; q < R::UNITS; ++q) out[q] = make_double2(r[2 * q], r[2 * q + 1]);
+}
+
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
                              double4* __restrict__ xh, int64_t n) {
@@ -213,46 +280,12 @@ __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const I
 
     const double b2 = bx * bx + by * by + bz * bz;
     const double vwave = sqrt(cs * cs + b2 / (s.mu0 * rho));
-    const double inv_rho = 1.0 / rho;
-    const double brx = bx * inv_rho, bry = by * inv_rho, brz = bz * inv_rho;
-    const double psipred = f[SPHMHD_F_PSIPRED][i];
-    double r[FREC];
-    r[FX] = f[SPHMHD_F_X][i];
-    r[FY] = f[SPHMHD_F_Y][i];
-    r[FZ] = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
-    r[FH] = h;
-    r[FM] = m;
-    r[FRHO] = rho;
-    r[FP] = p;
-    r[FE] = epred;
-    r[FU] = f[SPHMHD_F_UPRED][i];
-    r[FV] = f[SPHMHD_F_VPRED][i];
-    r[FW] = f[SPHMHD_F_WPRED][i];
-    r[FOMEGA] = f[SPHMHD_F_OMEGA][i];
-    r[FBX] = bx;
-    r[FBY] = by;
-    r[FBZ] = bz;
-    r[FALPHA] = f[SPHMHD_F_ALPHA1][i];
-    r[FAU] = f[SPHMHD_F_AU][i];
-    r[FAV] = f[SPHMHD_F_AV][i];
-    r[FAW] = f[SPHMHD_F_AW][i];
-    r[FPSICH] = psipred * (s.chf * vwave);
-    r[FVWAVE] = vwave;
-    r[FPBRHO2] = p / (rho * rho);
-    r[FRHO2INV] = 1.0 / (rho * rho);
-    r[FHALFINVRHO] = 0.5 * inv_rho;
-    r[FBRHOX] = brx;
-    r[FBRHOY] = bry;
-    r[FBRHOZ] = brz;
-    r[FBRO2] = (brx * brx + bry * bry + brz * brz) / s.mu0;
-    r[FCS] = cs;
-    r[FPSI] = psipred;
-    r[FH1] = 1.0 / h;
-    r[FSPARE] = 0.0;
-    xh[t] = make_double4(r[FX], r[FY], r[FZ], h);
-    double4* out = (double4*)(fpack + t * FREC);
-#pragma unroll
-    for (int q = 0; q < FREC / 4; ++q) out[q] = make_double4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    const double x = f[SPHMHD_F_X][i], y = f[SPHMHD_F_Y][i], z = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
+    xh[t] = make_double4(x, y, z, h);
+    if (dim == 2)
+        write_force_record<2>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf);
+    else
+        write_force_record<3>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,49 +308,106 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride) {
-    int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
-    bool valid = t < n;
-    int ki = valid ? inst_of[t] : 0;
+    using R = FRec<DIM>;
+    constexpr int U = R::UNITS;
+    const double2* __restrict__ pk = (const double2*)fpack;
+
+    const int64_t t0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
+    if (t0 >= n) return;  // whole block past the end (the block barrier below needs every wave)
+    const int64_t t = t0 + threadIdx.x;
+    const bool valid = t < n;
+    const int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
     double dt_adapt = 1.0e300;
     bool bad = false;
+
+    // destination values (rows of the pack are read once, so plain loads)
+    double me[R::N];
+    int cnt = 0;
     if (valid) {
-        const double* me = fpack + t * FREC;
-        const double xi = me[FX], yi = me[FY], zi = DIM == 3 ? me[FZ] : 0.0;
-        const double hi = me[FH], h1i = me[FH1];
-        const double rhoi = me[FRHO], pi_ = me[FP], ei = me[FE];
-        const double ui = me[FU], vi = me[FV], wi = me[FW];
-        const double omi = me[FOMEGA], ali = me[FALPHA];
-        const double bxi = me[FBX], byi = me[FBY], bzi = me[FBZ];
-        const double aui = me[FAU], avi = me[FAV], awi = me[FAW];
-        const double psichi = me[FPSICH], vwi = me[FVWAVE], pbi = me[FPBRHO2];
-        const double r2ii = me[FRHO2INV], hiri = me[FHALFINVRHO];
-        const double brxi = me[FBRHOX], bryi = me[FBRHOY], brzi = me[FBRHOZ], bro2i = me[FBRO2];
-        const double fac_i = (DIM == 2 ? kernel_sigma(K, 2) * h1i * h1i : kernel_sigma(K, 3) * h1i * h1i * h1i) * h1i;
-        const double inv_mu0 = 1.0 / s.mu0;
-        const double av = s.av, ac = s.ac, ar = s.ar;
-        const double sig2 = DIM == 2 ? kernel_sigma(K, 2) : kernel_sigma(K, 3);
+#pragma unroll
+        for (int c = 0; c < U; ++c) {
+            const double2 v = pk[t * U + c];
+            me[2 * c] = v.x;
+            me[2 * c + 1] = v.y;
+        }
+        cnt = min(nbcnt[t], stride);  // a longer list is re-run after growing
+    } else {
+#pragma unroll
+        for (int c = 0; c < R::N; ++c) me[c] = 1.0;
+    }
+    const double xi = me[R::X], yi = me[R::Y], zi = DIM == 3 ? me[R::Z] : 0.0;
+    const double h1i = me[R::H1];
+    const double rhoi = me[R::RHO], ei = me[R::E];
+    const double pi_ = s.gamma1 * rhoi * ei;  // IdealGasEOS, as k_prep_force
+    const double ui = me[R::U], vi = me[R::V], wi = me[R::W];
+    const double omi = me[R::OM], ali = me[R::AL];
+    const double bxi = me[R::BX], byi = me[R::BY], bzi = me[R::BZ];
+    const double aui = me[R::AU], avi = me[R::AV], awi = DIM == 3 ? me[R::AW] : 0.0;
+    const double psichi = me[R::PSICH], vwi = me[R::VW];
+    const double irhoi = 1.0 / rhoi, hiri = 0.5 * irhoi;
+    const double inv_mu0 = 1.0 / s.mu0;
+    const double r2ii = irhoi * irhoi;
+    const double pbi = pi_ * r2ii;
+    const double brxi = bxi * irhoi, bryi = byi * irhoi, brzi = bzi * irhoi;
+    const double bro2i = (brxi * brxi + bryi * bryi + brzi * brzi) * inv_mu0;
+    const double sig = DIM == 2 ? kernel_sigma(K, 2) : kernel_sigma(K, 3);
+    const double fac_i = (DIM == 2 ? sig * h1i * h1i : sig * h1i * h1i * h1i) * h1i;
+    const double av = s.av, ac = s.ac, ar = s.ar, gamma1 = s.gamma1;
+    const double Lx = s.L[0], Ly = s.L[1], Lz = s.L[2];
+    const double hLx = s.halfL[0], hLy = s.halfL[1], hLz = s.halfL[2];
+    const int periodic = s.periodic;
 
-        Acc<DIM> A = {};
-        const int cnt = min(nbcnt[t], stride);  // a longer list is re-run after growing
-        for (int q = 0; q < cnt; ++q) {
-            const int j = nb[(int64_t)q * n + t];
-            const double* o = fpack + (int64_t)j * FREC;
-            const double4 o0 = *(const double4*)(o + 0);   // x y z h
-            const double4 o1 = *(const double4*)(o + 4);   // m rho p e
-            const double4 o2 = *(const double4*)(o + 8);   // u v w omega
-            const double4 o3 = *(const double4*)(o + 12);  // bx by bz alpha
-            const double4 o4 = *(const double4*)(o + 16);  // au av aw psich
-            const double4 o5 = *(const double4*)(o + 20);  // vwave pbrho2 rho2inv halfinvrho
-            const double4 o6 = *(const double4*)(o + 24);  // brho(3) bro2
-            const double h1j = o[FH1];
+    // The pass is bound by the bytes gathered from neighbour records, so the
+    // record is kept small.  In 2-D the next neighbour's record is loaded while
+    // the current one is computed (the index one slot further ahead); 3-D and
+    // the Gaussian kernels have no registers to spare for that (it would halve
+    // their occupancy).
+    constexpr bool kPrefetch = DIM == 2 && K != SPHMHD_GAUSSIAN && K != SPHMHD_SUPER_GAUSSIAN;
+    int jn = cnt > (kPrefetch ? 1 : 0) ? nb[(kPrefetch ? (int64_t)n : 0) + t] : -1;
+    double2 nxt[U];
+    if (kPrefetch && cnt > 0) {
+        const int64_t j0 = nb[t];
+#pragma unroll
+        for (int c = 0; c < U; ++c) nxt[c] = pk[j0 * U + c];
+    }
 
-            double dx = min_image(xi - o0.x, s.L[0], s.halfL[0], s.periodic);
-            double dy = min_image(yi - o0.y, s.L[1], s.halfL[1], s.periodic);
-            double dz = DIM == 3 ? min_image(zi - o0.z, s.L[2], s.halfL[2], s.periodic) : 0.0;
+    Acc<DIM> A = {};
+    for (int q = 0; q < cnt; ++q) {
+        if (!kPrefetch) {
+#pragma unroll
+            for (int c = 0; c < U; ++c) nxt[c] = pk[(int64_t)jn * U + c];
+        }
+        double o[R::N];
+#pragma unroll
+        for (int c = 0; c < U; ++c) {
+            o[2 * c] = nxt[c].x;
+            o[2 * c + 1] = nxt[c].y;
+        }
+        if (kPrefetch && jn >= 0) {
+#pragma unroll
+            for (int c = 0; c < U; ++c) nxt[c] = pk[(int64_t)jn * U + c];
+        }
+        const int ahead = kPrefetch ? 2 : 1;
+        jn = q + ahead < cnt ? nb[(int64_t)(q + ahead) * n + t] : -1;
+        {
+            const double h1j = o[R::H1], mj = o[R::M], rhoj = o[R::RHO], ej = o[R::E];
+            const double pj = gamma1 * rhoj * ej;
+            const double uj = o[R::U], vj = o[R::V], wj = o[R::W], omj = o[R::OM];
+            const double bxj = o[R::BX], byj = o[R::BY], bzj = o[R::BZ], alj = o[R::AL];
+            const double auj = o[R::AU], avj = o[R::AV], awj = DIM == 3 ? o[R::AW] : 0.0;
+            const double psichj = o[R::PSICH], vwj = o[R::VW];
+            const double irhoj = 1.0 / rhoj, hirj = 0.5 * irhoj;
+            const double r2ij = irhoj * irhoj;
+            const double pbj = pj * r2ij;
+            const double brxj = bxj * irhoj, bryj = byj * irhoj, brzj = bzj * irhoj;
+            const double bro2j = (brxj * brxj + bryj * bryj + brzj * brzj) * inv_mu0;
+
+            double dx = min_image(xi - o[R::X], Lx, hLx, periodic);
+            double dy = min_image(yi - o[R::Y], Ly, hLy, periodic);
+            double dz = DIM == 3 ? min_image(zi - o[R::Z], Lz, hLz, periodic) : 0.0;
             const double r2 = r2_exact(dx, dy, dz);
             const double r = sqrt(r2);
-            const double mj = o1.x;
 
             // kernel gradients DWI, DWJ (0 for r <= 1e-12)
             double gi = 0.0, gj = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
@@ -325,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
             if (far12) {
                 const double ir = 1.0 / r;
                 gi = shape_dw<K, DIM>(r * h1i) * fac_i * ir;
-                const double facj = (DIM == 2 ? sig2 * h1j * h1j : sig2 * h1j * h1j * h1j) * h1j;
+                const double facj = (DIM == 2 ? sig * h1j * h1j : sig * h1j * h1j * h1j) * h1j;
                 gj = shape_dw<K, DIM>(r * h1j) * facj * ir;
                 ex = dx * ir;
                 ey = dy * ir;
@@ -335,9 +425,9 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
             const double dwjx = gj * dx, dwjy = gj * dy, dwjz = gj * dz;
 
             // relative velocity, projections
-            const double dvx = ui - o2.x, dvy = vi - o2.y, dvz = wi - o2.z;
+            const double dvx = ui - uj, dvy = vi - vj, dvz = wi - wj;
             const double gri = omi * (ex * dwix + ey * dwiy + ez * dwiz);
-            const double grj = o2.w * (ex * dwjx + ey * dwjy + ez * dwjz);
+            const double grj = omj * (ex * dwjx + ey * dwjy + ez * dwjz);
             const double dot = dvx * ex + dvy * ey + dvz * ez;
 
             // ---- MHDAccelerations (equations.py:217-275); unit vector is 0 below 1e-8
@@ -347,50 +437,50 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                 const double griA = near8 ? 0.0 : gri;
                 const double grjA = near8 ? 0.0 : grj;
                 A.div += mj * dotA * griA;
-                const double vsig2 = sqrt(fabs(pi_ - o1.z) / (0.5 * (rhoi + o1.y)));
-                double c = fmax(vwi - 2.0 * dotA, o5.x - 2.0 * dotA);
+                const double vsig2 = sqrt(fabs(pi_ - pj) / (0.5 * (rhoi + rhoj)));
+                double c = fmax(vwi - 2.0 * dotA, vwj - 2.0 * dotA);
                 c = fmax(c, 0.0);
                 A.dtcfl = fmax(A.dtcfl, c);
-                double sx = pbi * omi, sjx = o5.y * o2.w;  // P/rho^2 * omega
+                double sx = pbi * omi, sjx = pbj * omj;  // P/rho^2 * omega
                 if (dotA < 0.0) {
                     const double vsi = fmax(ali * vwi - 2.0 * dotA, 0.0);
-                    const double vsj = fmax(o3.w * o5.x - 2.0 * dotA, 0.0);
+                    const double vsj = fmax(alj * vwj - 2.0 * dotA, 0.0);
                     const double qi = -hiri * vsi * dotA * av;
-                    const double qj = -o5.w * vsj * dotA * av;
+                    const double qj = -hirj * vsj * dotA * av;
                     sx += qi * omi;
-                    sjx += qj * o2.w;
+                    sjx += qj * omj;
                     A.ae += mj * qi * dotA * griA;
                 }
                 A.ax -= mj * (sx * dwix + sjx * dwjx);
                 A.ay -= mj * (sx * dwiy + sjx * dwjy);
                 A.az -= mj * (sx * dwiz + sjx * dwjz);
                 A.ae += mj * pbi * omi * (dvx * dwix + dvy * dwiy + dvz * dwiz);
-                A.ae += ac * vsig2 * (ei - o1.w) * mj * (hiri * griA + o5.w * grjA);
+                A.ae += ac * vsig2 * (ei - ej) * mj * (hiri * griA + hirj * grjA);
             }
             // ---- MagneticStressAccelerations (equations.py:149-195)
             {
-                const double gradp = mj * (0.5 * bro2i * gri + 0.5 * o6.w * grj);
+                const double gradp = mj * (0.5 * bro2i * gri + 0.5 * bro2j * grj);
                 const double pi_b = (brxi * ex + bryi * ey + brzi * ez) * gri;
-                const double pj_b = (o6.x * ex + o6.y * ey + o6.z * ez) * grj;
+                const double pj_b = (brxj * ex + bryj * ey + brzj * ez) * grj;
                 const double sm = mj * inv_mu0;
-                A.ax += -ex * gradp + sm * (brxi * pi_b + o6.x * pj_b);
-                A.ay += -ey * gradp + sm * (bryi * pi_b + o6.y * pj_b);
-                A.az += -ez * gradp + sm * (brzi * pi_b + o6.z * pj_b);
+                A.ax += -ex * gradp + sm * (brxi * pi_b + brxj * pj_b);
+                A.ay += -ey * gradp + sm * (bryi * pi_b + bryj * pj_b);
+                A.az += -ez * gradp + sm * (brzi * pi_b + brzj * pj_b);
             }
             // ---- MagneticStateRates (equations.py:507-559)
             {
                 const double pbi_ = bxi * ex + byi * ey + bzi * ez;
-                const double pbj_ = o3.x * ex + o3.y * ey + o3.z * ez;
-                const double dbx = bxi - o3.x, dby = byi - o3.y, dbz = bzi - o3.z;
+                const double pbj_ = bxj * ex + byj * ey + bzj * ez;
+                const double dbx = bxi - bxj, dby = byi - byj, dbz = bzi - bzj;
                 const double pdb = dbx * ex + dby * ey + dbz * ez;
                 const double db2 = dbx * dbx + dby * dby + dbz * dbz;
                 const double pmi = mj * r2ii * gri;
-                const double pmj = mj * o5.z * grj;
+                const double pmj = mj * r2ij * grj;
                 const double brt = -(pmi * pbi_);
                 const double tx = dvx - dot * ex, ty = dvy - dot * ey, tz = dvz - dot * ez;
                 const double vsb = sqrt(tx * tx + ty * ty + tz * tz);
                 const double diss = 0.5 * (pmi + pmj) * vsb * ar;
-                const double dpsi = pmi * psichi + pmj * o4.w;
+                const double dpsi = pmi * psichi + pmj * psichj;
                 A.bx += brt * dvx + diss * dbx - dpsi * ex;
                 A.by += brt * dvy + diss * dby - dpsi * ey;
                 A.bz += brt * dvz + diss * dbz - dpsi * ez;
@@ -401,13 +491,13 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
             // ---- CullenDehnenAlphaDiagnostics loop (equations.py:319-354)
             {
                 const double rx = mj * dwix, ry = mj * dwiy, rz = mj * dwiz;
-                const double dax = aui - o4.x, day = avi - o4.y;
+                const double dax = aui - auj, day = avi - avj;
                 A.vxx += dvx * rx; A.vxy += dvx * ry;
                 A.vyx += dvy * rx; A.vyy += dvy * ry;
                 A.vzx += dvz * rx; A.vzy += dvz * ry;
                 A.axx += dax * rx; A.ayy += day * ry;
                 if constexpr (DIM == 3) {
-                    const double daz = awi - o4.z;
+                    const double daz = awi - awj;
                     A.rxx += -dx * rx; A.rxy += -dx * ry; A.rxz += -dx * rz;
                     A.ryy += -dy * ry; A.ryz += -dy * rz; A.rzz += -dz * rz;
                     A.vxz += dvx * rz; A.vyz += dvy * rz; A.vzz += dvz * rz;
@@ -417,11 +507,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                 }
             }
         }
+    }
 
+    if (valid) {
         // ---- epilogue ------------------------------------------------------
         const int32_t i = perm[t];
         double** f = F.f;
-        const double cs = me[FCS], psipred = me[FPSI];
+        const double hi = f[SPHMHD_F_H][i];
+        const double cs = f[SPHMHD_F_CS][i], psipred = f[SPHMHD_F_PSIPRED][i];
         // MHDAccelerations.post_loop (equations.py:279-280)
         const double div = -A.div / rhoi;
         // SmoothingLengthRateFromForceDivergence (equations.py:140)
@@ -527,23 +620,47 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
         f[SPHMHD_F_DIVVDT][i] = divvdt;
     }
 
-    // min dt_adapt per instance; NaN flag
+    // min dt_adapt and the NaN flag per instance: wave, then block, then one
+    // atomic per instance run of the block (same-address atomics serialise)
+    constexpr int NW = kBlock / kWave;
+    __shared__ unsigned long long wdb[NW];
+    __shared__ int wk[NW], wbad[NW];
+    const int w = threadIdx.x / kWave;
     unsigned long long db = dbits(dt_adapt);
-    int k0 = __shfl(ki, 0);
-    bool uniform = __all(ki == k0 || !valid);
-    bool anybad = __any(bad);
+    const int k0 = __shfl(ki, 0);  // lane 0 is valid whenever any lane is
+    const bool uniform = __all(ki == k0 || !valid);
+    const bool anybad = __any(bad);
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
             unsigned long long o = __shfl_xor(db, off);
             db = o < db ? o : db;
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&inst[k0].dtp_bits, db);
-            if (anybad) atomicOr(&inst[k0].status, 2);
-        }
     } else if (valid) {
         atomicMin(&inst[ki].dtp_bits, db);
         if (bad) atomicOr(&inst[ki].status, 2);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        wk[w] = uniform && valid ? k0 : -1;
+        wdb[w] = db;
+        wbad[w] = anybad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NW;) {
+            const int k = wk[q];
+            unsigned long long m = wdb[q];
+            int b = wbad[q];
+            int r = q + 1;
+            for (; r < NW && wk[r] == k; ++r) {
+                m = wdb[r] < m ? wdb[r] : m;
+                b |= wbad[r];
+            }
+            if (k >= 0) {
+                atomicMin(&inst[k].dtp_bits, m);
+                if (b) atomicOr(&inst[k].status, 2);
+            }
+            q = r;
+        }
     }
 }
 
