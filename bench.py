@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT / "pysph-mhd_amd"))
 
 METRIC = "particle-steps/s & HBM roofline % (plain vs batched vs fwd+adjoint), 1 GPU"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, vector and matrix alike
 
 # Algorithmic bytes of one force launch per particle (compulsory traffic: each
 # live input read once, each output written once, FP64): reads x, y, h, m,
@@ -39,6 +39,11 @@ FP64_PEAK_TFLOPS = 78.6
 # (20 fields, 160 B in 2-D; 21 in 3-D) and writes a(3), ae, ah, aBevol(3),
 # apsi, alphaloc, dt_adapt (11 fields, 88 B).
 FORCE_BYTES_PER_PARTICLE = {2: 248.0, 3: 256.0}
+
+# Measured HBM traffic of the force kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
+# per dispatch, scaled by a plain copy of known size), kept in the profile
+# summaries as "traffic_bytes_per_launch[<workload>]: <bytes>".
+TRAFFIC_FILE = ROOT / "profiles" / "k_force_hbm_traffic.md"
 
 
 def _flops_per_pair(dim: int) -> float:
@@ -50,16 +55,29 @@ def _flops_per_pair(dim: int) -> float:
     return 330.0 if dim == 2 else 390.0
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per force launch for this workload from the committed
+    profile summary, or None when it has not been profiled."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            key = f"traffic_bytes_per_launch[{workload}]:"
+            if line.strip().startswith(key):
+                return float(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="rotor512",
-                    choices=["rotor512", "ot1m", "blast128", "ensemble64", "ot128"])
+                    choices=["rotor512", "ot1m", "blast128", "ensemble64", "ot128", "ot128x64"])
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 512 MiB FP64 buffer 5 times (scales HBM byte counters)")
     return ap.parse_args()
@@ -91,7 +109,10 @@ def make_setup(name: str):
     if name == "ot1m":
         return B.orszag_tang_2d(1024), "orszag_tang_2d 1024x1024 (1,048,576 particles), quintic, h=1.2dx"
     if name == "ot128":
-        return B.orszag_tang_2d(128), "orszag_tang_2d 128x128 (16,384 particles), quintic, h=1.2dx"
+        return B.orszag_tang_2d(128), "orszag_tang_2d 128x128 (16,384 particles), quintic, h=1.2dx, one instance"
+    if name == "ot128x64":
+        s = B.ensemble_of([B.orszag_tang_2d(128) for _ in range(64)])
+        return s, "64 identical orszag_tang_2d 128x128 instances in one batch (1,048,576 particles)"
     if name == "blast128":
         return B.blast_3d(128), "blast_3d 128^3 (2,097,152 particles), quintic, h=0.81dx"
     if name == "ensemble64":
@@ -140,6 +161,11 @@ def run_gpu(name: str, steps: int, warmup: int):
 
 
 def roofline(res):
+    """Roofline of the dominant kernel (the fused force pass).  Its binding
+    roof is the FP64 rate: per launch it does flops_per_pair x pairs of FP64
+    work on bytes_alg of compulsory HBM traffic, about 24 flop/B against a
+    ridge of 9.8 flop/B.  The HBM fraction of the same launch is reported
+    beside it."""
     tm = res["timing"]
     n = res["particles"]
     dim = res["dim"]
@@ -150,26 +176,29 @@ def roofline(res):
     flops = _flops_per_pair(dim) * pairs
     gbs = bytes_alg / t_force / 1e9
     tflops = flops / t_force / 1e12
+    t_attain = max(bytes_alg / (HBM_PEAK_GBS * 1e9), flops / (FP64_PEAK_TFLOPS * 1e12))
     return {
-        "bound": "hbm",
+        "bound": "mfma",
+        "bound_note": "FP64 compute: the pass runs on the vector ALUs (no contraction, no MFMA); "
+                      "MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
         "kernel": "k_force (fused G3+G5+G7+G8 pair loop, G3/G5/G6/G8/G9 epilogue)",
-        "achieved": round(gbs, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(gbs / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "achieved": round(tflops, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+        "traffic": measured_traffic(res["workload"]),
         "avg_launch_ms": round(tm["force_ms"] / launches, 5),
         "launches": tm["force_launches"],
-        "bytes_per_launch": bytes_alg,
-        "fp64_vector": {
-            "achieved": round(tflops, 3),
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
-            "flops_per_pair": _flops_per_pair(dim),
-            "pairs_per_launch": pairs,
+        "flops_per_pair": _flops_per_pair(dim),
+        "pairs_per_launch": pairs,
+        "bytes_alg_per_launch": bytes_alg,
+        "hbm": {
+            "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 5),
         },
-        "binding_frac": round(max(gbs / HBM_PEAK_GBS, tflops / FP64_PEAK_TFLOPS), 4),
+        "attainable_frac": round(t_attain / t_force, 4),
     }
 
 
@@ -217,7 +246,7 @@ def main():
     head = run_gpu(args.workload, args.steps, args.warmup)
     extras = {}
     if not args.no_extras:
-        for name, steps in (("ot1m", 20), ("blast128", 5), ("ensemble64", 20)):
+        for name, steps in (("ot1m", 20), ("blast128", 5), ("ensemble64", 20), ("ot128", 50), ("ot128x64", 20)):
             if name == args.workload:
                 continue
             r = run_gpu(name, steps, 2)
@@ -230,6 +259,11 @@ def main():
                 "ms_per_step": round(r["ms_per_step"], 4),
                 "pair_interactions_per_s": round(r["stats"]["pairs_force"] * steps / r["wall_s"], 1),
                 "roofline": roofline(r),
+            }
+        if "ot128" in extras and "ot128x64" in extras:
+            extras["batched_vs_plain"] = {
+                "ratio": round(extras["ot128x64"]["particle_steps_per_s"] / extras["ot128"]["particle_steps_per_s"], 2),
+                "what": "particle-steps/s of 64 OT 128x128 instances in one batch over one OT 128x128 instance alone",
             }
     line = {
         "metric": METRIC,

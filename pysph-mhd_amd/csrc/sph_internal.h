@@ -211,6 +211,178 @@ __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, in
     for (; j < b; ++j) f(j, rec[j]);
 }
 
+// ---------------------------------------------------------------------------
+// block staging of candidate records
+// ---------------------------------------------------------------------------
+// A block's particles are consecutive in cell order, so they sit in one or two
+// cell rows (y, z fixed), and every stencil range of theirs lies inside a few
+// contiguous runs of sorted records: for each of the block's rows, each
+// stencil row offset and each periodic x piece, the run from the row's first
+// x - s to its last x + s.  Wave 0 lists the runs (one per lane), the block
+// copies them into LDS with coalesced loads, and each thread then scans its
+// own ranges out of LDS.  A range is read from LDS only when a staged run
+// contains it in sorted-index space, so the copy always holds exactly the
+// records the range names; anything else (more rows, wide stencils, an
+// instance boundary, too many records) is read from global memory as before.
+constexpr int kStageCap = 1024;     // staged 32-byte records per block (32 KB)
+constexpr int kStageSegs = 2;       // cell rows a staged block may span
+constexpr int kStageSlots = kWave;  // runs: one per lane of wave 0
+
+struct StageHdr {
+    int on;
+    int nslot;   // runs in use: segments x nz x ny x 2 pieces
+    int ny, nz;  // stencil rows per block row (y, z)
+    int64_t r0;  // first cell row (y + n1 z) of the block
+    int a[kStageSlots], b[kStageSlots], off[kStageSlots + 1];
+};
+
+// stencil span of the cells c0..c1 of one axis; for c0 == c1 exactly the span
+// for_each_row_range forms
+__device__ __forceinline__ void axis_span(int c0, int c1, int sr, int na, int periodic, int& lo, int& hi) {
+    if (periodic && 2 * sr + 1 + (c1 - c0) >= na) { lo = 0; hi = na - 1; }
+    else if (periodic) { lo = c0 - sr; hi = c1 + sr; }
+    else { lo = max(c0 - sr, 0); hi = min(c1 + sr, na - 1); }
+}
+
+// the x pieces of [lo, hi] after periodic wrapping (at most two)
+__device__ __forceinline__ int x_pieces(int lo, int hi, int n0, int* xa, int* xb) {
+    if (lo < 0) { xa[0] = lo + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi; return 2; }
+    if (hi >= n0) { xa[0] = lo; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi - n0; return 2; }
+    xa[0] = lo; xb[0] = hi; return 1;
+}
+
+// called by the 64 lanes of wave 0 for the block whose first particle is t0
+template <int DIM>
+__device__ void stage_plan(StageHdr& H, const uint32_t* __restrict__ skey, const int32_t* __restrict__ cstart,
+                           const int32_t* __restrict__ cend, const Inst* __restrict__ inst,
+                           const int32_t* __restrict__ inst_of, int64_t t0, int64_t n) {
+    const int lane = threadIdx.x;
+    const int64_t tl = min(t0 + (int64_t)kBlock, n) - 1;
+    const int k = inst_of[t0];
+    const Inst& s = inst[k];
+    const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
+    const int64_t l0 = (int64_t)skey[t0] - s.cell_base, l1 = (int64_t)skey[tl] - s.cell_base;
+    const int64_t r0 = l0 / n0, r1 = l1 / n0;
+    const int ny = min(2 * s.srad[1] + 1, n1);
+    const int nz = DIM == 3 ? min(2 * s.srad[2] + 1, n2) : 1;
+    const int64_t nseg = r1 - r0 + 1;
+    bool on = inst_of[tl] == k && nseg <= kStageSegs && ny <= kStageSlots && nz <= kStageSlots &&
+              nseg * ny * nz * 2 <= kStageSlots;
+    const int nslot = on ? (int)(nseg * ny * nz * 2) : 0;
+    int a = 0, len = 0;
+    if (lane < nslot) {
+        const int p = lane & 1;
+        int rest = lane >> 1;
+        const int iy = rest % ny;
+        rest /= ny;
+        const int iz = rest % nz;
+        const int seg = rest / nz;
+        const int64_t row = r0 + seg;
+        const int cy = (int)(row % n1), cz = DIM == 3 ? (int)(row / n1) : 0;
+        const int xa = seg == 0 ? (int)(l0 % n0) : 0;
+        const int xb = seg == nseg - 1 ? (int)(l1 % n0) : n0 - 1;
+        int lo0, hi0, lo1, hi1, lo2 = 0, hi2 = 0;
+        axis_span(xa, xb, s.srad[0], n0, s.periodic, lo0, hi0);
+        axis_span(cy, cy, s.srad[1], n1, s.periodic, lo1, hi1);
+        if (DIM == 3) axis_span(cz, cz, s.srad[2], n2, s.periodic, lo2, hi2);
+        int pa[2], pb[2];
+        const int np = x_pieces(lo0, hi0, n0, pa, pb);
+        const int qy = lo1 + iy, qz = lo2 + iz;
+        if (qy <= hi1 && qz <= hi2 && p < np) {
+            const int wy = ((qy % n1) + n1) % n1;
+            const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+            const int64_t rowc = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+            a = cstart[rowc + pa[p]];
+            len = cend[rowc + pb[p]] - a;
+        }
+    }
+    int incl = len;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    const int total = __shfl(incl, kWave - 1);
+    H.a[lane] = a;
+    H.b[lane] = a + len;
+    H.off[lane] = incl - len;
+    if (lane == 0) {
+        H.on = on && total <= kStageCap;
+        H.nslot = nslot;
+        H.ny = ny;
+        H.nz = nz;
+        H.r0 = r0;
+        H.off[kStageSlots] = total;
+    }
+}
+
+// all threads of the block: copy the planned runs into LDS
+template <class T>
+__device__ __forceinline__ void stage_fill(const StageHdr& H, T* stage, const T* __restrict__ src) {
+    if (!H.on) return;
+    const int total = H.off[kStageSlots];
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        int lo = 0, hi = kStageSlots;  // off[lo] <= idx < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (H.off[mid] <= idx) lo = mid;
+            else hi = mid;
+        }
+        stage[idx] = src[H.a[lo] + (idx - H.off[lo])];
+    }
+}
+
+// for_each_row_range over the staged copy: f(j, rec_j) for every candidate j,
+// from LDS where a staged run holds the range
+template <int DIM, class T, class F>
+__device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
+                                                   const int32_t* __restrict__ cend, const StageHdr& H,
+                                                   const T* stage, const T* __restrict__ glob, F&& f) {
+    const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
+    const int64_t local = key - s.cell_base;
+    const int cx = (int)(local % n0);
+    const int cy = (int)((local / n0) % n1);
+    const int cz = DIM == 3 ? (int)(local / ((int64_t)n0 * n1)) : 0;
+    int lo0, hi0, lo1, hi1, lo2 = 0, hi2 = 0;
+    axis_span(cx, cx, s.srad[0], n0, s.periodic, lo0, hi0);
+    axis_span(cy, cy, s.srad[1], n1, s.periodic, lo1, hi1);
+    if (DIM == 3) axis_span(cz, cz, s.srad[2], n2, s.periodic, lo2, hi2);
+    int xa[2], xb[2];
+    const int nseg = x_pieces(lo0, hi0, n0, xa, xb);
+    const int64_t seg = local / n0 - H.r0;
+    const bool staged = H.on && seg >= 0 && seg < kStageSegs;
+    for (int qz = lo2; qz <= hi2; ++qz) {
+        const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+        for (int qy = lo1; qy <= hi1; ++qy) {
+            const int wy = ((qy % n1) + n1) % n1;
+            const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+            const int base = (((int)seg * H.nz + (qz - lo2)) * H.ny + (qy - lo1)) * 2;
+            for (int g = 0; g < nseg; ++g) {
+                const int a = cstart[row + xa[g]], b = cend[row + xb[g]];
+                int k = -1;
+                if (staged && base + 1 < H.nslot) {
+                    if (H.a[base] <= a && b <= H.b[base]) k = base;
+                    else if (H.a[base + 1] <= a && b <= H.b[base + 1]) k = base + 1;
+                }
+                if (k >= 0) {
+                    const int delta = H.off[k] - H.a[k];
+                    int j = a;
+                    for (; j + 4 <= b; j += 4) {
+                        const T o0 = stage[j + delta], o1 = stage[j + 1 + delta];
+                        const T o2 = stage[j + 2 + delta], o3 = stage[j + 3 + delta];
+                        f(j, o0);
+                        f(j + 1, o1);
+                        f(j + 2, o2);
+                        f(j + 3, o3);
+                    }
+                    for (; j < b; ++j) f(j, stage[j + delta]);
+                } else {
+                    scan_range4(glob, a, b, f);
+                }
+            }
+        }
+    }
+}
+
 // time step after a step: the pending min dt_adapt (adaptive) or the current
 // dt, clipped to tf - t (solver.py:61)
 __host__ __device__ inline double next_dt(const Inst& s, double t, double dt_now, double pending) {

@@ -366,7 +366,23 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl) {
-    const int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
+    // 2-D blocks stage their candidates in LDS (measured 7-20 % faster); in
+    // 3-D the staged runs rarely fit and the LDS would cost occupancy
+    constexpr int kCap = DIM == 2 ? kStageCap : 1;
+    __shared__ StageHdr H;
+    __shared__ double4 stage[kCap];
+    const int64_t t0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
+    if (t0 >= n) return;  // whole block past the end (the block barriers below need every wave)
+    if constexpr (DIM == 2) {
+        if (threadIdx.x < kWave) stage_plan<DIM>(H, skey, cstart, cend, inst, inst_of, t0, n);
+    } else {
+        if (threadIdx.x == 0) H.on = 0;
+    }
+    __syncthreads();
+    stage_fill(H, stage, xh);
+    __syncthreads();
+
+    const int64_t t = t0 + threadIdx.x;
     const bool valid = t < n;
     int cnt = 0;
     if (valid) {
@@ -377,18 +393,16 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
-        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b) {
-            scan_range4(xh, a, b, [&](int j, const double4& o) {
-                const double dx = min_image(xi - o.x, L0, H0, per);
-                const double dy = min_image(yi - o.y, L1, H1, per);
-                const double dz = DIM == 3 ? min_image(zi - o.z, L2, H2, per) : 0.0;
-                const double r2 = r2_exact(dx, dy, dz);
-                const double rad = R * fmax(hi, o.w);
-                if (r2 < rad * rad) {
-                    if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
-                    ++cnt;
-                }
-            });
+        for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh, [&](int j, const double4& o) {
+            const double dx = min_image(xi - o.x, L0, H0, per);
+            const double dy = min_image(yi - o.y, L1, H1, per);
+            const double dz = DIM == 3 ? min_image(zi - o.z, L2, H2, per) : 0.0;
+            const double r2 = r2_exact(dx, dy, dz);
+            const double rad = R * fmax(hi, o.w);
+            if (r2 < rad * rad) {
+                if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
+                ++cnt;
+            }
         });
         nbcnt[t] = cnt;
     }

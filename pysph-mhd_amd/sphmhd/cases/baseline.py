@@ -14,6 +14,7 @@ Particles sit at lattice cell centres (i + 1/2) dx of periodic boxes.
                    with about 60 neighbours
   ensemble_ot      64 Orszag-Tang instances of uneven size with jittered
                    field amplitude and dissipation/cleaning parameters
+  ensemble_of      a batch of independent single-instance setups
   alfven_2d        circularly polarised Alfven wave tilted by 30 degrees
                    (alfven.py), with viscosity and cleaning switched on
 """
@@ -170,6 +171,19 @@ def ensemble_ot(count: int = 64, seed: int = 0, *, sizes=None, kernel: str = "qu
     fields = {name: np.concatenate([p[name] for p in parts]) for name in parts[0]}
     return Setup(f"ensemble_ot_{count}", params, np.asarray(offsets, dtype=np.int64), fields,
                  info={"sizes": n_tab.tolist(), "b0_factor": b0f.tolist(), "jitter": jit.tolist()})
+
+
+def ensemble_of(setups) -> Setup:
+    """One batch of independent single-instance setups, in the given order."""
+    params, offsets = [], [0]
+    for s in setups:
+        if len(s.params) != 1:
+            raise ValueError("ensemble_of takes single-instance setups")
+        params.append(s.params[0])
+        offsets.append(offsets[-1] + s.n)
+    fields = {name: np.concatenate([s.fields[name] for s in setups]) for name in setups[0].fields}
+    return Setup(f"ensemble_of_{len(setups)}", params, np.asarray(offsets, dtype=np.int64), fields,
+                 info={"members": [s.name for s in setups]})
 
 
 def alfven_2d(nx: int = 97, ny: int = 168, *, amplitude: float = 0.1, kernel: str = "wendland_c6",
