@@ -225,16 +225,18 @@ __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, in
 // records the range names; anything else (more rows, wide stencils, an
 // instance boundary, too many records) is read from global memory as before.
 constexpr int kStageCap = 1024;     // staged 32-byte records per block (32 KB)
-constexpr int kStageSegs = 2;       // cell rows a staged block may span
-constexpr int kStageSlots = kWave;  // runs: one per lane of wave 0
+constexpr int kStageSegs = 2;       // cell rows a staged group may span
 
-struct StageHdr {
+// runs staged for one group of particles: a[k] <= j < b[k] sits at off[k] + j - a[k]
+template <int SLOTS>
+struct StageRuns {
     int on;
     int nslot;   // runs in use: segments x nz x ny x 2 pieces
-    int ny, nz;  // stencil rows per block row (y, z)
-    int64_t r0;  // first cell row (y + n1 z) of the block
-    int a[kStageSlots], b[kStageSlots], off[kStageSlots + 1];
+    int ny, nz;  // stencil rows per group row (y, z)
+    int64_t r0;  // first cell row (y + n1 z) of the group
+    int a[SLOTS], b[SLOTS], off[SLOTS + 1];
 };
+using StageHdr = StageRuns<kWave>;
 
 // stencil span of the cells c0..c1 of one axis; for c0 == c1 exactly the span
 // for_each_row_range forms
@@ -251,13 +253,18 @@ __device__ __forceinline__ int x_pieces(int lo, int hi, int n0, int* xa, int* xb
     xa[0] = lo; xb[0] = hi; return 1;
 }
 
-// called by the 64 lanes of wave 0 for the block whose first particle is t0
-template <int DIM>
-__device__ void stage_plan(StageHdr& H, const uint32_t* __restrict__ skey, const int32_t* __restrict__ cstart,
-                           const int32_t* __restrict__ cend, const Inst* __restrict__ inst,
-                           const int32_t* __restrict__ inst_of, int64_t t0, int64_t n) {
-    const int lane = threadIdx.x;
-    const int64_t tl = min(t0 + (int64_t)kBlock, n) - 1;
+// Plans the runs of the group of particles [t0, t0 + count) (clipped to n).
+// Called by all 64 lanes of one wave; lane k (< SLOTS) computes run k.  The
+// plan is off when the group straddles instances, spans more than kStageSegs
+// cell rows, needs more than SLOTS runs or more than cap records.
+template <int DIM, int SLOTS>
+__device__ void stage_plan(StageRuns<SLOTS>& H, const uint32_t* __restrict__ skey,
+                           const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                           const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of, int64_t t0,
+                           int64_t count, int64_t n, int cap) {
+    static_assert(SLOTS <= kWave, "one run per lane");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t tl = min(t0 + count, n) - 1;
     const int k = inst_of[t0];
     const Inst& s = inst[k];
     const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
@@ -266,8 +273,8 @@ __device__ void stage_plan(StageHdr& H, const uint32_t* __restrict__ skey, const
     const int ny = min(2 * s.srad[1] + 1, n1);
     const int nz = DIM == 3 ? min(2 * s.srad[2] + 1, n2) : 1;
     const int64_t nseg = r1 - r0 + 1;
-    bool on = inst_of[tl] == k && nseg <= kStageSegs && ny <= kStageSlots && nz <= kStageSlots &&
-              nseg * ny * nz * 2 <= kStageSlots;
+    const bool on = inst_of[tl] == k && nseg <= kStageSegs && ny <= SLOTS && nz <= SLOTS &&
+                    nseg * ny * nz * 2 <= SLOTS;
     const int nslot = on ? (int)(nseg * ny * nz * 2) : 0;
     int a = 0, len = 0;
     if (lane < nslot) {
@@ -302,26 +309,28 @@ __device__ void stage_plan(StageHdr& H, const uint32_t* __restrict__ skey, const
         if (lane >= d) incl += v;
     }
     const int total = __shfl(incl, kWave - 1);
-    H.a[lane] = a;
-    H.b[lane] = a + len;
-    H.off[lane] = incl - len;
+    if (lane < SLOTS) {
+        H.a[lane] = a;
+        H.b[lane] = a + len;
+        H.off[lane] = incl - len;
+    }
     if (lane == 0) {
-        H.on = on && total <= kStageCap;
+        H.on = on && total <= cap;
         H.nslot = nslot;
         H.ny = ny;
         H.nz = nz;
         H.r0 = r0;
-        H.off[kStageSlots] = total;
+        H.off[SLOTS] = total;
     }
 }
 
 // all threads of the block: copy the planned runs into LDS
-template <class T>
-__device__ __forceinline__ void stage_fill(const StageHdr& H, T* stage, const T* __restrict__ src) {
+template <class T, int SLOTS>
+__device__ __forceinline__ void stage_fill(const StageRuns<SLOTS>& H, T* stage, const T* __restrict__ src) {
     if (!H.on) return;
-    const int total = H.off[kStageSlots];
+    const int total = H.off[SLOTS];
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int lo = 0, hi = kStageSlots;  // off[lo] <= idx < off[hi]
+        int lo = 0, hi = SLOTS;  // off[lo] <= idx < off[hi]
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (H.off[mid] <= idx) lo = mid;

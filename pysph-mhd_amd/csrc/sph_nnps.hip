@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
     const int64_t t0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
     if (t0 >= n) return;  // whole block past the end (the block barriers below need every wave)
     if constexpr (DIM == 2) {
-        if (threadIdx.x < kWave) stage_plan<DIM>(H, skey, cstart, cend, inst, inst_of, t0, n);
+        if (threadIdx.x < kWave) stage_plan<DIM, kWave>(H, skey, cstart, cend, inst, inst_of, t0, kBlock, n, kStageCap);
     } else {
         if (threadIdx.x == 0) H.on = 0;
     }
