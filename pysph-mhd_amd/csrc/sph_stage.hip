@@ -126,17 +126,119 @@ __global__ void k_stage2(Fields F, const Inst* __restrict__ inst, const int32_t*
 }
 
 // stage2 of step n, then initialize and stage1 of step n+1 with the step the
-// clock kernel is about to commit
+// clock kernel is about to commit.  The same arithmetic as do_stage2,
+// do_initialize, do_stage1 and do_wrap in that order, expression for
+// expression (so it rounds identically), but every field is loaded once into
+// registers and every output stored once: through the property pointers the
+// compiler cannot tell the fields apart and would reload and re-store them.
 __global__ void k_stage2_init_stage1(Fields F, const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                      int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Inst& s = inst[inst_of[i]];
-    do_stage2(F, s, i, s.dt);
-    const double dt_new = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
-    do_initialize(F, i);
-    do_stage1(F, s, i, dt_new);
-    do_wrap(F, s, i);
+    const double dt = s.dt;
+    const double dtn = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
+    const bool fluid = F.itype[i] == 1;
+    const bool has_z = F.f[SPHMHD_F_Z] != nullptr, has_z0 = F.f[SPHMHD_F_Z0] != nullptr;
+
+    double u = FLD(U), v = FLD(V), w = FLD(W), e = FLD(E);
+    double bx = FLD(BEVOLX), by = FLD(BEVOLY), bz = FLD(BEVOLZ), psi = FLD(PSI);
+    const double au = FLD(AU), av = FLD(AV), aw = FLD(AW), ae = FLD(AE);
+    const double abx = FLD(ABEVOLX), aby = FLD(ABEVOLY), abz = FLD(ABEVOLZ), apsi = FLD(APSI);
+    double x = FLD(X), y = FLD(Y), z = has_z ? FLD(Z) : 0.0;
+    double h = FLD(H), alpha = FLD(ALPHA1);
+
+    // stage2 (step n)
+    if (fluid) {
+        const double dtb2 = 0.5 * dt;
+        u += dtb2 * au;
+        v += dtb2 * av;
+        w += dtb2 * aw;
+        e = e + dtb2 * ae;
+        const double rho = FLD(RHO);
+        FLD(P) = s.gamma1 * rho * e;
+        bx = bx + dtb2 * abx;
+        by = by + dtb2 * aby;
+        bz = bz + dtb2 * abz;
+        psi += dtb2 * apsi;
+        FLD(BX) = bx * rho;
+        FLD(BY) = by * rho;
+        FLD(BZ) = bz * rho;
+    }
+    // initialize (step n+1)
+    const double x0 = x, y0 = y, u0 = u, v0 = v, w0 = w, e0 = e, h0 = h, a10 = alpha;
+    const double bx0 = bx, by0 = by, bz0 = bz, psi0 = psi;
+    FLD(X0) = x0;
+    FLD(Y0) = y0;
+    if (has_z0) FLD(Z0) = z;
+    FLD(U0) = u0;
+    FLD(V0) = v0;
+    FLD(W0) = w0;
+    FLD(E0) = e0;
+    FLD(H0) = h0;
+    FLD(CONVERGED) = 0.0;
+    FLD(OMEGA) = 1.0;
+    FLD(ALPHA10) = a10;
+    FLD(BEVOLX0) = bx0;
+    FLD(BEVOLY0) = by0;
+    FLD(BEVOLZ0) = bz0;
+    FLD(PSI0) = psi0;
+    double upred = u, vpred = v, wpred = w, epred = e;
+    double bxp = bx, byp = by, bzp = bz, psip = psi;
+    // stage1 (step n+1)
+    if (fluid) {
+        const double dtb2 = 0.5 * dtn;
+        u = u0 + dtb2 * au;
+        v = v0 + dtb2 * av;
+        w = w0 + dtb2 * aw;
+        upred = u0 + dtn * au;
+        vpred = v0 + dtn * av;
+        wpred = w0 + dtn * aw;
+        x = x0 + dtn * u;
+        y = y0 + dtn * v;
+        if (s.dim == 3) z = z + dtn * w;  // z0 == z here
+        e = e0 + dtb2 * ae;
+        epred = e0 + dtn * ae;
+        h = h0 + dtn * FLD(AH);
+        const double decay = 0.1 * FLD(CS) / h;
+        const double al = FLD(ALPHALOC);
+        alpha = a10 < al ? al : (a10 + dtn * al * decay) / (1.0 + dtn * decay);
+        bx = bx0 + dtb2 * abx;
+        by = by0 + dtb2 * aby;
+        bz = bz0 + dtb2 * abz;
+        psi = psi0 + dtb2 * apsi;
+        bxp = bx0 + dtn * abx;
+        byp = by0 + dtn * aby;
+        bzp = bz0 + dtn * abz;
+        psip = psi0 + dtn * apsi;
+    }
+    // wrap
+    if (s.periodic) {
+        x = wrap1(x, s.lo[0], s.hi[0], s.L[0]);
+        y = wrap1(y, s.lo[1], s.hi[1], s.L[1]);
+        if (s.dim == 3) z = wrap1(z, s.lo[2], s.hi[2], s.L[2]);
+    }
+    FLD(U) = u;
+    FLD(V) = v;
+    FLD(W) = w;
+    FLD(UPRED) = upred;
+    FLD(VPRED) = vpred;
+    FLD(WPRED) = wpred;
+    FLD(X) = x;
+    FLD(Y) = y;
+    if (s.dim == 3) FLD(Z) = z;
+    FLD(E) = e;
+    FLD(EPRED) = epred;
+    FLD(H) = h;
+    FLD(ALPHA1) = alpha;
+    FLD(BEVOLX) = bx;
+    FLD(BEVOLY) = by;
+    FLD(BEVOLZ) = bz;
+    FLD(PSI) = psi;
+    FLD(BEVOLXPRED) = bxp;
+    FLD(BEVOLYPRED) = byp;
+    FLD(BEVOLZPRED) = bzp;
+    FLD(PSIPRED) = psip;
 }
 
 __global__ void k_clock(Inst* inst, int ninst) {
