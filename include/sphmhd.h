@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 1
+#define SPHMHD_ABI_VERSION 2
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -127,6 +127,8 @@ typedef struct sphmhd_stats {
     int64_t cells;               /* cells of the last neighbour-search build */
     int32_t last_iterations_max; /* most density passes any instance needed in the last eval */
     int32_t max_neighbours;      /* largest neighbour count in the last force pass */
+    int64_t force_redos;         /* force passes redone (iteration needed more passes than
+                                    were launched ahead, or a neighbour list grew) */
 } sphmhd_stats;
 
 /* Device time of the hot kernels, measured with HIP events recorded on the

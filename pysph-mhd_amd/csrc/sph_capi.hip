@@ -66,6 +66,8 @@ struct sphmhd_ctx {
     int max_iter_all = 1;
     int min_iter_all = 1;
     int spec_passes = 2;  // density passes launched before the first wait
+    int spec_margin = 0;       // one extra pass for a while after a redo
+    int evals_since_redo = 0;
     Ctl* hctl = nullptr;  // pinned readback
     sphmhd_stats stats{};
     int64_t tiles = 0;
@@ -154,12 +156,32 @@ static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions) {
     return 0;
 }
 
-// One evaluation.  The density passes are launched speculatively (as many as
-// the previous evaluation needed, at least min_iterations): a pass launched
-// after an instance has finished is a no-op for it, so the host only waits
-// once for the iteration flags.  The list overflow check is read together
-// with the end-of-evaluation status; an overflowing list is grown and only
-// the list build and force pass are re-run (their inputs are unchanged).
+// prep (G1, G2, G4 and the force pack), neighbour lists, fused force pass
+// and the warning collection; first: the first pass of an evaluation, which
+// keeps the previous accelerations a redone pass needs
+static int force_pass(sphmhd_ctx* c, bool first) {
+    Dev& d = c->d;
+    tbegin(c, T_LISTS);
+    launch_prep_force(d, first);
+    launch_nb_build(d);
+    tend(c, T_LISTS);
+    tbegin(c, T_FORCE);
+    launch_force(d);
+    tend(c, T_FORCE);
+    launch_eval_end(d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One evaluation with one host wait.  The density passes are launched
+// speculatively (as many as recent evaluations needed, at least
+// min_iterations; a pass launched after an instance has finished is a no-op
+// for it) and the force pass follows at once.  The host then reads the
+// iteration flags and the list overflow flag together.  Rarely the iteration
+// needed more passes, or a list did not fit its slots: the remaining passes
+// run (one wait each) or the lists grow, and the force pass is redone from
+// the kept previous accelerations.  After a redo one extra speculative pass
+// is launched for the next 16 evaluations.
 static int do_eval(sphmhd_ctx* c) {
     Dev& d = c->d;
     tbegin(c, T_NNPS);
@@ -168,44 +190,52 @@ static int do_eval(sphmhd_ctx* c) {
     launch_pack_density(d);
     tend(c, T_NNPS);
     int passes = 0;
-    int spec = std::max(c->spec_passes, 1);
-    for (;;) {
-        for (int q = 0; q < spec; ++q) {
-            d.pass = passes;
-            if (q == 0) tbegin(c, T_DENSITY);
+    const int spec = std::max(c->spec_passes + c->spec_margin, 1);
+    tbegin(c, T_DENSITY);
+    for (int q = 0; q < spec; ++q) {
+        d.pass = passes++;
+        launch_density(d);
+        launch_g0_control(d);
+    }
+    tend(c, T_DENSITY);
+    if ((rc = force_pass(c, true))) return rc;
+    if ((rc = read_ctl(c))) return rc;
+    bool redo = false;
+    if (!c->hctl->all_done) {
+        // the prep overwrote the {x,y,z,m} density pack with {x,y,z,h}
+        launch_pack_density(d);
+        while (!c->hctl->all_done) {
+            if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
+            tbegin(c, T_DENSITY);
+            d.pass = passes++;
             launch_density(d);
-            if (q == spec - 1) tend(c, T_DENSITY);
             launch_g0_control(d);
-            ++passes;
+            tend(c, T_DENSITY);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_ctl(c))) return rc;
         }
-        HIPCHK(hipGetLastError());
-        rc = read_ctl(c);
-        if (rc) return rc;
-        if (c->hctl->all_done) break;
-        if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
-        spec = 1;
+        redo = true;
+        c->spec_margin = 1;
+        c->evals_since_redo = 0;
+    } else if (c->spec_margin && ++c->evals_since_redo >= 16) {
+        c->spec_margin = 0;
     }
     c->spec_passes = std::max(c->hctl->iters_max, c->min_iter_all);
     c->stats.density_iterations += c->hctl->iters_max;
     c->stats.last_iterations_max = c->hctl->iters_max;
-    for (int attempt = 0;; ++attempt) {
-        tbegin(c, T_LISTS);
-        if (attempt == 0) launch_prep_force(d);
-        launch_nb_build(d);
-        tend(c, T_LISTS);
-        tbegin(c, T_FORCE);
-        launch_force(d);
-        tend(c, T_FORCE);
-        launch_eval_end(d);
-        HIPCHK(hipGetLastError());
-        rc = read_ctl(c);
-        if (rc) return rc;
-        if (!c->hctl->overflow) break;
-        if (attempt > 0) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
-        rc = grow_nb(c, c->hctl->max_nb);
-        if (rc) return rc;
+    for (int grows = 0;;) {
+        if (c->hctl->overflow) {
+            if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
+            if ((rc = grow_nb(c, c->hctl->max_nb))) return rc;
+            redo = true;
+        }
+        if (!redo) break;
         launch_reset_nb_stats(d);
         launch_reset_force_outputs(d);
+        if ((rc = force_pass(c, false))) return rc;
+        if ((rc = read_ctl(c))) return rc;
+        redo = false;
+        c->stats.force_redos += 1;
     }
     c->stats.max_neighbours = c->hctl->max_nb;
     c->stats.pairs_force = c->hctl->pairs;
@@ -349,6 +379,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.act[0], n))) return cleanup(rc);
         if ((rc = dalloc(&d.act[1], n))) return cleanup(rc);
         if ((rc = dalloc(&d.scratch3, 3 * n))) return cleanup(rc);
+        if ((rc = dalloc(&d.aprev, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
         hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocDefault);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostMalloc failed"));
@@ -370,7 +401,8 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     Dev& d = c->d;
     (void)hipStreamSynchronize(d.stream);
     void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
-                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.totals, d.act[0], d.act[1]};
+                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
+                    d.act[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);

@@ -502,6 +502,8 @@ struct Dev {
     int32_t* nb;            // nb_stride x n, column major
     int32_t* nbcnt;
     double* scratch3;       // 3 x n (warm-start B copy)
+    double* aprev;          // 3 x n: the accelerations the force pass reads as "previous",
+                            // kept by the first prep of an evaluation for a redone pass
     double* totals;         // ninst x SPHMHD_NTOTALS
     Fields fields;
 };
@@ -517,7 +519,9 @@ void launch_keys_sort_ranges(Dev& d);
 void launch_pack_density(Dev& d);
 void launch_density(Dev& d);
 void launch_g0_control(Dev& d);
-void launch_prep_force(Dev& d);
+// keep_prev: the prep reads au, av, aw from the fields and keeps a copy in
+// aprev; otherwise (a redone force pass) it reads them back from aprev
+void launch_prep_force(Dev& d, bool keep_prev);
 void launch_nb_build(Dev& d);
 void launch_force(Dev& d);
 void launch_eval_end(Dev& d);

@@ -218,7 +218,7 @@ template <int DIM>
 __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, int64_t t, const Fields& F, int32_t i,
                                                    double x, double y, double z, double h, double m, double rho,
                                                    double p, double e, double bx, double by, double bz, double vwave,
-                                                   double chf) {
+                                                   double chf, double au, double av, double aw) {
     using R = FRec<DIM>;
     double* const* f = F.f;
     double r[R::N];
@@ -237,9 +237,9 @@ __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, i
     r[R::BY] = by;
     r[R::BZ] = bz;
     r[R::AL] = f[SPHMHD_F_ALPHA1][i];
-    r[R::AU] = f[SPHMHD_F_AU][i];
-    r[R::AV] = f[SPHMHD_F_AV][i];
-    if constexpr (DIM == 3) r[R::AW] = f[SPHMHD_F_AW][i];
+    r[R::AU] = au;
+    r[R::AV] = av;
+    if constexpr (DIM == 3) r[R::AW] = aw;
     r[R::PSICH] = f[SPHMHD_F_PSIPRED][i] * (chf * vwave);
     r[R::VW] = vwave;
     double2* out = (double2*)(fpack + t * R::N);
@@ -249,7 +249,7 @@ __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, i
 
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
-                             double4* __restrict__ xh, int64_t n) {
+                             double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev) {
     int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
@@ -282,10 +282,25 @@ __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const I
     const double vwave = sqrt(cs * cs + b2 / (s.mu0 * rho));
     const double x = f[SPHMHD_F_X][i], y = f[SPHMHD_F_Y][i], z = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
     xh[t] = make_double4(x, y, z, h);
+    // the previous accelerations (the force pass overwrites them): kept on the
+    // first prep of an evaluation, read back on a redone pass
+    double au, av, aw;
+    if (keep_prev) {
+        au = f[SPHMHD_F_AU][i];
+        av = f[SPHMHD_F_AV][i];
+        aw = f[SPHMHD_F_AW][i];
+        aprev[i] = au;
+        aprev[n + i] = av;
+        aprev[2 * n + i] = aw;
+    } else {
+        au = aprev[i];
+        av = aprev[n + i];
+        aw = aprev[2 * n + i];
+    }
     if (dim == 2)
-        write_force_record<2>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf);
+        write_force_record<2>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
     else
-        write_force_record<3>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf);
+        write_force_record<3>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,11 +724,10 @@ void launch_g0_control(Dev& d) {
     hipLaunchKernelGGL(k_g0_control, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl, (d.pass + 1) & 1);
 }
 
-void launch_prep_force(Dev& d) {
+void launch_prep_force(Dev& d, bool keep_prev) {
     if (d.n == 0) return;
     hipLaunchKernelGGL(k_prep_force, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst,
-                       d.inst_of,
-                       d.fpack, (double4*)d.dpack, d.n);
+                       d.inst_of, d.fpack, (double4*)d.dpack, d.n, d.aprev, keep_prev ? 1 : 0);
 }
 
 void launch_eval_end(Dev& d) {
