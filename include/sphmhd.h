@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 2
+#define SPHMHD_ABI_VERSION 3
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -129,6 +129,7 @@ typedef struct sphmhd_stats {
     int32_t max_neighbours;      /* largest neighbour count in the last force pass */
     int64_t force_redos;         /* force passes redone (iteration needed more passes than
                                     were launched ahead, or a neighbour list grew) */
+    int64_t graph_replays;       /* steps whose launches were replayed from a captured hipGraph */
 } sphmhd_stats;
 
 /* Device time of the hot kernels, measured with HIP events recorded on the

@@ -2,14 +2,16 @@
 //
 // The host side only sequences launches on the context's stream; every
 // per-particle and per-pair operation runs in the kernels of sph_nnps.hip,
-// sph_pairs.hip and sph_stage.hip.  Host synchronisation happens once per
-// density iteration (the reference also blocks there: equations.py:91-94
-// pulls equation_has_converged), once after the neighbour-list build (to
-// grow the list if a particle has more neighbours than its slots) and once
-// at the end of an evaluation to collect warnings.
+// sph_pairs.hip and sph_stage.hip.  An evaluation enqueues the neighbour
+// search, speculative density passes and the force pass without waiting,
+// then waits once (the reference also blocks per iteration:
+// equations.py:91-94 pulls equation_has_converged).  Inside sphmhd_step the
+// enqueue-only part of every step after the first is captured once as a
+// hipGraph and replayed.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -56,6 +58,11 @@ struct PhaseTimer {
     int64_t count = 0;
 };
 
+// the enqueue-only part of a step, in timed phases: stages, neighbour
+// search, density passes, prep + lists, force pass, then the tail (warning
+// collection and the control-block copy)
+enum { P_STAGE = 0, P_NNPS, P_DENSITY, P_LISTS, P_FORCE, P_TAIL, kSegs };
+
 struct sphmhd_ctx {
     Dev d{};
     bool timing = false;
@@ -71,16 +78,41 @@ struct sphmhd_ctx {
     Ctl* hctl = nullptr;  // pinned readback
     sphmhd_stats stats{};
     int64_t tiles = 0;
+    // captured steps, one per speculative pass count; generation bumps on
+    // bind, list growth and timing changes invalidate them
+    // With timing on, the step is captured as one graph per timed phase
+    // (launched between HIP events, which are not captured); otherwise as
+    // one graph.
+    struct StepGraph {
+        hipGraphExec_t seg[kSegs] = {};
+        int nseg = 0;
+        int spec = -1;
+        uint64_t gen = 0;
+        bool timed = false;
+    };
+    std::vector<StepGraph> graphs;
+    uint64_t graph_gen = 1;
+    bool use_graphs = true;
+    bool capturing = false;
+    // Multi-step runs go on a stream of the library's own (a capture cannot
+    // use the legacy default stream the caller may pass), ordered after the
+    // caller's stream by an event and synchronised before returning.
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev = nullptr;
 };
 
 enum { T_FORCE = 0, T_DENSITY, T_NNPS, T_LISTS, T_STAGE };
 
+// Inside a stream capture the records become event nodes of the graph
+// (hipEventRecordExternal), re-recorded at every replay.
+// no records while a graph is being captured: timed graphs are cut at the
+// phase boundaries and the events go between their launches
 static void tbegin(sphmhd_ctx* c, int k) {
-    if (c->timing && !c->tm[k].pending) (void)hipEventRecord(c->tm[k].a, c->d.stream);
+    if (c->timing && !c->capturing && !c->tm[k].pending) (void)hipEventRecord(c->tm[k].a, c->d.stream);
 }
 
 static void tend(sphmhd_ctx* c, int k) {
-    if (c->timing && !c->tm[k].pending) {
+    if (c->timing && !c->capturing && !c->tm[k].pending) {
         (void)hipEventRecord(c->tm[k].b, c->d.stream);
         c->tm[k].pending = true;
     }
@@ -98,6 +130,9 @@ static void tcollect(sphmhd_ctx* c) {
         }
         t.pending = false;
     }
+    // a failed query only drops that sample: keep it out of the sticky error
+    // state that launch checks read
+    (void)hipGetLastError();
 }
 
 static int read_ctl(sphmhd_ctx* c) {
@@ -129,6 +164,7 @@ static int grow_nb(sphmhd_ctx* c, int need) {
     int rc = dalloc(&c->d.nb, (int64_t)stride * c->d.n);
     if (rc) return rc;
     c->d.nb_stride = stride;
+    c->graph_gen += 1;
     return 0;
 }
 
@@ -176,30 +212,78 @@ static int force_pass(sphmhd_ctx* c, bool first) {
 // One evaluation with one host wait.  The density passes are launched
 // speculatively (as many as recent evaluations needed, at least
 // min_iterations; a pass launched after an instance has finished is a no-op
-// for it) and the force pass follows at once.  The host then reads the
-// iteration flags and the list overflow flag together.  Rarely the iteration
-// needed more passes, or a list did not fit its slots: the remaining passes
-// run (one wait each) or the lists grow, and the force pass is redone from
-// the kept previous accelerations.  After a redo one extra speculative pass
-// is launched for the next 16 evaluations.
-static int do_eval(sphmhd_ctx* c) {
+// for it) and the force pass follows at once (eval_front, enqueue only).  The
+// host then reads the iteration flags and the list overflow flag together
+// (eval_finish).  Rarely the iteration needed more passes, or a list did not
+// fit its slots: the remaining passes run (one wait each) or the lists grow,
+// and the force pass is redone from the kept previous accelerations.  After a
+// redo one extra speculative pass is launched for the next 16 evaluations.
+static int next_spec(const sphmhd_ctx* c) { return std::max(c->spec_passes + c->spec_margin, 1); }
+
+static const int kPhaseTimer[kSegs] = {T_STAGE, T_NNPS, T_DENSITY, T_LISTS, T_FORCE, -1};
+
+// the launches of one phase of the enqueue-only step front
+static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
     Dev& d = c->d;
-    tbegin(c, T_NNPS);
-    int rc = nnps_rebuild(c, true);
-    if (rc) return rc;
-    launch_pack_density(d);
-    tend(c, T_NNPS);
-    int passes = 0;
-    const int spec = std::max(c->spec_passes + c->spec_margin, 1);
-    tbegin(c, T_DENSITY);
-    for (int q = 0; q < spec; ++q) {
-        d.pass = passes++;
-        launch_density(d);
-        launch_g0_control(d);
+    switch (ph) {
+    case P_STAGE:
+        if (first) {
+            launch_initialize(d);
+            launch_stage1(d);
+        } else {
+            launch_stage2_init_stage1(d);
+            launch_clock(d);
+        }
+        break;
+    case P_NNPS: {
+        const int rc = nnps_rebuild(c, true);
+        if (rc) return rc;
+        launch_pack_density(d);
+        break;
     }
-    tend(c, T_DENSITY);
-    if ((rc = force_pass(c, true))) return rc;
-    if ((rc = read_ctl(c))) return rc;
+    case P_DENSITY:
+        for (int q = 0; q < spec; ++q) {
+            d.pass = q;
+            launch_density(d);
+            launch_g0_control(d);
+        }
+        break;
+    case P_LISTS:
+        launch_prep_force(d, true);
+        launch_nb_build(d);
+        break;
+    case P_FORCE:
+        launch_force(d);
+        break;
+    default:
+        launch_eval_end(d);
+        HIPCHK(hipMemcpyAsync(c->hctl, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, d.stream));
+        break;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// phases [from, kSegs) with their timers
+static int front_range(sphmhd_ctx* c, int from, bool first, int spec) {
+    for (int ph = from; ph < kSegs; ++ph) {
+        const int k = kPhaseTimer[ph];
+        if (k >= 0) tbegin(c, k);
+        const int rc = front_phase(c, ph, first, spec);
+        if (rc) return rc;
+        if (k >= 0) tend(c, k);
+    }
+    return 0;
+}
+
+static int eval_front(sphmhd_ctx* c, int spec) { return front_range(c, P_NNPS, false, spec); }
+
+static int eval_finish(sphmhd_ctx* c, int spec) {
+    Dev& d = c->d;
+    HIPCHK(hipStreamSynchronize(d.stream));
+    tcollect(c);
+    int rc = 0;
+    int passes = spec;
     bool redo = false;
     if (!c->hctl->all_done) {
         // the prep overwrote the {x,y,z,m} density pack with {x,y,z,h}
@@ -244,6 +328,97 @@ static int do_eval(sphmhd_ctx* c) {
     int st = c->hctl->status;
     if (st & 2) return SPHMHD_WNAN;
     if (st & 1) return SPHMHD_WNOCONV;
+    return 0;
+}
+
+static int do_eval(sphmhd_ctx* c) {
+    const int spec = next_spec(c);
+    int rc = eval_front(c, spec);
+    if (rc) return rc;
+    return eval_finish(c, spec);
+}
+
+// enqueue-only part of one step: the stages (the first step of a run starts
+// with initialize + stage1, later ones with the fused stage2 + initialize +
+// stage1 and the clock) and the evaluation front
+static int step_front(sphmhd_ctx* c, bool first, int spec) { return front_range(c, P_STAGE, first, spec); }
+
+static void release(sphmhd_ctx::StepGraph& g) {
+    for (hipGraphExec_t& e : g.seg)
+        if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+    g.nseg = 0;
+}
+
+static void drop_graphs(sphmhd_ctx* c) {
+    for (auto& g : c->graphs) release(g);
+    c->graphs.clear();
+}
+
+// captures phases [a, b) of a later step into one executable graph
+static int capture_phases(sphmhd_ctx* c, int a, int b, int spec, hipGraphExec_t* out) {
+    *out = nullptr;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamBeginCapture(c->d.stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = 0;
+    for (int ph = a; ph < b && rc == 0; ++ph) rc = front_phase(c, ph, false, spec);
+    c->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->d.stream, &graph);
+    const hipError_t ei = (rc == 0 && e == hipSuccess) ? hipGraphInstantiate(out, graph, nullptr, nullptr, 0)
+                                                       : hipErrorUnknown;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc || e != hipSuccess || ei != hipSuccess) {
+        std::fprintf(stderr, "sphmhd: step graph capture failed (%s; end capture: %s; instantiate: %s); "
+                             "using plain launches\n",
+                     rc ? g_err.c_str() : "launches ok", hipGetErrorString(e), hipGetErrorString(ei));
+        (void)hipGetLastError();
+        if (*out) (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
+        return 1;
+    }
+    return 0;
+}
+
+// replays the captured front of a later step, capturing it first when no
+// graph for this pass count, timing mode and generation exists; falls back
+// to plain launches (and stops using graphs) if a capture fails
+static int step_front_graph(sphmhd_ctx* c, int spec) {
+    sphmhd_ctx::StepGraph* g = nullptr;
+    for (auto& e : c->graphs)
+        if (e.spec == spec && e.gen == c->graph_gen && e.timed == c->timing) g = &e;
+    if (!g) {
+        for (auto& e : c->graphs)
+            if (e.gen != c->graph_gen) release(e);
+        c->graphs.erase(std::remove_if(c->graphs.begin(), c->graphs.end(),
+                                       [](const sphmhd_ctx::StepGraph& e) { return e.nseg == 0; }),
+                        c->graphs.end());
+        sphmhd_ctx::StepGraph ng;
+        ng.spec = spec;
+        ng.gen = c->graph_gen;
+        ng.timed = c->timing;
+        ng.nseg = ng.timed ? kSegs : 1;
+        for (int q = 0; q < ng.nseg; ++q) {
+            const int a = ng.timed ? q : 0, b = ng.timed ? q + 1 : kSegs;
+            if (capture_phases(c, a, b, spec, &ng.seg[q])) {
+                release(ng);
+                c->use_graphs = false;
+                return step_front(c, false, spec);
+            }
+        }
+        c->graphs.push_back(ng);
+        g = &c->graphs.back();
+    }
+    for (int q = 0; q < g->nseg; ++q) {
+        const int k = g->timed ? kPhaseTimer[q] : -1;
+        if (k >= 0) tbegin(c, k);
+        HIPCHK(hipGraphLaunch(g->seg[q], c->d.stream));
+        if (k >= 0) tend(c, k);
+    }
+    c->stats.graph_replays += 1;
     return 0;
 }
 
@@ -299,6 +474,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         d.dim = p0.dim;
         d.kernel = p0.kernel;
         d.stream = (hipStream_t)hip_stream;
+        if (const char* g = std::getenv("SPHMHD_GRAPHS")) c->use_graphs = std::strcmp(g, "0") != 0;
         int64_t cap = 0;
         for (int k = 0; k < n_instances; ++k) {
             int64_t nk = offsets[k + 1] - offsets[k];
@@ -400,6 +576,9 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (!c) return 0;
     Dev& d = c->d;
     (void)hipStreamSynchronize(d.stream);
+    drop_graphs(c);
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
+    if (c->gev) (void)hipEventDestroy(c->gev);
     void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
                     d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
@@ -423,6 +602,7 @@ int sphmhd_bind(sphmhd_ctx* c, double* const* fields, int32_t* itype) {
     }
     c->d.fields.itype = itype;
     c->bound = true;
+    c->graph_gen += 1;
     return 0;
 }
 
@@ -467,6 +647,20 @@ int sphmhd_advance_clock(sphmhd_ctx* c) {
     return 0;
 }
 
+// runs the steps of one sphmhd_step call on the library's stream when graphs
+// are in use, and restores the caller's stream afterwards
+struct StreamSwap {
+    sphmhd_ctx* c;
+    hipStream_t user;
+    bool on = false;
+    ~StreamSwap() {
+        if (on) {
+            (void)hipStreamSynchronize(c->d.stream);
+            c->d.stream = user;
+        }
+    }
+};
+
 int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
     int rc = check_ready(c);
     if (rc) return rc;
@@ -474,18 +668,22 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
     if (nsteps == 0) return 0;
     int worst = 0;
     try {
-        for (int64_t s = 0; s < nsteps; ++s) {
-            tbegin(c, T_STAGE);
-            if (s == 0) {
-                launch_initialize(c->d);
-                launch_stage1(c->d);
-            } else {
-                launch_stage2_init_stage1(c->d);
-                launch_clock(c->d);
+        StreamSwap swap{c, c->d.stream};
+        if (c->use_graphs && nsteps > 1) {
+            if (!c->gstream) {
+                HIPCHK(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&c->gev, hipEventDisableTiming));
             }
-            tend(c, T_STAGE);
-            HIPCHK(hipGetLastError());
-            rc = do_eval(c);
+            HIPCHK(hipEventRecord(c->gev, c->d.stream));
+            HIPCHK(hipStreamWaitEvent(c->gstream, c->gev, 0));
+            c->d.stream = c->gstream;
+            swap.on = true;
+        }
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int spec = next_spec(c);
+            rc = (s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec) : step_front(c, s == 0, spec);
+            if (rc < 0) return rc;
+            rc = eval_finish(c, spec);
             if (rc < 0) return rc;
             if (rc > worst) worst = rc;
         }
@@ -661,6 +859,7 @@ int sphmhd_set_timing(sphmhd_ctx* c, int32_t enable) {
         t.pending = false;
     }
     c->timing = enable != 0;
+    c->graph_gen += 1;  // captured steps record events only when timing was on
     return 0;
 }
 

@@ -473,12 +473,14 @@ void launch_grid(Dev& d) {
 
 void launch_keys_sort_ranges(Dev& d) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(k_keys, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.keys,
-                       d.perm, d.n);
+    // the keys start in whichever buffer makes the last pass land in d.keys,
+    // so the buffers (and a captured graph's arguments) never swap
+    const int passes = (d.key_bits + 7) / 8;
+    uint32_t *ka = (passes & 1) ? d.keys_alt : d.keys, *kb = (passes & 1) ? d.keys : d.keys_alt;
+    int32_t *va = (passes & 1) ? d.perm_alt : d.perm, *vb = (passes & 1) ? d.perm : d.perm_alt;
+    hipLaunchKernelGGL(k_keys, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, ka, va, d.n);
     int64_t tiles = (d.n + RS_TILE - 1) / RS_TILE;
     unsigned blocks = (unsigned)((tiles + RS_WAVES - 1) / RS_WAVES);
-    uint32_t *ka = d.keys, *kb = d.keys_alt;
-    int32_t *va = d.perm, *vb = d.perm_alt;
     for (int shift = 0; shift < d.key_bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(blocks), dim3(kBlock), 0, d.stream, ka, d.n, shift, d.rs_hist, tiles,
                            d.ctl);
@@ -489,10 +491,6 @@ void launch_keys_sort_ranges(Dev& d) {
                            d.rs_hist, d.rs_base, tiles, d.ctl);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
-    }
-    if (ka != d.keys) {  // odd number of passes: the result sits in the alternates
-        uint32_t* tk = d.keys; d.keys = d.keys_alt; d.keys_alt = tk;
-        int32_t* tv = d.perm; d.perm = d.perm_alt; d.perm_alt = tv;
     }
     hipLaunchKernelGGL(k_cell_bounds, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.keys, d.cstart, d.cend, d.n,
                        d.ctl, d.cell_cap);

@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -95,6 +95,7 @@ class Stats(ctypes.Structure):
         ("last_iterations_max", ctypes.c_int32),
         ("max_neighbours", ctypes.c_int32),
         ("force_redos", ctypes.c_int64),
+        ("graph_replays", ctypes.c_int64),
     ]
 
 

@@ -130,6 +130,37 @@ def test_fused_multistep_equals_single_steps(gpu):
     for nm in STATE + RATES:
         assert np.array_equal(a.get(nm), b.get(nm)), nm
     assert np.array_equal(a.clock()[0], b.clock()[0])
+    assert a.stats()["graph_replays"] == 5 and b.stats()["graph_replays"] == 0
+
+
+def test_graph_replays_equal_plain_launches(gpu, monkeypatch):
+    """Steps replayed from the captured hipGraph against the same steps
+    launched one by one (SPHMHD_GRAPHS=0), bit for bit, over several rounds:
+    each round replays the graph with the host's waits and the eager first
+    step of the round between replays, and a timing toggle recaptures it."""
+    from sphmhd.device import DeviceSystem
+
+    s = _ot(48)
+    monkeypatch.setenv("SPHMHD_GRAPHS", "0")
+    plain = DeviceSystem(s.params, s.offsets)
+    monkeypatch.setenv("SPHMHD_GRAPHS", "1")
+    graph = DeviceSystem(s.params, s.offsets)
+    for d in (plain, graph):
+        d.set_fields(init_fields(s))
+        d.warm_start()
+    for rnd in range(4):
+        if rnd == 2:
+            graph.set_timing(True)
+            plain.set_timing(True)
+        plain.step(5)
+        graph.step(5)
+        for nm in STATE + RATES:
+            assert np.array_equal(plain.get(nm), graph.get(nm)), (rnd, nm)
+        assert np.array_equal(plain.clock()[0], graph.clock()[0])
+    assert plain.stats()["graph_replays"] == 0
+    assert graph.stats()["graph_replays"] == 16
+    tm = graph.timing()
+    assert tm["force_launches"] >= 10 and tm["force_ms"] > 0.0
 
 
 @pytest.mark.parametrize("kernel", ["cubic", "wendland_c2", "wendland_c4", "wendland_c6", "gaussian",
