@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 3
+#define SPHMHD_ABI_VERSION 4
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -225,6 +225,45 @@ int sphmhd_get_timing(sphmhd_ctx* ctx, sphmhd_timing* out);
 
 /* Block until the context's stream is idle. */
 int sphmhd_synchronize(sphmhd_ctx* ctx);
+
+/* ---- Tangent-linear step -------------------------------------------------
+ * No counterpart in the reference, which has no derivatives; it serves the
+ * gradients of final-time energies with respect to the initial state and the
+ * parameters below.  Every field carries SPHMHD_NTANGENTS directional
+ * derivatives beside its value; one tangent step advances values and
+ * derivatives together (forward mode), taking exactly the primal step's
+ * branches: the density Newton iteration as run, neighbour sets and cells
+ * held fixed, the adaptive dt through the particle attaining the minimum.
+ * Masses carry no tangent. */
+#define SPHMHD_NTANGENTS 3
+
+enum sphmhd_tparam {
+    SPHMHD_TP_ARTIFICIAL_VISCOSITY = 0,
+    SPHMHD_TP_ARTIFICIAL_THERMAL_CONDUCTIVITY,
+    SPHMHD_TP_ARTIFICIAL_MAGNETIC_DISSIPATION,
+    SPHMHD_TP_CLEANING_SPEED_FACTOR,
+    SPHMHD_TP_CLEANING_DAMPING_FACTOR,
+    SPHMHD_TP_ALPHA_MAX,
+    SPHMHD_NTPARAMS
+};
+
+/* Tangent rows: SPHMHD_NTANGENTS * SPHMHD_NFIELDS device pointers,
+ * direction-major (tangent d of field f at [d * SPHMHD_NFIELDS + f]), each n
+ * doubles in storage order, borrowed like the bound fields; null where the
+ * bound field is null.  The caller sets the initial tangents. */
+int sphmhd_tangent_bind(sphmhd_ctx* ctx, double* const* tangent_fields);
+/* Parameter tangents, host array of n_instances * SPHMHD_NTPARAMS *
+ * SPHMHD_NTANGENTS doubles (d param / d direction); also zeroes the tangents
+ * of the current and pending dt. */
+int sphmhd_tangent_seed(sphmhd_ctx* ctx, const double* param_tangents);
+/* sphmhd_warm_start / sphmhd_step over values and tangents. */
+int sphmhd_tangent_warm_start(sphmhd_ctx* ctx);
+int sphmhd_tangent_step(sphmhd_ctx* ctx, int64_t nsteps);
+/* Totals with tangents, host array n_instances * SPHMHD_NTOTALS *
+ * (1 + SPHMHD_NTANGENTS): the value, then its tangents. */
+int sphmhd_tangent_totals(sphmhd_ctx* ctx, double* out);
+/* Tangents of the current dt, host array n_instances * SPHMHD_NTANGENTS. */
+int sphmhd_tangent_get_dt(sphmhd_ctx* ctx, double* dt_tangent);
 
 #ifdef __cplusplus
 }

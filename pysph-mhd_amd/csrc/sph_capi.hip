@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -99,6 +100,9 @@ struct sphmhd_ctx {
     // caller's stream by an event and synchronised before returning.
     hipStream_t gstream = nullptr;
     hipEvent_t gev = nullptr;
+    // tangent-linear step
+    TanDev tg;
+    bool tbound = false;
 };
 
 enum { T_FORCE = 0, T_DENSITY, T_NNPS, T_LISTS, T_STAGE };
@@ -577,6 +581,8 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     Dev& d = c->d;
     (void)hipStreamSynchronize(d.stream);
     drop_graphs(c);
+    for (void* p : {(void*)c->tg.itan, (void*)c->tg.aprev, (void*)c->tg.bsave, (void*)c->tg.totals})
+        if (p) (void)hipFree(p);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
     void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
@@ -836,6 +842,178 @@ int sphmhd_neighbours(sphmhd_ctx* c, int64_t* offs, int32_t* idx, int64_t cap, i
 int sphmhd_get_stats(sphmhd_ctx* c, sphmhd_stats* out) {
     if (!c || !out) return fail(SPHMHD_EINVAL, "null argument");
     *out = c->stats;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tangent-linear step
+// ---------------------------------------------------------------------------
+
+static int t_check(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->tbound) return fail(SPHMHD_ESTATE, "no tangent rows bound");
+    return 0;
+}
+
+// one evaluation over values and tangents: the primal neighbour search and
+// list build on the values, dual density passes (one wait each), dual prep,
+// the list (grown if needed), the dual force pass and the dt tangent
+static int t_eval(sphmhd_ctx* c) {
+    Dev& d = c->d;
+    TanDev& g = c->tg;
+    int rc = nnps_rebuild(c, true);
+    if (rc) return rc;
+    launch_t_eval_begin(d, g);
+    launch_pack_density(d);
+    int passes = 0;
+    for (;;) {
+        d.pass = passes++;
+        launch_t_density(d, g);
+        launch_g0_control(d);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_ctl(c))) return rc;
+        if (c->hctl->all_done) break;
+        if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
+    }
+    c->stats.density_iterations += c->hctl->iters_max;
+    c->stats.last_iterations_max = c->hctl->iters_max;
+    launch_t_prep(d, g);
+    for (int grows = 0;;) {
+        launch_reset_nb_stats(d);
+        launch_nb_build(d);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_ctl(c))) return rc;
+        if (!c->hctl->overflow) break;
+        if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
+        if ((rc = grow_nb(c, c->hctl->max_nb))) return rc;
+    }
+    launch_t_force(d, g);
+    launch_eval_end(d);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_ctl(c))) return rc;
+    c->stats.max_neighbours = c->hctl->max_nb;
+    c->stats.pairs_force = c->hctl->pairs;
+    c->stats.cells = c->hctl->total_cells;
+    c->stats.evals += 1;
+    const int st = c->hctl->status;
+    if (st & 2) return SPHMHD_WNAN;
+    if (st & 1) return SPHMHD_WNOCONV;
+    return 0;
+}
+
+int sphmhd_tangent_bind(sphmhd_ctx* c, double* const* tangent_fields) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!tangent_fields) return fail(SPHMHD_EINVAL, "null argument");
+    Dev& d = c->d;
+    TanDev& g = c->tg;
+    for (int q = 0; q < kTanDirs; ++q)
+        for (int f = 0; f < SPHMHD_NFIELDS; ++f) {
+            double* p = tangent_fields[q * SPHMHD_NFIELDS + f];
+            if (!p && d.fields.f[f]) return fail(SPHMHD_EINVAL, "tangent row " + std::to_string(f) + " is null");
+            g.tf.f[q][f] = p;
+        }
+    if (!g.itan) {
+        if ((rc = dalloc(&g.itan, d.ninst))) return rc;
+        if ((rc = dalloc(&g.aprev, (int64_t)3 * (1 + kTanDirs) * d.n))) return rc;
+        if ((rc = dalloc(&g.bsave, (int64_t)3 * (1 + kTanDirs) * d.n))) return rc;
+        if ((rc = dalloc(&g.totals, (int64_t)d.ninst * SPHMHD_NTOTALS * (1 + kTanDirs)))) return rc;
+    }
+    c->tbound = true;
+    return 0;
+}
+
+int sphmhd_tangent_seed(sphmhd_ctx* c, const double* param_tangents) {
+    int rc = t_check(c);
+    if (rc) return rc;
+    if (!param_tangents) return fail(SPHMHD_EINVAL, "null argument");
+    std::vector<InstTan> h(c->d.ninst);
+    for (int k = 0; k < c->d.ninst; ++k) {
+        InstTan& it = h[k];
+        std::memset(&it, 0, sizeof(it));
+        it.argmin = INT32_MAX;
+        for (int p = 0; p < SPHMHD_NTPARAMS; ++p)
+            for (int q = 0; q < kTanDirs; ++q)
+                it.par[p][q] = param_tangents[((int64_t)k * SPHMHD_NTPARAMS + p) * kTanDirs + q];
+    }
+    HIPCHK(hipMemcpyAsync(c->tg.itan, h.data(), sizeof(InstTan) * h.size(), hipMemcpyHostToDevice, c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+int sphmhd_tangent_warm_start(sphmhd_ctx* c) {
+    int rc = t_check(c);
+    if (rc) return rc;
+    Dev& d = c->d;
+    TanDev& g = c->tg;
+    const int64_t blk = (int64_t)(1 + kTanDirs) * d.n;
+    const int bf[3] = {SPHMHD_F_BX, SPHMHD_F_BY, SPHMHD_F_BZ};
+    for (int q = 0; q < 3; ++q) launch_t_save(d, g, bf[q], g.bsave + q * blk, false);
+    launch_t_copy(d, g, SPHMHD_F_H0, SPHMHD_F_H);
+    launch_t_sync_initial(d, g);
+    rc = t_eval(c);
+    if (rc < 0) return rc;
+    int64_t bad = 0;
+    if ((rc = count_unconverged(c, &bad))) return rc;
+    if (bad) return fail(SPHMHD_ENOTCONV, "warm start: " + std::to_string(bad) + " particles did not converge");
+    for (int q = 0; q < 3; ++q) launch_t_save(d, g, bf[q], g.bsave + q * blk, true);
+    launch_t_copy(d, g, SPHMHD_F_H0, SPHMHD_F_H);
+    launch_t_sync_initial(d, g);
+    for (int r = 0; r < 2; ++r) {
+        rc = t_eval(c);
+        if (rc < 0) return rc;
+    }
+    if ((rc = count_unconverged(c, &bad))) return rc;
+    if (bad) return fail(SPHMHD_ENOTCONV, "warm start: " + std::to_string(bad) + " particles did not converge");
+    launch_warm_dt(d);
+    launch_t_warm_dt(d, g);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_ctl(c))) return rc;
+    if (c->hctl->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
+    return 0;
+}
+
+int sphmhd_tangent_step(sphmhd_ctx* c, int64_t nsteps) {
+    int rc = t_check(c);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(SPHMHD_EINVAL, "nsteps < 0");
+    int worst = 0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        launch_t_initialize(c->d, c->tg);
+        launch_t_stage1(c->d, c->tg);
+        rc = t_eval(c);
+        if (rc < 0) return rc;
+        if (rc > worst) worst = rc;
+        launch_t_stage2(c->d, c->tg);
+        launch_t_clock(c->d, c->tg);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return worst;
+}
+
+int sphmhd_tangent_totals(sphmhd_ctx* c, double* out) {
+    int rc = t_check(c);
+    if (rc) return rc;
+    if (!out) return fail(SPHMHD_EINVAL, "null argument");
+    launch_t_totals(c->d, c->tg, c->tg.totals);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->tg.totals, sizeof(double) * c->d.ninst * SPHMHD_NTOTALS * (1 + kTanDirs),
+                          hipMemcpyDeviceToHost, c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    return 0;
+}
+
+int sphmhd_tangent_get_dt(sphmhd_ctx* c, double* dt_tangent) {
+    int rc = t_check(c);
+    if (rc) return rc;
+    if (!dt_tangent) return fail(SPHMHD_EINVAL, "null argument");
+    std::vector<InstTan> h(c->d.ninst);
+    HIPCHK(hipMemcpyAsync(h.data(), c->tg.itan, sizeof(InstTan) * h.size(), hipMemcpyDeviceToHost, c->d.stream));
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    for (int k = 0; k < c->d.ninst; ++k)
+        for (int q = 0; q < kTanDirs; ++q) dt_tangent[(int64_t)k * kTanDirs + q] = h[k].dt[q];
     return 0;
 }
 

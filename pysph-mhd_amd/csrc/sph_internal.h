@@ -418,10 +418,13 @@ __host__ __device__ inline double kernel_sigma(int k, int dim) {
     return 0.0;
 }
 
-template <int K, int DIM>
-__device__ __forceinline__ void shape(double q, double& w, double& dw) {
+// W and dW/dq of the kernel shapes (w(q) scaled by sigma / h^d elsewhere);
+// generic in the scalar type so the tangent step differentiates the same
+// arithmetic
+template <int K, int DIM, class T>
+__device__ __forceinline__ void shape(T q, T& w, T& dw) {
     if constexpr (K == SPHMHD_CUBIC) {
-        double t = fmax(2.0 - q, 0.0);
+        T t = fmax(2.0 - q, 0.0);
         if (q > 1.0) {
             w = 0.25 * t * t * t;
             dw = -0.75 * t * t;
@@ -430,44 +433,44 @@ __device__ __forceinline__ void shape(double q, double& w, double& dw) {
             dw = -3.0 * q * (1.0 - 0.75 * q);
         }
     } else if constexpr (K == SPHMHD_QUINTIC) {
-        double t3 = fmax(3.0 - q, 0.0), t2 = fmax(2.0 - q, 0.0), t1 = fmax(1.0 - q, 0.0);
-        double a3 = t3 * t3, a2 = t2 * t2, a1 = t1 * t1;
-        double b3 = a3 * a3, b2 = a2 * a2, b1 = a1 * a1;
+        T t3 = fmax(3.0 - q, 0.0), t2 = fmax(2.0 - q, 0.0), t1 = fmax(1.0 - q, 0.0);
+        T a3 = t3 * t3, a2 = t2 * t2, a1 = t1 * t1;
+        T b3 = a3 * a3, b2 = a2 * a2, b1 = a1 * a1;
         w = b3 * t3 - 6.0 * (b2 * t2) + 15.0 * (b1 * t1);
         dw = -5.0 * b3 + 30.0 * b2 - 75.0 * b1;
     } else if constexpr (K == SPHMHD_WENDLAND_C2) {
-        double t = fmax(1.0 - 0.5 * q, 0.0);
-        double t3 = t * t * t;
+        T t = fmax(1.0 - 0.5 * q, 0.0);
+        T t3 = t * t * t;
         w = t3 * t * (2.0 * q + 1.0);
         dw = -5.0 * q * t3;
     } else if constexpr (K == SPHMHD_WENDLAND_C4) {
-        double t = fmax(1.0 - 0.5 * q, 0.0);
-        double t2 = t * t;
-        double t5 = t2 * t2 * t;
+        T t = fmax(1.0 - 0.5 * q, 0.0);
+        T t2 = t * t;
+        T t5 = t2 * t2 * t;
         w = t5 * t * ((35.0 / 12.0) * q * q + 3.0 * q + 1.0);
         dw = -(14.0 / 3.0) * q * (1.0 + 2.5 * q) * t5;
     } else if constexpr (K == SPHMHD_WENDLAND_C6) {
-        double t = fmax(1.0 - 0.5 * q, 0.0);
-        double t2 = t * t;
-        double t4 = t2 * t2;
-        double t7 = t4 * t2 * t;
+        T t = fmax(1.0 - 0.5 * q, 0.0);
+        T t2 = t * t;
+        T t4 = t2 * t2;
+        T t7 = t4 * t2 * t;
         w = t7 * t * (4.0 * q * q * q + 6.25 * q * q + 4.0 * q + 1.0);
         dw = -5.5 * q * t7 * (1.0 + 3.5 * q + 4.0 * q * q);
     } else if constexpr (K == SPHMHD_GAUSSIAN) {
-        double g = q < 3.0 ? exp(-q * q) : 0.0;
+        T g = q < 3.0 ? exp(-q * q) : T(0.0);
         w = g;
         dw = -2.0 * q * g;
     } else {
-        double g = q < 3.0 ? exp(-q * q) : 0.0;
+        T g = q < 3.0 ? exp(-q * q) : T(0.0);
         const double d2 = 0.5 * DIM;
         w = g * (d2 + 1.0 - q * q);
         dw = g * (-2.0 * q * (d2 + 1.0 - q * q) - 2.0 * q);
     }
 }
 
-template <int K, int DIM>
-__device__ __forceinline__ double shape_dw(double q) {
-    double w, dw;
+template <int K, int DIM, class T>
+__device__ __forceinline__ T shape_dw(T q) {
+    T w, dw;
     shape<K, DIM>(q, w, dw);
     return dw;
 }
@@ -507,6 +510,43 @@ struct Dev {
     double* totals;         // ninst x SPHMHD_NTOTALS
     Fields fields;
 };
+
+// tangent-linear step (sph_tangent.hip)
+constexpr int kTanDirs = SPHMHD_NTANGENTS;
+
+struct TFields {
+    double* f[kTanDirs][SPHMHD_NFIELDS];
+};
+
+struct InstTan {
+    double dt[kTanDirs];                   // tangent of the current dt
+    double dtp[kTanDirs];                  // tangent of the pending min dt_adapt
+    double par[SPHMHD_NTPARAMS][kTanDirs]; // parameter tangents
+    int32_t argmin;                        // storage index attaining the pending min
+    int32_t pad;
+};
+
+struct TanDev {
+    TFields tf;
+    InstTan* itan = nullptr;  // ninst
+    double* aprev = nullptr;  // 3 x (1 + kTanDirs) x n: previous accelerations and tangents
+    double* bsave = nullptr;  // 3 x (1 + kTanDirs) x n: warm-start B copy
+    double* totals = nullptr; // ninst x SPHMHD_NTOTALS x (1 + kTanDirs)
+};
+
+void launch_t_eval_begin(Dev& d, TanDev& g);
+void launch_t_density(Dev& d, TanDev& g);
+void launch_t_prep(Dev& d, TanDev& g);
+void launch_t_force(Dev& d, TanDev& g);
+void launch_t_initialize(Dev& d, TanDev& g);
+void launch_t_stage1(Dev& d, TanDev& g);
+void launch_t_stage2(Dev& d, TanDev& g);
+void launch_t_clock(Dev& d, TanDev& g);
+void launch_t_warm_dt(Dev& d, TanDev& g);
+void launch_t_sync_initial(Dev& d, TanDev& g);
+void launch_t_save(Dev& d, TanDev& g, int field, double* buf, bool restore);
+void launch_t_copy(Dev& d, TanDev& g, int dst, int src);
+void launch_t_totals(Dev& d, TanDev& g, double* out);
 
 void launch_eval_begin(Dev& d, bool reset_dt);
 void launch_pack_xh(Dev& d);

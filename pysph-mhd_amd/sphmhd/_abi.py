@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -49,6 +49,18 @@ KERNELS = {
     "gaussian": 5,
     "super_gaussian": 6,
 }
+
+# tangent-linear step: directions and differentiable parameters (sphmhd_tparam)
+NTANGENTS = 3
+TPARAMS = (
+    "artificial_viscosity",
+    "artificial_thermal_conductivity",
+    "artificial_magnetic_dissipation",
+    "cleaning_speed_factor",
+    "cleaning_damping_factor",
+    "alpha_max",
+)
+NTPARAMS = len(TPARAMS)
 
 OK, EINVAL, EHIP, ECAP, ESTATE, ENOTCONV = 0, -1, -2, -3, -4, -5
 WNOCONV, WNAN = 1, 2
@@ -215,6 +227,12 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_neighbours": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "sphmhd_get_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
         "sphmhd_synchronize": ([vp], ctypes.c_int),
+        "sphmhd_tangent_bind": ([vp, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "sphmhd_tangent_seed": ([vp, dp], ctypes.c_int),
+        "sphmhd_tangent_warm_start": ([vp], ctypes.c_int),
+        "sphmhd_tangent_step": ([vp, i64], ctypes.c_int),
+        "sphmhd_tangent_totals": ([vp, dp], ctypes.c_int),
+        "sphmhd_tangent_get_dt": ([vp, dp], ctypes.c_int),
         "sphmhd_set_timing": ([vp, i32], ctypes.c_int),
         "sphmhd_get_timing": ([vp, ctypes.POINTER(Timing)], ctypes.c_int),
     }
