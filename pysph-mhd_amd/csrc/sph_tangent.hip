@@ -270,7 +270,40 @@ __device__ __forceinline__ D3 ld_aprev(const double* __restrict__ aprev, int c, 
 // G3 + G5 + G7 + G8 pair loops and G3/G5/G6/G8/G9 epilogues (k_force over duals)
 // ---------------------------------------------------------------------------
 
-template <int DIM>
+// loads and stores of ND directions starting at direction d0
+template <int ND>
+__device__ __forceinline__ Dual<ND> ldN(const Fields& F, const TFields& G, int k, int64_t i, int d0) {
+    Dual<ND> r(F.f[k][i]);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) r.d[q] = G.f[d0 + q][k][i];
+    return r;
+}
+
+template <int ND>
+__device__ __forceinline__ void stN(const Fields& F, const TFields& G, int k, int64_t i, const Dual<ND>& x, int d0,
+                                    bool value) {
+    if (value) F.f[k][i] = x.v;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) G.f[d0 + q][k][i] = x.d[q];
+}
+
+template <int ND>
+__device__ __forceinline__ Dual<ND> parN(double v, const InstTan& it, int p, int d0) {
+    Dual<ND> r(v);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) r.d[q] = it.par[p][d0 + q];
+    return r;
+}
+
+template <int ND>
+__device__ __forceinline__ Dual<ND> ld_aprevN(const double* __restrict__ aprev, int c, int64_t n, int64_t i, int d0) {
+    Dual<ND> r(aprev[(int64_t)(c * (1 + kTanDirs)) * n + i]);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) r.d[q] = aprev[(int64_t)(c * (1 + kTanDirs) + 1 + d0 + q) * n + i];
+    return r;
+}
+
+template <int DIM, class D3>
 struct AccT {
     D3 ax, ay, az, ae, div, dtcfl;
     D3 bx, by, bz, divbs, divbd;
@@ -281,49 +314,59 @@ struct AccT {
 
 // the neighbour-side values the pair loop reads (the force-pack record's
 // contents, rebuilt from the fields with the prep's arithmetic)
+template <class D3>
 struct SideT {
     D3 x, y, z, h1, m, rho, p, e, u, v, w, om, bx, by, bz, al, au, av, aw, psich, vw, hir;
 };
 
-template <int DIM>
-__device__ __forceinline__ SideT load_side(const Fields& F, const TFields& G, const double* __restrict__ aprev,
-                                           int64_t n, int32_t i, const Inst& s, const D3& chf) {
-    SideT o;
-    o.x = ldT(F, G, TFLD(X), i);
-    o.y = ldT(F, G, TFLD(Y), i);
-    o.z = DIM == 3 ? ldT(F, G, TFLD(Z), i) : D3(0.0);
-    const D3 h = ldT(F, G, TFLD(H), i);
+template <int DIM, int ND>
+__device__ __forceinline__ SideT<Dual<ND>> load_side(const Fields& F, const TFields& G,
+                                                     const double* __restrict__ aprev, int64_t n, int32_t i,
+                                                     const Inst& s, const Dual<ND>& chf, int d0) {
+    using D3 = Dual<ND>;
+    SideT<D3> o;
+    o.x = ldN<ND>(F, G, TFLD(X), i, d0);
+    o.y = ldN<ND>(F, G, TFLD(Y), i, d0);
+    o.z = DIM == 3 ? ldN<ND>(F, G, TFLD(Z), i, d0) : D3(0.0);
+    const D3 h = ldN<ND>(F, G, TFLD(H), i, d0);
     o.h1 = 1.0 / h;
     o.m = D3(F.f[SPHMHD_F_M][i]);
-    o.rho = ldT(F, G, TFLD(RHO), i);
-    o.e = ldT(F, G, TFLD(EPRED), i);
+    o.rho = ldN<ND>(F, G, TFLD(RHO), i, d0);
+    o.e = ldN<ND>(F, G, TFLD(EPRED), i, d0);
     o.p = s.gamma1 * o.rho * o.e;
-    o.u = ldT(F, G, TFLD(UPRED), i);
-    o.v = ldT(F, G, TFLD(VPRED), i);
-    o.w = ldT(F, G, TFLD(WPRED), i);
-    o.om = ldT(F, G, TFLD(OMEGA), i);
-    o.bx = ldT(F, G, TFLD(BX), i);
-    o.by = ldT(F, G, TFLD(BY), i);
-    o.bz = ldT(F, G, TFLD(BZ), i);
-    o.al = ldT(F, G, TFLD(ALPHA1), i);
-    o.au = ld_aprev(aprev, 0, n, i);
-    o.av = ld_aprev(aprev, 1, n, i);
-    o.aw = ld_aprev(aprev, 2, n, i);
-    const D3 cs = ldT(F, G, TFLD(CS), i);
+    o.u = ldN<ND>(F, G, TFLD(UPRED), i, d0);
+    o.v = ldN<ND>(F, G, TFLD(VPRED), i, d0);
+    o.w = ldN<ND>(F, G, TFLD(WPRED), i, d0);
+    o.om = ldN<ND>(F, G, TFLD(OMEGA), i, d0);
+    o.bx = ldN<ND>(F, G, TFLD(BX), i, d0);
+    o.by = ldN<ND>(F, G, TFLD(BY), i, d0);
+    o.bz = ldN<ND>(F, G, TFLD(BZ), i, d0);
+    o.al = ldN<ND>(F, G, TFLD(ALPHA1), i, d0);
+    o.au = ld_aprevN<ND>(aprev, 0, n, i, d0);
+    o.av = ld_aprevN<ND>(aprev, 1, n, i, d0);
+    o.aw = ld_aprevN<ND>(aprev, 2, n, i, d0);
+    const D3 cs = ldN<ND>(F, G, TFLD(CS), i, d0);
     const D3 b2 = o.bx * o.bx + o.by * o.by + o.bz * o.bz;
     o.vw = sqrt(cs * cs + b2 / (s.mu0 * o.rho));
-    o.psich = ldT(F, G, TFLD(PSIPRED), i) * (chf * o.vw);
+    o.psich = ldN<ND>(F, G, TFLD(PSIPRED), i, d0) * (chf * o.vw);
     o.hir = 0.5 * (1.0 / o.rho);
     return o;
 }
 
-template <int K, int DIM>
+// ND directions per workgroup, the directions spread over blockIdx.y: with
+// one direction the dual accumulators fit in registers (three would spill),
+// and small systems get three times the workgroups.  Only the direction-0
+// workgroups store values and reduce the pending dt.
+template <int K, int DIM, int ND>
 __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const double* __restrict__ aprev,
                                                     const int32_t* __restrict__ perm,
                                                     const int32_t* __restrict__ nb,
                                                     const int32_t* __restrict__ nbcnt, Inst* inst,
                                                     const InstTan* __restrict__ itan,
                                                     const int32_t* __restrict__ inst_of, int64_t n, int stride) {
+    using D3 = Dual<ND>;
+    const int d0 = blockIdx.y * ND;
+    const bool lead = blockIdx.y == 0;
     const int64_t t0 = (int64_t)blockIdx.x * kBlock;
     if (t0 >= n) return;
     const int64_t t = t0 + threadIdx.x;
@@ -335,13 +378,13 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
     bool bad = false;
     if (valid) {
         const int32_t i = perm[t];
-        const D3 av = parT(s.av, it, SPHMHD_TP_ARTIFICIAL_VISCOSITY);
-        const D3 ac = parT(s.ac, it, SPHMHD_TP_ARTIFICIAL_THERMAL_CONDUCTIVITY);
-        const D3 ar = parT(s.ar, it, SPHMHD_TP_ARTIFICIAL_MAGNETIC_DISSIPATION);
-        const D3 chf = parT(s.chf, it, SPHMHD_TP_CLEANING_SPEED_FACTOR);
-        const D3 sigma = parT(s.sigma, it, SPHMHD_TP_CLEANING_DAMPING_FACTOR);
-        const D3 alpha_max = parT(s.alpha_max, it, SPHMHD_TP_ALPHA_MAX);
-        const SideT me = load_side<DIM>(F, G, aprev, n, i, s, chf);
+        const D3 av = parN<ND>(s.av, it, SPHMHD_TP_ARTIFICIAL_VISCOSITY, d0);
+        const D3 ac = parN<ND>(s.ac, it, SPHMHD_TP_ARTIFICIAL_THERMAL_CONDUCTIVITY, d0);
+        const D3 ar = parN<ND>(s.ar, it, SPHMHD_TP_ARTIFICIAL_MAGNETIC_DISSIPATION, d0);
+        const D3 chf = parN<ND>(s.chf, it, SPHMHD_TP_CLEANING_SPEED_FACTOR, d0);
+        const D3 sigma = parN<ND>(s.sigma, it, SPHMHD_TP_CLEANING_DAMPING_FACTOR, d0);
+        const D3 alpha_max = parN<ND>(s.alpha_max, it, SPHMHD_TP_ALPHA_MAX, d0);
+        const SideT<D3> me = load_side<DIM, ND>(F, G, aprev, n, i, s, chf, d0);
         const D3 xi = me.x, yi = me.y, zi = me.z, h1i = me.h1;
         const D3 rhoi = me.rho, pi_ = me.p, ei = me.e;
         const D3 ui = me.u, vi = me.v, wi = me.w, omi = me.om, ali = me.al;
@@ -360,11 +403,11 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
         const double hLx = s.halfL[0], hLy = s.halfL[1], hLz = s.halfL[2];
         const int periodic = s.periodic;
 
-        AccT<DIM> A;
+        AccT<DIM, D3> A;
         const int cnt = min(nbcnt[t], stride);
         for (int q = 0; q < cnt; ++q) {
             const int32_t jj = perm[nb[(int64_t)q * n + t]];
-            const SideT o = load_side<DIM>(F, G, aprev, n, jj, s, chf);
+            const SideT<D3> o = load_side<DIM, ND>(F, G, aprev, n, jj, s, chf, d0);
             const D3 h1j = o.h1, mj = o.m, rhoj = o.rho, pj = o.p, ej = o.e;
             const D3 uj = o.u, vj = o.v, wj = o.w, omj = o.om;
             const D3 bxj = o.bx, byj = o.by, bzj = o.bz, alj = o.al;
@@ -477,8 +520,8 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
         }
 
         // ---- epilogue (as k_force) ----
-        const D3 hi = ldT(F, G, TFLD(H), i);
-        const D3 cs = ldT(F, G, TFLD(CS), i), psipred = ldT(F, G, TFLD(PSIPRED), i);
+        const D3 hi = ldN<ND>(F, G, TFLD(H), i, d0);
+        const D3 cs = ldN<ND>(F, G, TFLD(CS), i, d0), psipred = ldN<ND>(F, G, TFLD(PSIPRED), i, d0);
         const D3 div = -A.div / rhoi;
         const D3 ah = hi * div * (1.0 / (double)DIM);
         D3 div_a, dvxdx, dvxdy, dvxdz, dvydx, dvydy, dvydz, dvzdx, dvzdy, dvzdz;
@@ -553,26 +596,27 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
         if (f2 > 0.0) dt_adapt = fmin(dt_adapt, s.c_force * sqrt(hi / sqrt(f2)));
         bad = !isfinite(f2) || !isfinite(ae);
 
-        stT(F, G, TFLD(AU), i, ax);
-        stT(F, G, TFLD(AV), i, ay);
-        stT(F, G, TFLD(AW), i, az);
-        stT(F, G, TFLD(AE), i, ae);
-        stT(F, G, TFLD(AH), i, ah);
-        stT(F, G, TFLD(DIV), i, div);
-        stT(F, G, TFLD(DIV_FOR_PSI), i, div);
-        stT(F, G, TFLD(DT_CFL), i, dtcfl);
-        stT(F, G, TFLD(DT_ADAPT), i, dt_adapt);
-        stT(F, G, TFLD(ABEVOLX), i, A.bx);
-        stT(F, G, TFLD(ABEVOLY), i, A.by);
-        stT(F, G, TFLD(ABEVOLZ), i, A.bz);
-        stT(F, G, TFLD(APSI), i, apsi);
-        stT(F, G, TFLD(DIVBSYMM), i, rhoi * divb);
-        stT(F, G, TFLD(DIVBDIFF), i, A.divbd);
-        stT(F, G, TFLD(ALPHALOC), i, alphaloc);
-        stT(F, G, TFLD(XILIMITER), i, xil);
-        stT(F, G, TFLD(DIVVDT), i, divvdt);
+        stN<ND>(F, G, TFLD(AU), i, ax, d0, lead);
+        stN<ND>(F, G, TFLD(AV), i, ay, d0, lead);
+        stN<ND>(F, G, TFLD(AW), i, az, d0, lead);
+        stN<ND>(F, G, TFLD(AE), i, ae, d0, lead);
+        stN<ND>(F, G, TFLD(AH), i, ah, d0, lead);
+        stN<ND>(F, G, TFLD(DIV), i, div, d0, lead);
+        stN<ND>(F, G, TFLD(DIV_FOR_PSI), i, div, d0, lead);
+        stN<ND>(F, G, TFLD(DT_CFL), i, dtcfl, d0, lead);
+        stN<ND>(F, G, TFLD(DT_ADAPT), i, dt_adapt, d0, lead);
+        stN<ND>(F, G, TFLD(ABEVOLX), i, A.bx, d0, lead);
+        stN<ND>(F, G, TFLD(ABEVOLY), i, A.by, d0, lead);
+        stN<ND>(F, G, TFLD(ABEVOLZ), i, A.bz, d0, lead);
+        stN<ND>(F, G, TFLD(APSI), i, apsi, d0, lead);
+        stN<ND>(F, G, TFLD(DIVBSYMM), i, rhoi * divb, d0, lead);
+        stN<ND>(F, G, TFLD(DIVBDIFF), i, A.divbd, d0, lead);
+        stN<ND>(F, G, TFLD(ALPHALOC), i, alphaloc, d0, lead);
+        stN<ND>(F, G, TFLD(XILIMITER), i, xil, d0, lead);
+        stN<ND>(F, G, TFLD(DIVVDT), i, divvdt, d0, lead);
     }
     // the primal's min dt_adapt and NaN flag per instance
+    if (!lead) return;  // the whole workgroup: uniform
     constexpr int NW = kBlock / kWave;
     __shared__ unsigned long long wdb[NW];
     __shared__ int wk[NW], wbad[NW];
@@ -910,8 +954,8 @@ void launch_t_prep(Dev& d, TanDev& g) {
 void launch_t_force(Dev& d, TanDev& g) {
     if (d.n == 0) return;
 #define CALL_TF(KK, DD)                                                                                      \
-    { hipLaunchKernelGGL((k_t_force<KK, DD>), dim3(tblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, g.tf,     \
-                         g.aprev, d.perm, d.nb, d.nbcnt, d.inst, g.itan, d.inst_of, d.n, d.nb_stride); }
+    { hipLaunchKernelGGL((k_t_force<KK, DD, 1>), dim3(tblk(d.n), kTanDirs), dim3(kBlock), 0, d.stream, d.fields, \
+                         g.tf, g.aprev, d.perm, d.nb, d.nbcnt, d.inst, g.itan, d.inst_of, d.n, d.nb_stride); }
     T_DISPATCH(d.kernel, d.dim, CALL_TF)
 #undef CALL_TF
     hipLaunchKernelGGL(k_t_dt_argmin, dim3(tblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, g.itan,
