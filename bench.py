@@ -173,6 +173,49 @@ def run_gpu(name: str, steps: int, warmup: int, timing_steps: int = 10):
     return res
 
 
+def run_tangent(steps: int, warmup: int = 2):
+    """Forward step with derivatives against the plain step on the travelling
+    Alfven wave (97x168, Wendland C6, h=1.9dx): the tangent-linear step
+    advances the values and NTANGENTS directional derivatives (here d/dA of
+    the initial amplitude, d/d alpha_max, d/d sigma), so one sweep gives the
+    gradient of any final-time total in those three inputs."""
+    import torch
+
+    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.device import DeviceSystem
+    from sphmhd.tangent import TangentSystem, _ic_tangent
+
+    names = ("amplitude", "alpha_max", "cleaning_damping_factor")
+    theta = dict(zip(names, (0.1, 1.0, 1.0)))
+    setup = alfven_2d(97, 168)
+    out = {"desc": "alfven_2d 97x168 (16,296 particles), wendland_c6, h=1.9dx; derivatives along "
+                   "d/d amplitude, d/d alpha_max, d/d cleaning_damping_factor",
+           "particles": setup.n, "steps": steps}
+    for label, cls in (("plain", DeviceSystem), ("tangent", TangentSystem)):
+        d = cls(setup.params, setup.offsets, device="cuda:0")
+        d.set_fields(dict(setup.fields))
+        if cls is TangentSystem:
+            d.set_tangents(0, _ic_tangent(lambda **kw: alfven_2d(97, 168, **kw), theta, "amplitude"))
+            d.seed_params({(1, "alpha_max"): 1.0, (2, "cleaning_damping_factor"): 1.0})
+        d.warm_start()
+        d.step(warmup)
+        d.set_timing(False)
+        d.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.step(steps)
+        d.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[label + "_ms_per_step"] = round(1e3 * wall / steps, 4)
+        out[label + "_particle_steps_per_s"] = round(setup.n * steps / wall, 1)
+        d.close()
+        del d
+    torch.cuda.empty_cache()
+    out["tangent_over_plain"] = round(out["tangent_ms_per_step"] / out["plain_ms_per_step"], 3)
+    return out
+
+
 def roofline(res):
     """Roofline of the dominant kernel (the fused force pass).  Its binding
     roof is the FP64 rate: per launch it does flops_per_pair x pairs of FP64
@@ -278,6 +321,7 @@ def main():
                 "ratio": round(extras["ot128x64"]["particle_steps_per_s"] / extras["ot128"]["particle_steps_per_s"], 2),
                 "what": "particle-steps/s of 64 OT 128x128 instances in one batch over one OT 128x128 instance alone",
             }
+        extras["fwd_tangent"] = run_tangent(100)
     line = {
         "metric": METRIC,
         "value": round(head["particle_steps_per_s"], 1),
