@@ -78,7 +78,6 @@ struct sphmhd_ctx {
     int evals_since_redo = 0;
     Ctl* hctl = nullptr;  // pinned readback
     sphmhd_stats stats{};
-    int64_t tiles = 0;
     // captured steps, one per speculative pass count; generation bumps on
     // bind, list growth and timing changes invalidate them
     // With timing on, the step is captured as one graph per timed phase
@@ -485,11 +484,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
             cap += 2 * nk > 1 ? 2 * nk : 1;
         }
         d.cell_cap = cap;
-        int bits = 1;
-        while (bits < 32 && ((int64_t)1 << bits) < cap) ++bits;
-        d.key_bits = bits;
         d.nb_stride = p0.dim == 2 ? 64 : 96;
-        c->tiles = (n + kRadixTile - 1) / kRadixTile;
 
         c->hinst.resize(n_instances);
         for (int k = 0; k < n_instances; ++k) {
@@ -548,8 +543,9 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.keys_alt, n))) return cleanup(rc);
         if ((rc = dalloc(&d.perm, n))) return cleanup(rc);
         if ((rc = dalloc(&d.perm_alt, n))) return cleanup(rc);
-        if ((rc = dalloc(&d.rs_hist, 256 * c->tiles))) return cleanup(rc);
-        if ((rc = dalloc(&d.rs_base, 256))) return cleanup(rc);
+        if ((rc = dalloc(&d.ccount, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.cblock, (cap + kScanCells - 1) / kScanCells))) return cleanup(rc);
+        if ((rc = dalloc(&d.bigc, n / (kSmallBucket + 1) + 1))) return cleanup(rc);
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
@@ -566,6 +562,11 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         e = hipMemcpy(d.inst, c->hinst.data(), sizeof(Inst) * n_instances, hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
         e = hipMemcpy(d.inst_of, inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
+        // the cell sort visits particles in the previous sort's order: start
+        // from the storage order
+        for (int64_t i = 0; i < n; ++i) inst_of[i] = (int32_t)i;
+        e = hipMemcpy(d.perm, inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
         *out = c;
         return 0;
@@ -585,7 +586,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
-    void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.rs_hist, d.rs_base,
+    void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)

@@ -23,7 +23,10 @@ namespace sph {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kRadixTile = 256;  // keys per wave tile of the radix sort
+constexpr int kScanItems = 4;    // cells per thread of the cell-count scan
+constexpr int kScanCells = 256 * kScanItems;  // cells per scan block
+constexpr int kSmallBucket = 64; // cells up to this size are ordered by rank, one thread per particle
+constexpr int kBigLds = 4096;    // larger cells up to this size are sorted in LDS
 
 // Force-pack record: the FP64 values the force pair loop reads from a
 // neighbour, 18 in 2-D (144 B) and 20 in 3-D (160 B), packed densely so that
@@ -83,7 +86,7 @@ struct Ctl {
     int64_t count;
     int32_t act_cnt[2];  // lengths of the two active-particle lists of the density passes
     int32_t overflow;    // a neighbour list did not fit its slots
-    int32_t pad;
+    int32_t nbig;        // cells with more than kSmallBucket particles in this sort
 };
 
 struct Fields {
@@ -485,7 +488,6 @@ struct Dev {
     int32_t ninst;
     int32_t dim, kernel;
     int64_t cell_cap;
-    int32_t key_bits;
     int32_t nb_stride;      // neighbour-list capacity per particle
     int32_t pass;           // density pass index within the evaluation
     hipStream_t stream;
@@ -495,8 +497,9 @@ struct Dev {
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
     uint32_t *keys, *keys_alt;
     int32_t *perm, *perm_alt;
-    uint32_t* rs_hist;      // 256 x tiles
-    uint32_t* rs_base;      // 256
+    int32_t* ccount;        // particles per cell (cell_cap; zero between sorts)
+    int32_t* cblock;        // per scan block of kScanCells cells: exclusive offset
+    int32_t* bigc;          // cells with more than kSmallBucket particles
     int32_t *cstart, *cend;
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build

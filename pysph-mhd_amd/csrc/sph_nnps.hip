@@ -1,4 +1,5 @@
-// sph_nnps.hip -- cell-list neighbour search rebuilt as an on-GPU radix sort.
+// sph_nnps.hip -- cell-list neighbour search rebuilt as an on-GPU sort of the
+// particles' cell keys (a one-digit radix sort: count, scan, place).
 //
 // Replaces PySPH's NNPS + DomainManager(periodic_mode="minimum_image") as
 // used by the reference (application.py:63-78, scheme.py:71 update_nnps).
@@ -8,9 +9,10 @@
 //   key    = cell_base + ix + n_x (iy + n_y iz),  i_a = clamp(floor((x_a-lo_a)/c_eff_a))
 //   sort   = stable by key (storage index breaks ties)
 //   pairs  = r^2 < (R max(h_i, h_j))^2 with the minimum image
-// All of it is HBM-bound integer work: coalesced key writes, one LDS
-// histogram per wave tile and a stable multisplit scatter per 8-bit digit.
+// All of it is HBM-bound integer work: one key per particle, one counter
+// per cell, a scan over the cells and one scattered placement per particle.
 #include <algorithm>
+#include <climits>
 
 #include "sph_internal.h"
 
@@ -49,8 +51,11 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
 // (kHmaxBlocks blocks, each over one contiguous chunk): a thread keeps a
 // running max for the instance it is in and flushes it when the instance
 // changes (instances are contiguous), then waves and the block combine what
-// is left into one atomic per block for a single instance.
+// is left into one atomic per block for a single instance.  Each thread
+// loads kHmaxBatch elements before folding them in, so the loads of a small
+// grid are in flight together rather than one latency at a time.
 constexpr int kHmaxBlocks = 128;
+constexpr int kHmaxBatch = 8;
 
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
                        int64_t n, int next, int64_t chunk) {
@@ -60,15 +65,27 @@ __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__
     const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = min(c0 + chunk, n);
     int kcur = -1;
     unsigned long long b = 0ull;
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-        const int k = inst_of[i];
-        if (k != kcur) {
-            if (kcur >= 0) atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
-            kcur = k;
-            b = 0ull;
+    for (int64_t i0 = c0 + threadIdx.x; i0 < c1; i0 += (int64_t)kHmaxBatch * blockDim.x) {
+        int kk[kHmaxBatch];
+        unsigned long long vv[kHmaxBatch];
+#pragma unroll
+        for (int u = 0; u < kHmaxBatch; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            const bool ok = i < c1;
+            kk[u] = ok ? inst_of[i] : -1;
+            vv[u] = ok ? dbits(h[i]) : 0ull;
         }
-        const unsigned long long v = dbits(h[i]);
-        b = v > b ? v : b;
+#pragma unroll
+        for (int u = 0; u < kHmaxBatch; ++u) {
+            const int k = kk[u];
+            if (k < 0) continue;
+            if (k != kcur) {
+                if (kcur >= 0) atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
+                kcur = k;
+                b = 0ull;
+            }
+            b = vv[u] > b ? vv[u] : b;
+        }
     }
     // threads with no element carry kcur = -1 and b = 0
     int k0 = kcur;
@@ -139,177 +156,230 @@ __global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) {
             base += inst[q].ncell;
         }
         ctl->total_cells = base;
+        ctl->nbig = 0;
     }
 }
 
-__global__ void k_keys(Fields F, const int32_t* __restrict__ inst_of, const Inst* __restrict__ inst,
-                       uint32_t* __restrict__ keys, int32_t* __restrict__ vals, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Inst& s = inst[inst_of[i]];
-    int ix = cell_coord(F.f[SPHMHD_F_X][i], s.lo[0], s.ceff[0], s.n[0]);
-    int iy = cell_coord(F.f[SPHMHD_F_Y][i], s.lo[1], s.ceff[1], s.n[1]);
-    int iz = s.dim == 3 ? cell_coord(F.f[SPHMHD_F_Z][i], s.lo[2], s.ceff[2], s.n[2]) : 0;
-    int64_t local = ix + (int64_t)s.n[0] * (iy + (int64_t)s.n[1] * iz);
-    keys[i] = (uint32_t)(s.cell_base + local);
-    vals[i] = (int32_t)i;
-}
-
 // ---------------------------------------------------------------------------
-// stable LSD radix sort, 8-bit digits, wave-sized tiles
+// cell sort: one radix digit spanning the whole cell key (a counting sort).
+// count -> two-level exclusive scan (cstart/cend of every cell) -> each
+// particle placed in its cell's bucket -> each bucket put in storage order,
+// which makes the sort stable (ties by storage index, as the definition
+// asks).  Buckets are small (a cell is R h_max wide), so each particle of a
+// bucket of up to kSmallBucket finds its place by its rank; larger buckets
+// go to a workgroup bitonic sort (in LDS up to kBigLds, in global memory
+// beyond: a one-cell grid holds every particle).  ccount returns to zero in
+// the placement, ready for the next sort.
 // ---------------------------------------------------------------------------
 
-constexpr int RS_TILE = kRadixTile;          // keys per wave tile
-constexpr int RS_ITEMS = RS_TILE / kWave;
-constexpr int RS_WAVES = kBlock / kWave;
-
-// A pass whose digit lies above the highest bit of the largest key (the cell
-// count is only known on the device) is an identity: hist/scan return at
-// once and the scatter becomes a copy, so the result always ends in the same
-// buffer whatever the number of live passes.
-__device__ __forceinline__ bool rs_pass_dead(const Ctl* ctl, int shift) {
-    const int64_t top = ctl->total_cells - 1;
-    return shift > 0 && top >= 0 && (top >> shift) == 0;
+// Particles are visited in the previous sort's order (perm holds it; the
+// identity before the first sort), where the particles of one cell are
+// mostly adjacent: each run of equal keys in a wave does one atomic.
+__device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool valid) {
+    const uint32_t prev = __shfl_up(key, 1);
+    const int lane = threadIdx.x & (kWave - 1);
+    return __ballot(valid && (lane == 0 || prev != key));
 }
 
-__global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                          uint32_t* __restrict__ hist, int64_t tiles, const Ctl* ctl) {
-    if (rs_pass_dead(ctl, shift)) return;
-    __shared__ uint32_t h[RS_WAVES][256];
-    int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    for (int d = lane; d < 256; d += kWave) h[wave][d] = 0;
+// one past the last lane of the run that starts at head lane `lane`
+__device__ __forceinline__ int run_end(unsigned long long heads, unsigned long long vmask, int lane) {
+    const unsigned long long above = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
+    return above ? __ffsll((long long)above) - 1 : kWave - __clzll(vmask);
+}
+
+__global__ void k_cell_count(Fields F, const int32_t* __restrict__ inst_of, const Inst* __restrict__ inst,
+                             const int32_t* __restrict__ prev_perm, uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ ccount, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        const int32_t i = prev_perm[t];
+        const Inst& s = inst[inst_of[i]];
+        const int ix = cell_coord(F.f[SPHMHD_F_X][i], s.lo[0], s.ceff[0], s.n[0]);
+        const int iy = cell_coord(F.f[SPHMHD_F_Y][i], s.lo[1], s.ceff[1], s.n[1]);
+        const int iz = s.dim == 3 ? cell_coord(F.f[SPHMHD_F_Z][i], s.lo[2], s.ceff[2], s.n[2]) : 0;
+        const int64_t local = ix + (int64_t)s.n[0] * (iy + (int64_t)s.n[1] * iz);
+        key = (uint32_t)(s.cell_base + local);
+        keys[i] = key;
+    }
+    const unsigned long long heads = run_heads(key, valid);
+    const unsigned long long vmask = __ballot(valid);
+    const int lane = threadIdx.x & (kWave - 1);
+    if ((heads >> lane) & 1ull) atomicAdd(&ccount[key], run_end(heads, vmask, lane) - lane);
+}
+
+// exclusive scan of a block-sized value per thread (kBlock threads)
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    int incl = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
     __syncthreads();
-    int64_t tile = (int64_t)blockIdx.x * RS_WAVES + wave;
-    if (tile < tiles) {
-        int64_t base = tile * RS_TILE;
-        for (int it = 0; it < RS_ITEMS; ++it) {
-            int64_t idx = base + it * kWave + lane;
-            if (idx < n) atomicAdd(&h[wave][(keys[idx] >> shift) & 255u], 1u);
+    int base = 0;
+    total = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+        if (q < w) base += wsum[q];
+        total += wsum[q];
+    }
+    return base + incl - v;
+}
+
+// per block of kScanCells cells: local exclusive/inclusive prefix into
+// cstart/cend, block total into cblock
+__global__ void __launch_bounds__(kBlock) k_cell_scan_local(const int32_t* __restrict__ ccount,
+                                                            int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
+                                                            int32_t* __restrict__ cblock, const Ctl* ctl) {
+    __shared__ int wsum[kBlock / kWave];
+    const int64_t total = ctl->total_cells;
+    const int64_t c0 = (int64_t)blockIdx.x * kScanCells;
+    if (c0 >= total) return;  // whole block: uniform
+    const int64_t cb = c0 + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        v[q] = cb + q < total ? ccount[cb + q] : 0;
+        sum += v[q];
+    }
+    int btot;
+    int run = block_excl_scan(sum, wsum, btot);
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        if (cb + q < total) {
+            cstart[cb + q] = run;
+            cend[cb + q] = run + v[q];
         }
+        run += v[q];
     }
-    __syncthreads();
-    if (tile < tiles)
-        for (int d = lane; d < 256; d += kWave) hist[(int64_t)d * tiles + tile] = h[wave][d];
+    if (threadIdx.x == 0) cblock[blockIdx.x] = btot;
 }
 
-// exclusive scan of one digit's row of tile counts; row totals to tot[d]
-__global__ void k_rs_scan_rows(uint32_t* __restrict__ hist, int64_t tiles, uint32_t* __restrict__ tot,
-                               int shift, const Ctl* ctl) {
-    if (rs_pass_dead(ctl, shift)) return;
-    __shared__ uint32_t part[kBlock];
-    uint32_t* row = hist + (int64_t)blockIdx.x * tiles;
-    int64_t per = (tiles + kBlock - 1) / kBlock;
-    int64_t a = threadIdx.x * per, b = a + per < tiles ? a + per : tiles;
-    uint32_t s = 0;
-    for (int64_t q = a; q < b; ++q) s += row[q];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
+// exclusive scan of the block totals (one block)
+__global__ void __launch_bounds__(kBlock) k_cell_scan_top(int32_t* __restrict__ cblock, const Ctl* ctl) {
+    __shared__ int wsum[kBlock / kWave];
+    const int64_t nb = (ctl->total_cells + kScanCells - 1) / kScanCells;
+    const int64_t per = (nb + kBlock - 1) / kBlock;
+    const int64_t a = (int64_t)threadIdx.x * per, b = a + per < nb ? a + per : nb;
+    int sum = 0;
+    for (int64_t q = a; q < b; ++q) sum += cblock[q];
+    int tot;
+    int run = block_excl_scan(sum, wsum, tot);
     for (int64_t q = a; q < b; ++q) {
-        uint32_t c = row[q];
-        row[q] = run;
+        const int c = cblock[q];
+        cblock[q] = run;
         run += c;
     }
-    if (threadIdx.x == kBlock - 1) tot[blockIdx.x] = part[kBlock - 1];
 }
 
-__global__ void k_rs_scan_digits(uint32_t* __restrict__ tot, int shift, const Ctl* ctl) {
-    if (rs_pass_dead(ctl, shift)) return;
-    __shared__ uint32_t part[256];
-    uint32_t v = tot[threadIdx.x];
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        uint32_t o = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += o;
-        __syncthreads();
+// each run takes a block of its cell's slots (filled from the end down, so
+// the counts return to zero); the order inside a bucket is arbitrary here
+__global__ void k_cell_place(const int32_t* __restrict__ prev_perm, const uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ ccount, const int32_t* __restrict__ cstart,
+                             const int32_t* __restrict__ cblock, int32_t* __restrict__ bucket, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
+    const int32_t i = valid ? prev_perm[t] : 0;
+    const uint32_t key = valid ? keys[i] : 0xFFFFFFFFu;
+    const unsigned long long heads = run_heads(key, valid);
+    const unsigned long long vmask = __ballot(valid);
+    const int lane = threadIdx.x & (kWave - 1);
+    int32_t top = 0;
+    if ((heads >> lane) & 1ull) {
+        const int len = run_end(heads, vmask, lane) - lane;
+        top = cstart[key] + cblock[key / kScanCells] + atomicSub(&ccount[key], len) - len;
     }
-    tot[threadIdx.x] = part[threadIdx.x] - v;
+    // this lane's run head: the highest head at or below it
+    const unsigned long long upto = heads & (lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1));
+    const int hl = upto ? kWave - 1 - __clzll(upto) : lane;
+    const int32_t base = __shfl(top, hl);
+    if (valid) bucket[base + (lane - hl)] = i;
 }
 
-__global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
-                             uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
-                             const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dbase, int64_t tiles,
-                             const Ctl* ctl) {
-    __shared__ uint32_t run[RS_WAVES][256];
-    int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    int64_t tile = (int64_t)blockIdx.x * RS_WAVES + wave;
-    if (rs_pass_dead(ctl, shift)) {
-        if (tile < tiles)
-            for (int it = 0; it < RS_ITEMS; ++it) {
-                int64_t idx = tile * RS_TILE + it * kWave + lane;
-                if (idx < n) {
-                    kout[idx] = kin[idx];
-                    vout[idx] = vin[idx];
+// final cell table; cells above kSmallBucket particles are listed for k_cell_big
+__global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
+                              const int32_t* __restrict__ cblock, int32_t* __restrict__ bigc, Ctl* ctl, int64_t cap) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cap || c >= ctl->total_cells) return;
+    const int32_t off = cblock[c / kScanCells];
+    const int32_t s = cstart[c] + off, e = cend[c] + off;
+    cstart[c] = s;
+    cend[c] = e;
+    if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)c;
+}
+
+// small cells: a particle's place is its rank by storage index in its bucket
+__global__ void k_cell_rank(const int32_t* __restrict__ bucket, const uint32_t* __restrict__ keys,
+                            const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int32_t v = bucket[q];
+    const uint32_t c = keys[v];
+    const int32_t s = cstart[c], e = cend[c];
+    if (e - s > kSmallBucket) return;
+    int32_t r = 0;
+    for (int32_t k = s; k < e; ++k) r += bucket[k] < v ? 1 : 0;
+    perm[s + r] = v;
+    skey[s + r] = c;
+}
+
+// ascending bitonic network over P = 2^k slots of which the first m are
+// real; comparisons with a slot >= m are skipped, which is exact because a
+// missing slot stands for +infinity (every compare-exchange puts the smaller
+// value at the lower index)
+__device__ __forceinline__ void bitonic_sort(int32_t* a, int64_t m, int64_t P) {
+    for (int64_t k = 2; k <= P; k <<= 1) {
+        for (int64_t j = k >> 1; j > 0; j >>= 1) {
+            for (int64_t q = threadIdx.x; q < P; q += blockDim.x) {
+                const int64_t p = j == (k >> 1) ? (q ^ (k - 1)) : (q ^ j);
+                if (p > q && p < m) {
+                    const int32_t x = a[q], y = a[p];
+                    if (x > y) {
+                        a[q] = y;
+                        a[p] = x;
+                    }
                 }
             }
-        return;
+            __syncthreads();
+        }
     }
-    for (int d = lane; d < 256; d += kWave)
-        run[wave][d] = tile < tiles ? dbase[d] + hist[(int64_t)d * tiles + tile] : 0u;
-    __syncthreads();
-    if (tile >= tiles) return;
-    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    int64_t base = tile * RS_TILE;
-    for (int it = 0; it < RS_ITEMS; ++it) {
-        int64_t idx = base + it * kWave + lane;
-        bool valid = idx < n;
-        uint32_t key = valid ? kin[idx] : 0u;
-        int32_t val = valid ? vin[idx] : 0;
-        uint32_t dg = (key >> shift) & 255u;
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            bool bit = (dg >> b) & 1u;
-            unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_big(const int32_t* __restrict__ cstart,
+                                                     const int32_t* __restrict__ cend,
+                                                     const int32_t* __restrict__ bucket, int32_t* perm,
+                                                     uint32_t* __restrict__ skey, const int32_t* __restrict__ bigc,
+                                                     const Ctl* ctl) {
+    __shared__ int32_t lds[kBigLds];
+    const int nbig = ctl->nbig;
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int32_t c = bigc[b];
+        const int32_t s = cstart[c];
+        const int64_t m = cend[c] - s;
+        int64_t P = 1;
+        while (P < m) P <<= 1;
+        for (int64_t q = threadIdx.x; q < m; q += blockDim.x) skey[s + q] = (uint32_t)c;
+        if (m <= kBigLds) {
+            for (int64_t q = threadIdx.x; q < P; q += blockDim.x) lds[q] = q < m ? bucket[s + q] : INT_MAX;
+            __syncthreads();
+            bitonic_sort(lds, P, P);
+            for (int64_t q = threadIdx.x; q < m; q += blockDim.x) perm[s + q] = lds[q];
+        } else {
+            for (int64_t q = threadIdx.x; q < m; q += blockDim.x) perm[s + q] = bucket[s + q];
+            __syncthreads();
+            bitonic_sort(perm + s, m, P);
         }
-        uint32_t before = (uint32_t)__popcll(peers & lt);
-        uint32_t cnt = (uint32_t)__popcll(peers);
-        uint32_t pos = run[wave][dg] + before;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            kout[pos] = key;
-            vout[pos] = val;
-            if (before == 0) run[wave][dg] = pos + cnt;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// cell table and packs
+// packs
 // ---------------------------------------------------------------------------
-
-// cell table: cstart[c] = first sorted position with key >= c and cend[c] =
-// first with key > c, for every cell (binary search of the sorted keys), so
-// empty cells still bound their neighbours' ranges
-__device__ __forceinline__ int32_t lower_bound_key(const uint32_t* __restrict__ skey, int64_t n, uint32_t k) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (skey[mid] < k) lo = mid + 1;
-        else hi = mid;
-    }
-    return (int32_t)lo;
-}
-
-__global__ void k_cell_bounds(const uint32_t* __restrict__ skey, int32_t* __restrict__ cstart,
-                              int32_t* __restrict__ cend, int64_t n, const Ctl* ctl, int64_t cap) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap || c >= ctl->total_cells) return;
-    cstart[c] = lower_bound_key(skey, n, (uint32_t)c);
-    cend[c] = lower_bound_key(skey, n, (uint32_t)(c + 1));
-}
 
 __global__ void k_pack_density(Fields F, const int32_t* __restrict__ perm, double4* __restrict__ dpack, int64_t n,
                                int dim) {
@@ -471,29 +541,26 @@ void launch_grid(Dev& d) {
     hipLaunchKernelGGL(k_grid, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
 }
 
+constexpr int kBigBlocks = 64;
+
 void launch_keys_sort_ranges(Dev& d) {
     if (d.n == 0) return;
-    // the keys start in whichever buffer makes the last pass land in d.keys,
-    // so the buffers (and a captured graph's arguments) never swap
-    const int passes = (d.key_bits + 7) / 8;
-    uint32_t *ka = (passes & 1) ? d.keys_alt : d.keys, *kb = (passes & 1) ? d.keys : d.keys_alt;
-    int32_t *va = (passes & 1) ? d.perm_alt : d.perm, *vb = (passes & 1) ? d.perm : d.perm_alt;
-    hipLaunchKernelGGL(k_keys, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, ka, va, d.n);
-    int64_t tiles = (d.n + RS_TILE - 1) / RS_TILE;
-    unsigned blocks = (unsigned)((tiles + RS_WAVES - 1) / RS_WAVES);
-    for (int shift = 0; shift < d.key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(blocks), dim3(kBlock), 0, d.stream, ka, d.n, shift, d.rs_hist, tiles,
-                           d.ctl);
-        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(kBlock), 0, d.stream, d.rs_hist, tiles, d.rs_base, shift,
-                           d.ctl);
-        hipLaunchKernelGGL(k_rs_scan_digits, dim3(1), dim3(256), 0, d.stream, d.rs_base, shift, d.ctl);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(blocks), dim3(kBlock), 0, d.stream, ka, va, kb, vb, d.n, shift,
-                           d.rs_hist, d.rs_base, tiles, d.ctl);
-        uint32_t* tk = ka; ka = kb; kb = tk;
-        int32_t* tv = va; va = vb; vb = tv;
-    }
-    hipLaunchKernelGGL(k_cell_bounds, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.keys, d.cstart, d.cend, d.n,
-                       d.ctl, d.cell_cap);
+    // keys_alt: keys by storage index; perm_alt: the buckets; perm: the
+    // previous order on entry, the new one on exit
+    hipLaunchKernelGGL(k_cell_count, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.perm,
+                       d.keys_alt, d.ccount, d.n);
+    const unsigned sblocks = (unsigned)((d.cell_cap + kScanCells - 1) / kScanCells);
+    hipLaunchKernelGGL(k_cell_scan_local, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cstart, d.cend,
+                       d.cblock, d.ctl);
+    hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(kBlock), 0, d.stream, d.cblock, d.ctl);
+    hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
+                       d.cstart, d.cblock, d.perm_alt, d.n);
+    hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,
+                       d.bigc, d.ctl, d.cell_cap);
+    hipLaunchKernelGGL(k_cell_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm_alt, d.keys_alt, d.cstart,
+                       d.cend, d.perm, d.keys, d.n);
+    hipLaunchKernelGGL(k_cell_big, dim3(kBigBlocks), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.perm_alt, d.perm,
+                       d.keys, d.bigc, d.ctl);
 }
 
 void launch_pack_density(Dev& d) {

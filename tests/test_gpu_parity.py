@@ -62,6 +62,21 @@ def test_cells_bit_exact(gpu):
     assert np.array_equal(pd, po)
 
 
+@pytest.mark.parametrize("nx,hbig", [(48, 0.09), (48, 0.5), (80, 0.5)])
+def test_cells_crowded_bit_exact(gpu, nx, hbig):
+    """One particle with a large h coarsens the grid: cells of 256 particles
+    (3x3 cells), one cell holding all 2,304, one holding all 6,400 (past the
+    in-LDS bucket sort).  Keys and permutation still match the stable sort."""
+    s = _ot(nx)
+    s.fields["h"] = s.fields["h"].copy()
+    s.fields["h"][7] = hbig
+    dev, orc = _pair(s)
+    kd, pd = dev.cells()
+    ko, po = orc.cells()
+    assert np.array_equal(kd, ko)
+    assert np.array_equal(pd, po)
+
+
 @pytest.mark.parametrize("which", ["ot2d", "blast3d", "ensemble"])
 def test_neighbours_bit_exact(gpu, which):
     from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered
