@@ -62,7 +62,7 @@ struct PhaseTimer {
 // the enqueue-only part of a step, in timed phases: stages, neighbour
 // search, density passes, prep + lists, force pass, then the tail (warning
 // collection and the control-block copy)
-enum { P_STAGE = 0, P_NNPS, P_DENSITY, P_LISTS, P_FORCE, P_TAIL, kSegs };
+enum { P_STAGE = 0, P_NNPS, P_DENSITY, P_LISTS, P_FORCE, kSegs };
 
 struct sphmhd_ctx {
     Dev d{};
@@ -139,7 +139,7 @@ static void tcollect(sphmhd_ctx* c) {
 }
 
 static int read_ctl(sphmhd_ctx* c) {
-    HIPCHK(hipMemcpyAsync(c->hctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->d.stream));
+    launch_ctl_export(c->d);
     HIPCHK(hipStreamSynchronize(c->d.stream));
     tcollect(c);
     return 0;
@@ -186,10 +186,11 @@ static int build_lists(sphmhd_ctx* c) {
     return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
 }
 
-static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions) {
-    launch_eval_begin(c->d, reset_clock_reductions);
-    launch_hmax(c->d, false);
-    launch_grid(c->d);
+// begun: the evaluation's start (eval_begin_body with reset_dt) already ran
+// in the tail of the fused stage kernel
+static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions, bool begun = false) {
+    if (!begun) launch_eval_begin(c->d, reset_clock_reductions);
+    launch_hmax_grid(c->d);
     launch_keys_sort_ranges(c->d);
     HIPCHK(hipGetLastError());
     return 0;
@@ -205,9 +206,8 @@ static int force_pass(sphmhd_ctx* c, bool first) {
     launch_nb_build(d);
     tend(c, T_LISTS);
     tbegin(c, T_FORCE);
-    launch_force(d);
+    launch_force(d);  // ends with the evaluation's end (warning bits, host mirror)
     tend(c, T_FORCE);
-    launch_eval_end(d);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -223,7 +223,7 @@ static int force_pass(sphmhd_ctx* c, bool first) {
 // redo one extra speculative pass is launched for the next 16 evaluations.
 static int next_spec(const sphmhd_ctx* c) { return std::max(c->spec_passes + c->spec_margin, 1); }
 
-static const int kPhaseTimer[kSegs] = {T_STAGE, T_NNPS, T_DENSITY, T_LISTS, T_FORCE, -1};
+static const int kPhaseTimer[kSegs] = {T_STAGE, T_NNPS, T_DENSITY, T_LISTS, T_FORCE};
 
 // the launches of one phase of the enqueue-only step front
 static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
@@ -234,12 +234,11 @@ static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
             launch_initialize(d);
             launch_stage1(d);
         } else {
-            launch_stage2_init_stage1(d);
-            launch_clock(d);
+            launch_stage2_init_stage1(d);  // also commits the clock and begins the evaluation
         }
         break;
     case P_NNPS: {
-        const int rc = nnps_rebuild(c, true);
+        const int rc = nnps_rebuild(c, true, !first);
         if (rc) return rc;
         launch_pack_density(d);
         break;
@@ -247,20 +246,15 @@ static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
     case P_DENSITY:
         for (int q = 0; q < spec; ++q) {
             d.pass = q;
-            launch_density(d);
-            launch_g0_control(d);
+            launch_density(d);  // ends with the group control
         }
         break;
     case P_LISTS:
         launch_prep_force(d, true);
         launch_nb_build(d);
         break;
-    case P_FORCE:
-        launch_force(d);
-        break;
     default:
-        launch_eval_end(d);
-        HIPCHK(hipMemcpyAsync(c->hctl, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, d.stream));
+        launch_force(d);  // ends with the evaluation's end (warning bits, host mirror)
         break;
     }
     HIPCHK(hipGetLastError());
@@ -279,7 +273,9 @@ static int front_range(sphmhd_ctx* c, int from, bool first, int spec) {
     return 0;
 }
 
-static int eval_front(sphmhd_ctx* c, int spec) { return front_range(c, P_NNPS, false, spec); }
+// a standalone evaluation: no fused stage kernel ran before it, so the
+// evaluation's start is launched (first = true)
+static int eval_front(sphmhd_ctx* c, int spec) { return front_range(c, P_NNPS, true, spec); }
 
 static int eval_finish(sphmhd_ctx* c, int spec) {
     Dev& d = c->d;
@@ -296,7 +292,6 @@ static int eval_finish(sphmhd_ctx* c, int spec) {
             tbegin(c, T_DENSITY);
             d.pass = passes++;
             launch_density(d);
-            launch_g0_control(d);
             tend(c, T_DENSITY);
             HIPCHK(hipGetLastError());
             if ((rc = read_ctl(c))) return rc;
@@ -538,6 +533,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         };
         if ((rc = dalloc(&d.inst, n_instances))) return cleanup(rc);
         if ((rc = dalloc(&d.ctl, 1))) return cleanup(rc);
+        if ((rc = dalloc(&d.arrive, kArriveInts))) return cleanup(rc);
         if ((rc = dalloc(&d.inst_of, n))) return cleanup(rc);
         if ((rc = dalloc(&d.keys, n))) return cleanup(rc);
         if ((rc = dalloc(&d.keys_alt, n))) return cleanup(rc);
@@ -557,8 +553,12 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.scratch3, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.aprev, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
-        hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocDefault);
+        // the control block's host mirror: pinned, mapped and coherent, so the
+        // kernel that ends an evaluation writes it directly (no copy)
+        hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostMalloc failed"));
+        e = hipHostGetDevicePointer((void**)&d.hctl_dev, c->hctl, 0);
+        if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostGetDevicePointer failed"));
         e = hipMemcpy(d.inst, c->hinst.data(), sizeof(Inst) * n_instances, hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
         e = hipMemcpy(d.inst_of, inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
@@ -586,7 +586,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
-    void* ptrs[] = {d.inst, d.ctl, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
+    void* ptrs[] = {d.inst, d.ctl, d.arrive, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)

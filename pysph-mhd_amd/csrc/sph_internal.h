@@ -87,6 +87,7 @@ struct Ctl {
     int32_t act_cnt[2];  // lengths of the two active-particle lists of the density passes
     int32_t overflow;    // a neighbour list did not fit its slots
     int32_t nbig;        // cells with more than kSmallBucket particles in this sort
+    int32_t pad;
 };
 
 struct Fields {
@@ -125,6 +126,89 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double ceff, int 
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// True in exactly one workgroup of a launch: the last to arrive here.  Every
+// workgroup of the grid must call it (no early exit before it); the counter
+// is back at zero when the launch ends.  This runs a launch's small serial
+// tail (group control, grid set-up, a top-level scan) inside the launch
+// instead of as a kernel of its own.
+//
+// Visibility: each workgroup's memory operations have completed before it
+// arrives (a workgroup-scope release waits for them), but plain stores may
+// still sit in its XCD's L2.  A device-scope fence would write back and
+// invalidate the L2 in every workgroup (measured: it doubled the step), so
+// what a tail reads of its own launch's results is written with device
+// atomics and read with ld_dev / st_dev, which are coherent across the XCDs.
+//
+// Same-address atomics serialise (about 10 ns each), so a thousand
+// workgroups do not share one counter: they arrive in kArriveGroups groups
+// (blockIdx mod kArriveGroups, one 128-byte line per group counter) and the
+// last of each group arrives at the top counter.
+constexpr int kArriveGroups = 32;
+constexpr int kArriveStride = 32;  // int32 per counter line
+constexpr int kArriveInts = (kArriveGroups + 1) * kArriveStride;
+
+__device__ __forceinline__ bool last_to_arrive(int32_t* arrive) {
+    __shared__ int s_last;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = (int)(gridDim.x * gridDim.y * gridDim.z);
+        const int b = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+        const int g = b % kArriveGroups;
+        const int members = total / kArriveGroups + (g < total % kArriveGroups ? 1 : 0);
+        const int groups = total < kArriveGroups ? total : kArriveGroups;
+        int last = 0;
+        int32_t* gc = arrive + (1 + g) * kArriveStride;
+        if (atomicAdd(gc, 1) == members - 1) {
+            atomicExch(gc, 0);
+            if (atomicAdd(arrive, 1) == groups - 1) {
+                atomicExch(arrive, 0);
+                last = 1;
+            }
+        }
+        s_last = last;
+    }
+    __syncthreads();
+    return s_last;
+}
+
+// device-coherent load and store (bypass the non-coherent caches)
+template <class T>
+__device__ __forceinline__ T ld_dev(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_dev(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// start of an evaluation: control block and per-instance reductions cleared
+// (reset_dt: the pending dt too); run by one workgroup
+__device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
+    if (threadIdx.x == 0) {
+        ctl->all_done = 0;
+        ctl->status = 0;
+        ctl->max_nb = 0;
+        ctl->iters_max = 0;
+        ctl->pairs = 0;
+        ctl->count = 0;
+        ctl->act_cnt[0] = 0;
+        ctl->act_cnt[1] = 0;
+        ctl->overflow = 0;
+    }
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+        Inst& s = inst[k];
+        s.hmax_bits = 0ull;
+        s.hmax_next_bits = 0ull;
+        s.hmax_conv_bits = 0ull;
+        if (reset_dt) s.dtp_bits = 0x7FF0000000000000ull;  // +inf
+        s.iters = 0;
+        s.unconv = 0;
+        s.done = 0;
+        s.status = 0;
+    }
+}
 
 // stencil half-width per axis from the instance's current h_max
 __device__ inline void stencil_from_hmax(Inst& s) {
@@ -494,6 +578,8 @@ struct Dev {
     // device arrays
     Inst* inst;
     Ctl* ctl;
+    Ctl* hctl_dev;          // device view of the pinned host mirror of ctl
+    int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
     uint32_t *keys, *keys_alt;
     int32_t *perm, *perm_alt;
@@ -556,8 +642,7 @@ void launch_pack_xh(Dev& d);
 void launch_reset_nb_stats(Dev& d);
 void launch_reset_force_outputs(Dev& d);
 void launch_count_unconverged(Dev& d);
-void launch_hmax(Dev& d, bool next);
-void launch_grid(Dev& d);
+void launch_hmax_grid(Dev& d);
 void launch_keys_sort_ranges(Dev& d);
 void launch_pack_density(Dev& d);
 void launch_density(Dev& d);
@@ -568,6 +653,7 @@ void launch_prep_force(Dev& d, bool keep_prev);
 void launch_nb_build(Dev& d);
 void launch_force(Dev& d);
 void launch_eval_end(Dev& d);
+void launch_ctl_export(Dev& d);
 
 void launch_initialize(Dev& d);
 void launch_stage1(Dev& d);

@@ -23,28 +23,7 @@ namespace sph {
 // ---------------------------------------------------------------------------
 
 __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) {
-        ctl->all_done = 0;
-        ctl->status = 0;
-        ctl->max_nb = 0;
-        ctl->iters_max = 0;
-        ctl->pairs = 0;
-        ctl->count = 0;
-        ctl->act_cnt[0] = 0;
-        ctl->act_cnt[1] = 0;
-        ctl->overflow = 0;
-    }
-    if (k >= ninst) return;
-    Inst& s = inst[k];
-    s.hmax_bits = 0ull;
-    s.hmax_next_bits = 0ull;
-    s.hmax_conv_bits = 0ull;
-    if (reset_dt) s.dtp_bits = 0x7FF0000000000000ull;  // +inf
-    s.iters = 0;
-    s.unconv = 0;
-    s.done = 0;
-    s.status = 0;
+    eval_begin_body(inst, ninst, ctl, reset_dt);
 }
 
 // max h per instance.  Same-address atomics serialise, so the grid is small
@@ -57,8 +36,11 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
 constexpr int kHmaxBlocks = 128;
 constexpr int kHmaxBatch = 8;
 
+__device__ void grid_setup(Inst* inst, int ninst, Ctl* ctl);
+
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
-                       int64_t n, int next, int64_t chunk) {
+                       int64_t n, int next, int64_t chunk, int ninst, Ctl* ctl, int with_grid,
+                       int32_t* arrive) {
     __shared__ unsigned long long wmax[kBlock / kWave];
     __shared__ int wk[kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -114,17 +96,19 @@ __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__
             q = r;
         }
     }
+    if (with_grid && last_to_arrive(arrive)) grid_setup(inst, ninst, ctl);
 }
 
-// Grid set-up: one thread per instance, then a serial prefix of cell bases.
-__global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) {
+// Grid set-up (one workgroup): one thread per instance, then a serial prefix
+// of cell bases.
+__device__ void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
     int k = threadIdx.x;
     for (; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
         int64_t nk = s.off1 - s.off0;
         int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
         double R = kernel_radius(s.kernel);
-        double c = R * bitsd(s.hmax_bits);
+        double c = R * bitsd(ld_dev(&s.hmax_bits));  // reduced by this launch's atomics
         double lmax = 0.0;
         for (int a = 0; a < s.dim; ++a) lmax = fmax(lmax, s.L[a]);
         if (!(c > 0.0) || !(c <= lmax)) c = lmax;  // h_max of 0, NaN or above the box: one cell
@@ -159,6 +143,8 @@ __global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) {
         ctl->nbig = 0;
     }
 }
+
+__global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) { grid_setup(inst, ninst, ctl); }
 
 // ---------------------------------------------------------------------------
 // cell sort: one radix digit spanning the whole cell key (a counting sort).
@@ -228,51 +214,54 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
     return base + incl - v;
 }
 
-// per block of kScanCells cells: local exclusive/inclusive prefix into
-// cstart/cend, block total into cblock
-__global__ void __launch_bounds__(kBlock) k_cell_scan_local(const int32_t* __restrict__ ccount,
-                                                            int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
-                                                            int32_t* __restrict__ cblock, const Ctl* ctl) {
-    __shared__ int wsum[kBlock / kWave];
-    const int64_t total = ctl->total_cells;
-    const int64_t c0 = (int64_t)blockIdx.x * kScanCells;
-    if (c0 >= total) return;  // whole block: uniform
-    const int64_t cb = c0 + (int64_t)threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int sum = 0;
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q) {
-        v[q] = cb + q < total ? ccount[cb + q] : 0;
-        sum += v[q];
-    }
-    int btot;
-    int run = block_excl_scan(sum, wsum, btot);
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q) {
-        if (cb + q < total) {
-            cstart[cb + q] = run;
-            cend[cb + q] = run + v[q];
-        }
-        run += v[q];
-    }
-    if (threadIdx.x == 0) cblock[blockIdx.x] = btot;
-}
-
-// exclusive scan of the block totals (one block)
-__global__ void __launch_bounds__(kBlock) k_cell_scan_top(int32_t* __restrict__ cblock, const Ctl* ctl) {
+// exclusive scan of the block totals (one workgroup)
+__device__ void cell_scan_top(int32_t* __restrict__ cblock, const Ctl* ctl) {
     __shared__ int wsum[kBlock / kWave];
     const int64_t nb = (ctl->total_cells + kScanCells - 1) / kScanCells;
     const int64_t per = (nb + kBlock - 1) / kBlock;
     const int64_t a = (int64_t)threadIdx.x * per, b = a + per < nb ? a + per : nb;
     int sum = 0;
-    for (int64_t q = a; q < b; ++q) sum += cblock[q];
+    for (int64_t q = a; q < b; ++q) sum += ld_dev(&cblock[q]);  // written by this launch
     int tot;
     int run = block_excl_scan(sum, wsum, tot);
     for (int64_t q = a; q < b; ++q) {
-        const int c = cblock[q];
+        const int c = ld_dev(&cblock[q]);
         cblock[q] = run;
         run += c;
     }
+}
+
+// per block of kScanCells cells: local exclusive/inclusive prefix into
+// cstart/cend and the block total into cblock; the last workgroup to finish
+// scans the block totals
+__global__ void __launch_bounds__(kBlock) k_cell_scan(const int32_t* __restrict__ ccount,
+                                                      int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
+                                                      int32_t* __restrict__ cblock, Ctl* ctl, int32_t* arrive) {
+    __shared__ int wsum[kBlock / kWave];
+    const int64_t total = ctl->total_cells;
+    const int64_t c0 = (int64_t)blockIdx.x * kScanCells;
+    if (c0 < total) {  // whole workgroup: uniform
+        const int64_t cb = c0 + (int64_t)threadIdx.x * kScanItems;
+        int v[kScanItems];
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            v[q] = cb + q < total ? ccount[cb + q] : 0;
+            sum += v[q];
+        }
+        int btot;
+        int run = block_excl_scan(sum, wsum, btot);
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            if (cb + q < total) {
+                cstart[cb + q] = run;
+                cend[cb + q] = run + v[q];
+            }
+            run += v[q];
+        }
+        if (threadIdx.x == 0) st_dev(&cblock[blockIdx.x], btot);  // read by the tail
+    }
+    if (last_to_arrive(arrive)) cell_scan_top(cblock, ctl);
 }
 
 // each run takes a block of its cell's slots (filled from the end down, so
@@ -511,8 +500,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 void launch_eval_begin(Dev& d, bool reset_dt) {
-    hipLaunchKernelGGL(k_eval_begin, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl,
-                       reset_dt ? 1 : 0);
+    hipLaunchKernelGGL(k_eval_begin, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl, reset_dt ? 1 : 0);
 }
 
 void launch_pack_xh(Dev& d) {
@@ -529,16 +517,16 @@ void launch_reset_nb_stats(Dev& d) {
     hipLaunchKernelGGL(k_reset_nb_stats, dim3(1), dim3(1), 0, d.stream, d.ctl);
 }
 
-void launch_hmax(Dev& d, bool next) {
-    if (d.n == 0) return;
+// h_max per instance, then (the launch's last workgroup) the cell grid
+void launch_hmax_grid(Dev& d) {
+    if (d.n == 0) {
+        hipLaunchKernelGGL(k_grid, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+        return;
+    }
     const int64_t blocks = std::min<int64_t>(kHmaxBlocks, nblk(d.n));
     const int64_t chunk = (d.n + blocks - 1) / blocks;
     hipLaunchKernelGGL(k_hmax, dim3((unsigned)blocks), dim3(kBlock), 0, d.stream, d.fields.f[SPHMHD_F_H], d.inst_of,
-                       d.inst, d.n, next ? 1 : 0, chunk);
-}
-
-void launch_grid(Dev& d) {
-    hipLaunchKernelGGL(k_grid, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+                       d.inst, d.n, 0, chunk, d.ninst, d.ctl, 1, d.arrive);
 }
 
 constexpr int kBigBlocks = 64;
@@ -550,9 +538,8 @@ void launch_keys_sort_ranges(Dev& d) {
     hipLaunchKernelGGL(k_cell_count, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.perm,
                        d.keys_alt, d.ccount, d.n);
     const unsigned sblocks = (unsigned)((d.cell_cap + kScanCells - 1) / kScanCells);
-    hipLaunchKernelGGL(k_cell_scan_local, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cstart, d.cend,
-                       d.cblock, d.ctl);
-    hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(kBlock), 0, d.stream, d.cblock, d.ctl);
+    hipLaunchKernelGGL(k_cell_scan, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cstart, d.cend, d.cblock,
+                       d.ctl, d.arrive);
     hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
                        d.cstart, d.cblock, d.perm_alt, d.n);
     hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,

@@ -32,12 +32,12 @@ __device__ __forceinline__ double iroot_dim(double x, int dim) { return dim == 2
 // the neighbours' m, x), so skipping it is exact.  Particles that stay
 // unconverged are appended to act_out.
 template <int K, int DIM>
-__global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __restrict__ dpack,
-                                                    const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
-                                                    const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                                                    Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
-                                                    const int32_t* __restrict__ act_in, const int32_t* cnt_in,
-                                                    int32_t* __restrict__ act_out, int32_t* cnt_out) {
+__device__ __forceinline__ void density_block(Fields F, const double4* __restrict__ dpack,
+                                              const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
+                                              const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                                              Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
+                                              const int32_t* __restrict__ act_in, const int32_t* cnt_in,
+                                              int32_t* __restrict__ act_out, int32_t* cnt_out) {
     const int64_t idx0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
     if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
@@ -183,7 +183,8 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
 // Group iteration control: iterations >= min and converged, or == max.
 // Also clears the counter of the active list the next pass will write
 // (next_list), so passes can be launched back to back without the host.
-__global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
+// One workgroup.
+__device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) {
     __shared__ int alldone;
     if (threadIdx.x == 0) {
         alldone = 1;
@@ -193,12 +194,15 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
         if (!s.done) {
+            // unconv and the two h maxima were reduced by the pass's atomics
+            const int unconv = ld_dev(&s.unconv);
+            const unsigned long long hconv = ld_dev(&s.hmax_conv_bits), hnext = ld_dev(&s.hmax_next_bits);
             s.iters += 1;
-            bool conv = s.unconv == 0;
+            bool conv = unconv == 0;
             bool stop = !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
             if (s.density_iterate && !(s.iters >= s.min_iter && conv) && s.iters >= s.max_iter) s.status |= 1;
             s.unconv = 0;
-            s.hmax_bits = s.hmax_conv_bits > s.hmax_next_bits ? s.hmax_conv_bits : s.hmax_next_bits;
+            s.hmax_bits = hconv > hnext ? hconv : hnext;
             s.hmax_next_bits = 0ull;
             stencil_from_hmax(s);
             if (stop) s.done = 1;
@@ -208,6 +212,23 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
     }
     __syncthreads();
     if (threadIdx.x == 0) ctl->all_done = alldone;
+}
+
+__global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
+    g0_control_body(inst, ninst, ctl, next_list);
+}
+
+// one density pass; its last workgroup runs the group control
+template <int K, int DIM>
+__global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __restrict__ dpack,
+                                                    const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
+                                                    const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                                                    Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
+                                                    const int32_t* __restrict__ act_in, const int32_t* cnt_in,
+                                                    int32_t* __restrict__ act_out, int32_t* cnt_out, Ctl* ctl,
+                                                    int ninst, int next_list, int32_t* arrive) {
+    density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out, cnt_out);
+    if (last_to_arrive(arrive)) g0_control_body(inst, ninst, ctl, next_list);
 }
 
 // ---------------------------------------------------------------------------
@@ -319,10 +340,10 @@ struct Acc {
 };
 
 template <int K, int DIM>
-__global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
-                                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
-                                                  const int32_t* __restrict__ nbcnt, Inst* inst,
-                                                  const int32_t* __restrict__ inst_of, int64_t n, int stride) {
+__device__ __forceinline__ void force_block(Fields F, const double* __restrict__ fpack,
+                                            const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
+                                            const int32_t* __restrict__ nbcnt, Inst* inst,
+                                            const int32_t* __restrict__ inst_of, int64_t n, int stride) {
     using R = FRec<DIM>;
     constexpr int U = R::UNITS;
     const double2* __restrict__ pk = (const double2*)fpack;
@@ -679,10 +700,41 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
     }
 }
 
-// collect per-instance warning bits (1: density hit max_iterations, 2: NaN)
-__global__ void k_eval_end(const Inst* __restrict__ inst, int ninst, Ctl* ctl) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < ninst && inst[k].status) atomicOr(&ctl->status, inst[k].status);
+// end of an evaluation (one workgroup): the instances' warning bits (1:
+// density hit max_iterations, 2: NaN) into ctl->status, then ctl into its
+// host mirror, which the host reads after the stream's wait
+__device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) {
+    __shared__ int wst[kBlock / kWave];
+    int st = 0;
+    for (int k = threadIdx.x; k < ninst; k += kBlock) st |= ld_dev(&inst[k].status);  // force pass's atomics
+    for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) wst[threadIdx.x / kWave] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; ++w) st |= wst[w];
+        Ctl v = *ctl;
+        v.status |= st;
+        ctl->status = v.status;
+        *hout = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_eval_end(const Inst* __restrict__ inst, int ninst, Ctl* ctl,
+                                                     Ctl* hout) {
+    eval_end_body(inst, ninst, ctl, hout);
+}
+
+__global__ void k_ctl_export(const Ctl* ctl, Ctl* hout) { *hout = *ctl; }
+
+// the fused force pass; its last workgroup ends the evaluation
+template <int K, int DIM>
+__global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
+                                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
+                                                  const int32_t* __restrict__ nbcnt, Inst* inst,
+                                                  const int32_t* __restrict__ inst_of, int64_t n, int stride,
+                                                  int ninst, Ctl* ctl, Ctl* hout, int32_t* arrive) {
+    force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride);
+    if (last_to_arrive(arrive)) eval_end_body(inst, ninst, ctl, hout);
 }
 
 // ---------------------------------------------------------------------------
@@ -703,7 +755,8 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kB
     }
 
 // pass 0 scans every particle and writes list 0; pass p > 0 reads list
-// (p - 1) % 2 and writes list p % 2
+// (p - 1) % 2 and writes list p % 2; the pass ends with the group control
+// (the next pass writes list (p + 1) % 2)
 void launch_density(Dev& d) {
     if (d.n == 0) return;
     const int p = d.pass;
@@ -714,7 +767,7 @@ void launch_density(Dev& d) {
 #define CALL_DENS(KK, DD)                                                                                   \
     { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
-                         ain, cin, aout, cout); }
+                         ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS
 }
@@ -731,14 +784,19 @@ void launch_prep_force(Dev& d, bool keep_prev) {
 }
 
 void launch_eval_end(Dev& d) {
-    hipLaunchKernelGGL(k_eval_end, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl);
+    hipLaunchKernelGGL(k_eval_end, dim3(1), dim3(kBlock), 0, d.stream, d.inst, d.ninst, d.ctl, d.hctl_dev);
+}
+
+void launch_ctl_export(Dev& d) {
+    hipLaunchKernelGGL(k_ctl_export, dim3(1), dim3(1), 0, d.stream, d.ctl, d.hctl_dev);
 }
 
 void launch_force(Dev& d) {
     if (d.n == 0) return;
 #define CALL_FORCE(KK, DD)                                                                                  \
     { hipLaunchKernelGGL((k_force<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
-                         d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride); }
+                         d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride, d.ninst, d.ctl,          \
+                         d.hctl_dev, d.arrive); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)
 #undef CALL_FORCE
 }

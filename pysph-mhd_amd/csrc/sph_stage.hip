@@ -131,10 +131,8 @@ __global__ void k_stage2(Fields F, const Inst* __restrict__ inst, const int32_t*
 // expression (so it rounds identically), but every field is loaded once into
 // registers and every output stored once: through the property pointers the
 // compiler cannot tell the fields apart and would reload and re-store them.
-__global__ void k_stage2_init_stage1(Fields F, const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
-                                     int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void stage2_init_stage1_one(const Fields& F, const Inst* __restrict__ inst,
+                                                       const int32_t* __restrict__ inst_of, int64_t i) {
     const Inst& s = inst[inst_of[i]];
     const double dt = s.dt;
     const double dtn = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
@@ -241,13 +239,31 @@ __global__ void k_stage2_init_stage1(Fields F, const Inst* __restrict__ inst, co
     FLD(PSIPRED) = psip;
 }
 
-__global__ void k_clock(Inst* inst, int ninst) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ninst) return;
-    Inst& s = inst[k];
+// t += dt; dt = the pending step clipped to tf (solver.py:57-61)
+__device__ __forceinline__ void advance_clock(Inst& s) {
     const double t = s.t + s.dt;
     s.dt = next_dt(s, t, s.dt, bitsd(s.dtp_bits));
     s.t = t;
+}
+
+// the last workgroup then commits the clock and starts the next evaluation
+// (eval_begin_body with the pending dt cleared), which would otherwise be
+// two kernels of one thread per instance
+__global__ void k_stage2_init_stage1(Fields F, Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
+                                     int ninst, Ctl* ctl, int32_t* arrive) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) stage2_init_stage1_one(F, inst, inst_of, i);
+    if (last_to_arrive(arrive)) {
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) advance_clock(inst[k]);
+        __syncthreads();
+        eval_begin_body(inst, ninst, ctl, 1);
+    }
+}
+
+__global__ void k_clock(Inst* inst, int ninst) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    advance_clock(inst[k]);
 }
 
 // application.py:134-147 followed by the solver's first _get_timestep
@@ -356,7 +372,7 @@ void launch_stage2(Dev& d) {
 void launch_stage2_init_stage1(Dev& d) {
     if (d.n)
         hipLaunchKernelGGL(k_stage2_init_stage1, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
-                           d.inst_of, d.n);
+                           d.inst_of, d.n, d.ninst, d.ctl, d.arrive);
 }
 void launch_clock(Dev& d) {
     hipLaunchKernelGGL(k_clock, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);
