@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 4
+#define SPHMHD_ABI_VERSION 5
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -130,6 +130,10 @@ typedef struct sphmhd_stats {
     int64_t force_redos;         /* force passes redone (iteration needed more passes than
                                     were launched ahead, or a neighbour list grew) */
     int64_t graph_replays;       /* steps whose launches were replayed from a captured hipGraph */
+    int32_t block_density;       /* workgroup size of the density passes */
+    int32_t block_lists;         /* workgroup size of the prep + neighbour-list launches */
+    int32_t block_force;         /* workgroup size of the force pass */
+    int32_t launch_tuned;        /* 1 once the launch shapes were chosen by timing */
 } sphmhd_stats;
 
 /* Device time of the hot kernels, measured with HIP events recorded on the
@@ -137,7 +141,7 @@ typedef struct sphmhd_stats {
 typedef struct sphmhd_timing {
     double force_ms;          /* fused force pass (G3+G5..G9) */
     double density_ms;        /* density + Newton passes (G0) */
-    double nnps_ms;           /* cell keys + radix sort + cell table */
+    double nnps_ms;           /* cell keys + cell sort + cell table */
     double lists_ms;          /* G1/G2/G4 finalize + force pack + neighbour lists */
     double stage_ms;          /* integrator stage kernels */
     int64_t force_launches;
@@ -225,6 +229,17 @@ int sphmhd_get_timing(sphmhd_ctx* ctx, sphmhd_timing* out);
 
 /* Block until the context's stream is idle. */
 int sphmhd_synchronize(sphmhd_ctx* ctx);
+
+/* Launch shapes.  No counterpart in the reference (PySPH picks its own launch
+ * sizes); the results of a step do not depend on them, only its speed does.
+ * blocks[3] = workgroup sizes of the density passes, the prep + neighbour-list
+ * launches and the force pass, each 64, 128 or 256.  By default the first
+ * steps of a context's first sphmhd_step calls run every shape (with HIP
+ * events; the results are those of any shape) and keep the fastest per
+ * launch; sphmhd_set_launch fixes them instead, and blocks == NULL tunes
+ * again.  The environment variable SPHMHD_TUNE=0 disables tuning (all 256). */
+int sphmhd_set_launch(sphmhd_ctx* ctx, const int32_t* blocks);
+int sphmhd_get_launch(sphmhd_ctx* ctx, int32_t* blocks, int32_t* tuned);
 
 /* ---- Tangent-linear step -------------------------------------------------
  * No counterpart in the reference, which has no derivatives; it serves the

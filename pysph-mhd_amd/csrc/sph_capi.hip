@@ -102,7 +102,22 @@ struct sphmhd_ctx {
     // tangent-linear step
     TanDev tg;
     bool tbound = false;
+    // launch-shape tuner: the first kTuneSteps steps of sphmhd_step rotate
+    // the workgroup size of the tuned launches through kShapes (the results
+    // do not depend on it) and time each launch class with HIP events
+    struct Tuner {
+        int state = 0;  // 0: not started, 1: running, 2: tuned, 3: fixed (caller or SPHMHD_TUNE=0)
+        int step = 0;
+        double best_ms[kLaunchClasses][3];
+        bool saved_timing = false;
+        double saved_ms[5];
+        int64_t saved_count[5];
+    } tune;
 };
+
+static const int kShapes[3] = {64, 128, 256};
+constexpr int kTuneRounds = 2;
+constexpr int kTuneSteps = 3 * kTuneRounds;
 
 enum { T_FORCE = 0, T_DENSITY, T_NNPS, T_LISTS, T_STAGE };
 
@@ -136,6 +151,72 @@ static void tcollect(sphmhd_ctx* c) {
     // a failed query only drops that sample: keep it out of the sticky error
     // state that launch checks read
     (void)hipGetLastError();
+}
+
+// ---- launch-shape tuner (see sphmhd_set_launch) ----
+static int tune_events(sphmhd_ctx* c) {
+    for (PhaseTimer& t : c->tm)
+        if (!t.a) {
+            HIPCHK(hipEventCreate(&t.a));
+            HIPCHK(hipEventCreate(&t.b));
+        }
+    return 0;
+}
+
+// before a tuning step: this step's shape, fresh sums (the caller's saved)
+static int tune_begin_step(sphmhd_ctx* c) {
+    auto& T = c->tune;
+    if (T.state == 0) {
+        int rc = tune_events(c);
+        if (rc) return rc;
+        T.state = 1;
+        T.step = 0;
+        for (auto& row : T.best_ms)
+            for (double& v : row) v = 1.0e300;
+        T.saved_timing = c->timing;
+        for (int k = 0; k < 5; ++k) {
+            T.saved_ms[k] = c->tm[k].ms;
+            T.saved_count[k] = c->tm[k].count;
+        }
+    }
+    const int b = kShapes[T.step % 3];
+    for (int q = 0; q < kLaunchClasses; ++q) c->d.blk[q] = b;
+    c->timing = true;
+    for (PhaseTimer& t : c->tm) {
+        t.ms = 0.0;
+        t.count = 0;
+        t.pending = false;
+    }
+    return 0;
+}
+
+// after it (the evaluation's wait has collected the events): keep the
+// fastest sample per shape and class; after the last one pick the shapes
+// (a smaller workgroup must be 3% faster to replace a larger one) and give
+// the caller back its timing mode and sums
+static void tune_end_step(sphmhd_ctx* c) {
+    auto& T = c->tune;
+    const int s = T.step % 3;
+    const int timer[kLaunchClasses] = {T_DENSITY, T_LISTS, T_FORCE};
+    for (int q = 0; q < kLaunchClasses; ++q) {
+        const PhaseTimer& t = c->tm[timer[q]];
+        if (t.count > 0) T.best_ms[q][s] = std::min(T.best_ms[q][s], t.ms);
+    }
+    if (++T.step < kTuneSteps) return;
+    for (int q = 0; q < kLaunchClasses; ++q) {
+        int best = 2;
+        for (int k = 1; k >= 0; --k)
+            if (T.best_ms[q][k] < 0.97 * T.best_ms[q][best]) best = k;
+        c->d.blk[q] = kShapes[best];
+    }
+    T.state = 2;
+    c->timing = T.saved_timing;
+    for (int k = 0; k < 5; ++k) {
+        c->tm[k].ms = T.saved_ms[k];
+        c->tm[k].count = T.saved_count[k];
+        c->tm[k].pending = false;
+    }
+    c->graph_gen += 1;  // captured steps hold the old shapes
 }
 
 static int read_ctl(sphmhd_ctx* c) {
@@ -473,6 +554,9 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         d.kernel = p0.kernel;
         d.stream = (hipStream_t)hip_stream;
         if (const char* g = std::getenv("SPHMHD_GRAPHS")) c->use_graphs = std::strcmp(g, "0") != 0;
+        for (int q = 0; q < kLaunchClasses; ++q) d.blk[q] = kBlock;
+        if (const char* g = std::getenv("SPHMHD_TUNE"))
+            if (std::strcmp(g, "0") == 0) c->tune.state = 3;
         int64_t cap = 0;
         for (int k = 0; k < n_instances; ++k) {
             int64_t nk = offsets[k + 1] - offsets[k];
@@ -687,12 +771,17 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
             swap.on = true;
         }
         for (int64_t s = 0; s < nsteps; ++s) {
+            // tuning steps are launched one by one with events around the phases
+            const bool tuning = c->tune.state <= 1;
+            if (tuning && (rc = tune_begin_step(c))) return rc;
             const int spec = next_spec(c);
-            rc = (s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec) : step_front(c, s == 0, spec);
+            rc = (!tuning && s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec)
+                                                                 : step_front(c, s == 0, spec);
             if (rc < 0) return rc;
             rc = eval_finish(c, spec);
             if (rc < 0) return rc;
             if (rc > worst) worst = rc;
+            if (tuning) tune_end_step(c);
         }
         launch_stage2(c->d);
         launch_clock(c->d);
@@ -843,6 +932,33 @@ int sphmhd_neighbours(sphmhd_ctx* c, int64_t* offs, int32_t* idx, int64_t cap, i
 int sphmhd_get_stats(sphmhd_ctx* c, sphmhd_stats* out) {
     if (!c || !out) return fail(SPHMHD_EINVAL, "null argument");
     *out = c->stats;
+    out->block_density = c->d.blk[LC_DENSITY];
+    out->block_lists = c->d.blk[LC_LISTS];
+    out->block_force = c->d.blk[LC_FORCE];
+    out->launch_tuned = c->tune.state == 2 ? 1 : 0;
+    return 0;
+}
+
+int sphmhd_set_launch(sphmhd_ctx* c, const int32_t* blocks) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    if (c->tune.state == 1) return fail(SPHMHD_ESTATE, "launch shapes are being tuned");
+    if (blocks) {
+        for (int q = 0; q < kLaunchClasses; ++q)
+            if (blocks[q] != 64 && blocks[q] != 128 && blocks[q] != 256)
+                return fail(SPHMHD_EINVAL, "workgroup sizes must be 64, 128 or 256");
+        for (int q = 0; q < kLaunchClasses; ++q) c->d.blk[q] = blocks[q];
+        c->tune.state = 3;
+    } else {
+        c->tune.state = 0;
+    }
+    c->graph_gen += 1;
+    return 0;
+}
+
+int sphmhd_get_launch(sphmhd_ctx* c, int32_t* blocks, int32_t* tuned) {
+    if (!c || !blocks) return fail(SPHMHD_EINVAL, "null argument");
+    for (int q = 0; q < kLaunchClasses; ++q) blocks[q] = c->d.blk[q];
+    if (tuned) *tuned = c->tune.state == 2 ? 1 : 0;
     return 0;
 }
 

@@ -21,8 +21,10 @@
 
 namespace sph {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // workgroup size (the largest shape of the tuned launches)
 constexpr int kWave = 64;
+// launch classes whose workgroup size is tuned per context (64, 128 or 256)
+enum { LC_DENSITY = 0, LC_LISTS, LC_FORCE, kLaunchClasses };
 constexpr int kScanItems = 4;    // cells per thread of the cell-count scan
 constexpr int kScanCells = 256 * kScanItems;  // cells per scan block
 constexpr int kSmallBucket = 64; // cells up to this size are ordered by rank, one thread per particle
@@ -311,7 +313,7 @@ __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, in
 // contains it in sorted-index space, so the copy always holds exactly the
 // records the range names; anything else (more rows, wide stencils, an
 // instance boundary, too many records) is read from global memory as before.
-constexpr int kStageCap = 1024;     // staged 32-byte records per block (32 KB)
+constexpr int kStageCap = 1024;     // staged 32-byte records per block of kBlock threads (32 KB)
 constexpr int kStageSegs = 2;       // cell rows a staged group may span
 
 // runs staged for one group of particles: a[k] <= j < b[k] sits at off[k] + j - a[k]
@@ -580,6 +582,7 @@ struct Dev {
     Ctl* ctl;
     Ctl* hctl_dev;          // device view of the pinned host mirror of ctl
     int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
+    int32_t blk[3];         // workgroup size of the density, list (prep + build) and force launches
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
     uint32_t *keys, *keys_alt;
     int32_t *perm, *perm_alt;

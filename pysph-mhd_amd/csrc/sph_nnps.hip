@@ -424,16 +424,17 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
-                                                     int stride, Ctl* ctl) {
-    // 2-D blocks stage their candidates in LDS (measured 7-20 % faster); in
-    // 3-D the staged runs rarely fit and the LDS would cost occupancy
-    constexpr int kCap = DIM == 2 ? kStageCap : 1;
+                                                     int stride, Ctl* ctl, int cap) {
+    // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
+    // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
+    // rarely fit and the LDS would cost occupancy (cap 0)
     __shared__ StageHdr H;
-    __shared__ double4 stage[kCap];
-    const int64_t t0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
+    extern __shared__ double4 stage[];
+    const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
+    const int64_t t0 = xcd_tile((n + B - 1) / B) * B;
     if (t0 >= n) return;  // whole block past the end (the block barriers below need every wave)
     if constexpr (DIM == 2) {
-        if (threadIdx.x < kWave) stage_plan<DIM, kWave>(H, skey, cstart, cend, inst, inst_of, t0, kBlock, n, kStageCap);
+        if (threadIdx.x < kWave) stage_plan<DIM, kWave>(H, skey, cstart, cend, inst, inst_of, t0, B, n, cap);
     } else {
         if (threadIdx.x == 0) H.on = 0;
     }
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kBlock / kWave; ++w) {
+        for (int w = 1; w < B / kWave; ++w) {
             m = max(m, bm[w]);
             pc += bpc[w];
         }
@@ -558,14 +559,17 @@ void launch_pack_density(Dev& d) {
 
 void launch_nb_build(Dev& d) {
     if (d.n == 0) return;
-    if (d.dim == 2)
-        hipLaunchKernelGGL(k_nb_build<2>, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
-                           d.cstart,
-                           d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
-    else
-        hipLaunchKernelGGL(k_nb_build<3>, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
-                           d.cstart,
-                           d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl);
+    const int B = d.blk[LC_LISTS];
+    const unsigned grid = xcd_grid((d.n + B - 1) / B);
+    if (d.dim == 2) {
+        const int cap = kStageCap * B / kBlock;
+        hipLaunchKernelGGL(k_nb_build<2>, dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
+                           (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
+                           d.nb_stride, d.ctl, cap);
+    } else {
+        hipLaunchKernelGGL(k_nb_build<3>, dim3(grid), dim3(B), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
+                           d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl, 0);
+    }
 }
 
 }  // namespace sph

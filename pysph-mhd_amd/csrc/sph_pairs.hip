@@ -38,7 +38,8 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
                                               Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                               const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                               int32_t* __restrict__ act_out, int32_t* cnt_out) {
-    const int64_t idx0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
+    const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
+    const int64_t idx0 = xcd_tile((n + B - 1) / B) * B;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
     if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
     const int64_t idx = idx0 + threadIdx.x;
@@ -114,7 +115,8 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
     // atomic per quantity and block (per instance run when a block straddles
     // instances; a wave that straddles instances falls back to lane atomics).
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    constexpr int NW = kBlock / kWave;
+    constexpr int NW = kBlock / kWave;  // LDS slots: waves of the largest shape
+    const int nw = blockDim.x / kWave;
     __shared__ int wcnt[NW], wbase[NW], wk[NW], wuc[NW];
     __shared__ unsigned long long whc[NW], whu[NW];
     const unsigned long long um = __ballot(unconv);
@@ -147,20 +149,20 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
         int tot = 0;
-        for (int q = 0; q < NW; ++q) {
+        for (int q = 0; q < nw; ++q) {
             wbase[q] = tot;
             tot += wcnt[q];
         }
         if (tot) {
             const int base = atomicAdd(cnt_out, tot);
-            for (int q = 0; q < NW; ++q) wbase[q] += base;
+            for (int q = 0; q < nw; ++q) wbase[q] += base;
         }
-        for (int q = 0; q < NW;) {
+        for (int q = 0; q < nw;) {
             const int k = wk[q];
             unsigned long long c = whc[q], u = whu[q];
             int cu = wuc[q];
             int r = q + 1;
-            for (; r < NW && wk[r] == k; ++r) {
+            for (; r < nw && wk[r] == k; ++r) {
                 c = whc[r] > c ? whc[r] : c;
                 u = whu[r] > u ? whu[r] : u;
                 cu += wuc[r];
@@ -271,7 +273,7 @@ __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, i
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
                              double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev) {
-    int64_t t = xcd_tile((n + kBlock - 1) / kBlock) * blockDim.x + threadIdx.x;
+    int64_t t = xcd_tile((n + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
     const int32_t i = perm[t];
@@ -348,7 +350,8 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
     constexpr int U = R::UNITS;
     const double2* __restrict__ pk = (const double2*)fpack;
 
-    const int64_t t0 = xcd_tile((n + kBlock - 1) / kBlock) * kBlock;
+    const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
+    const int64_t t0 = xcd_tile((n + B - 1) / B) * B;
     if (t0 >= n) return;  // whole block past the end (the block barrier below needs every wave)
     const int64_t t = t0 + threadIdx.x;
     const bool valid = t < n;
@@ -658,7 +661,8 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
 
     // min dt_adapt and the NaN flag per instance: wave, then block, then one
     // atomic per instance run of the block (same-address atomics serialise)
-    constexpr int NW = kBlock / kWave;
+    constexpr int NW = kBlock / kWave;  // LDS slots: waves of the largest shape
+    const int nw = blockDim.x / kWave;
     __shared__ unsigned long long wdb[NW];
     __shared__ int wk[NW], wbad[NW];
     const int w = threadIdx.x / kWave;
@@ -682,12 +686,12 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 0; q < NW;) {
+        for (int q = 0; q < nw;) {
             const int k = wk[q];
             unsigned long long m = wdb[q];
             int b = wbad[q];
             int r = q + 1;
-            for (; r < NW && wk[r] == k; ++r) {
+            for (; r < nw && wk[r] == k; ++r) {
                 m = wdb[r] < m ? wdb[r] : m;
                 b |= wbad[r];
             }
@@ -706,12 +710,12 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
 __device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) {
     __shared__ int wst[kBlock / kWave];
     int st = 0;
-    for (int k = threadIdx.x; k < ninst; k += kBlock) st |= ld_dev(&inst[k].status);  // force pass's atomics
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) st |= ld_dev(&inst[k].status);  // force pass's atomics
     for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
     if ((threadIdx.x & (kWave - 1)) == 0) wst[threadIdx.x / kWave] = st;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kBlock / kWave; ++w) st |= wst[w];
+        for (int w = 1; w < (int)blockDim.x / kWave; ++w) st |= wst[w];
         Ctl v = *ctl;
         v.status |= st;
         ctl->status = v.status;
@@ -741,7 +745,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
 // launchers (kernel x dim dispatch)
 // ---------------------------------------------------------------------------
 
-static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
 #define SPH_DISPATCH(KID, DIMV, CALL)                                                    \
     switch (KID) {                                                                       \
@@ -764,8 +768,9 @@ void launch_density(Dev& d) {
     const int32_t* cin = p == 0 ? nullptr : &d.ctl->act_cnt[(p - 1) & 1];
     int32_t* aout = d.act[p & 1];
     int32_t* cout = &d.ctl->act_cnt[p & 1];
+    const int B = d.blk[LC_DENSITY];
 #define CALL_DENS(KK, DD)                                                                                   \
-    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, \
+    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields,   \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
                          ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
@@ -779,7 +784,8 @@ void launch_g0_control(Dev& d) {
 
 void launch_prep_force(Dev& d, bool keep_prev) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(k_prep_force, dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.perm, d.inst,
+    const int B = d.blk[LC_LISTS];
+    hipLaunchKernelGGL(k_prep_force, dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields, d.perm, d.inst,
                        d.inst_of, d.fpack, (double4*)d.dpack, d.n, d.aprev, keep_prev ? 1 : 0);
 }
 
@@ -793,8 +799,9 @@ void launch_ctl_export(Dev& d) {
 
 void launch_force(Dev& d) {
     if (d.n == 0) return;
+    const int B = d.blk[LC_FORCE];
 #define CALL_FORCE(KK, DD)                                                                                  \
-    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(xcd_grid(nblk(d.n))), dim3(kBlock), 0, d.stream, d.fields, d.fpack, \
+    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields, d.fpack, \
                          d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride, d.ninst, d.ctl,          \
                          d.hctl_dev, d.arrive); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)

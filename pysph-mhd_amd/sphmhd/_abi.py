@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -108,6 +108,10 @@ class Stats(ctypes.Structure):
         ("max_neighbours", ctypes.c_int32),
         ("force_redos", ctypes.c_int64),
         ("graph_replays", ctypes.c_int64),
+        ("block_density", ctypes.c_int32),
+        ("block_lists", ctypes.c_int32),
+        ("block_force", ctypes.c_int32),
+        ("launch_tuned", ctypes.c_int32),
     ]
 
 
@@ -234,6 +238,8 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_tangent_totals": ([vp, dp], ctypes.c_int),
         "sphmhd_tangent_get_dt": ([vp, dp], ctypes.c_int),
         "sphmhd_set_timing": ([vp, i32], ctypes.c_int),
+        "sphmhd_set_launch": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "sphmhd_get_launch": ([vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_get_timing": ([vp, ctypes.POINTER(Timing)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():

@@ -178,6 +178,22 @@ class DeviceSystem:
         check(self.lib.sphmhd_get_stats(self._ctx, ctypes.byref(s)))
         return {name: getattr(s, name) for name, _ in Stats._fields_}
 
+    def set_launch(self, blocks=None) -> None:
+        """Workgroup sizes (density, lists, force), each 64, 128 or 256; None
+        tunes them again in the next step call (the default: the first steps
+        time every shape and keep the fastest; results never depend on it)."""
+        if blocks is None:
+            check(self.lib.sphmhd_set_launch(self._ctx, None))
+        else:
+            arr = (ctypes.c_int32 * 3)(*[int(b) for b in blocks])
+            check(self.lib.sphmhd_set_launch(self._ctx, arr))
+
+    def launch(self) -> tuple[tuple[int, int, int], bool]:
+        arr = (ctypes.c_int32 * 3)()
+        tuned = ctypes.c_int32(0)
+        check(self.lib.sphmhd_get_launch(self._ctx, arr, ctypes.byref(tuned)))
+        return (arr[0], arr[1], arr[2]), bool(tuned.value)
+
     def set_timing(self, enable: bool = True) -> None:
         """Enable HIP-event timing of the hot kernels (resets the sums)."""
         check(self.lib.sphmhd_set_timing(self._ctx, 1 if enable else 0))

@@ -137,6 +137,7 @@ def test_fused_multistep_equals_single_steps(gpu):
     a = DeviceSystem(s.params, s.offsets)
     b = DeviceSystem(s.params, s.offsets)
     for d in (a, b):
+        d.set_launch((256, 256, 256))  # no tuning steps: they are launched one by one
         d.set_fields(init_fields(s))
         d.warm_start()
     a.step(6)
@@ -161,6 +162,7 @@ def test_graph_replays_equal_plain_launches(gpu, monkeypatch):
     monkeypatch.setenv("SPHMHD_GRAPHS", "1")
     graph = DeviceSystem(s.params, s.offsets)
     for d in (plain, graph):
+        d.set_launch((256, 256, 256))  # no tuning steps: they are launched one by one
         d.set_fields(init_fields(s))
         d.warm_start()
     for rnd in range(4):
