@@ -57,7 +57,8 @@ struct Inst {
     // static parameters (filled at create)
     int64_t off0, off1;
     int32_t dim, kernel, density_iterate, min_iter;
-    int32_t max_iter, periodic, adaptive, pad0;
+    int32_t max_iter, periodic, adaptive;
+    int32_t range_shift;  // the stencil's per-range image shifts equal the minimum image (see stencil_from_hmax)
     double lo[3], hi[3], L[3], halfL[3];
     double gamma, gamma1, mu0, hfact;
     double av, ac, ar, chf;
@@ -213,17 +214,26 @@ __device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int rese
 }
 
 // stencil half-width per axis from the instance's current h_max
+// range_shift: a candidate of stencil cell offset k along an axis lies within
+// (|k| + 1) cell widths of the particle, so when every axis has more than
+// 2 (s + 1) cells (s the stencil half-width) no unwrapped candidate is more
+// than half a box away and every wrapped one is: the minimum image of a whole
+// stencil range is then one shift (0 or +-L), and adding it gives the very
+// operation min_image performs (d + (-L) == d - L in IEEE arithmetic)
 __device__ inline void stencil_from_hmax(Inst& s) {
     double R = kernel_radius(s.kernel);
     double hm = bitsd(s.hmax_bits);
+    int shift_ok = 1;
     for (int a = 0; a < 3; ++a) {
         if (a < s.dim) {
             double r = ceil(R * hm / s.ceff[a]);
             s.srad[a] = !(r < 1.0e6) ? 1000000 : (r < 1.0 ? 1 : (int)r);
+            if (s.periodic && !((int64_t)s.n[a] > 2 * ((int64_t)s.srad[a] + 1))) shift_ok = 0;
         } else {
             s.srad[a] = 0;
         }
     }
+    s.range_shift = shift_ok;
 }
 
 // ---------------------------------------------------------------------------
@@ -252,8 +262,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t tiles) {
 // cstart[c] / cend[c] hold, for every cell c (empty cells included), the
 // sorted positions [first key >= c, first key > c), so the x-adjacent cells of
 // one stencil row form ONE contiguous range [cstart[first], cend[last]).
-// for_each_row_range calls f(a, b) for each such range in cell order (z, y
-// outer, x inner), splitting a row in two where it wraps a periodic boundary.
+// for_each_row_range calls f(a, b, sx, sy, sz) for each such range in cell
+// order (z, y outer, x inner), splitting a row in two where it wraps a
+// periodic boundary; (sx, sy, sz) is the range's image shift (0 or +-L per
+// axis: +L where the stencil wrapped below 0, -L above n), which is the
+// minimum image of each of its candidates when s.range_shift is set.
 template <int DIM, class F>
 __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
                                                    const int32_t* __restrict__ cend, F&& f) {
@@ -276,12 +289,17 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
     if (lo[0] < 0) { xa[0] = lo[0] + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0]; nseg = 2; }
     else if (hi[0] >= n0) { xa[0] = lo[0]; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0] - n0; nseg = 2; }
     else { xa[0] = lo[0]; xb[0] = hi[0]; nseg = 1; }
+    double sxg[2] = {0.0, 0.0};
+    if (lo[0] < 0) sxg[0] = s.L[0];
+    else if (hi[0] >= n0) sxg[1] = -s.L[0];
     for (int qz = lo[2]; qz <= hi[2]; ++qz) {
         const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+        const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
         for (int qy = lo[1]; qy <= hi[1]; ++qy) {
             const int wy = ((qy % n1) + n1) % n1;
+            const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
             const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
-            for (int g = 0; g < nseg; ++g) f(cstart[row + xa[g]], cend[row + xb[g]]);
+            for (int g = 0; g < nseg; ++g) f(cstart[row + xa[g]], cend[row + xb[g]], sxg[g], sy, sz);
         }
     }
 }
@@ -429,8 +447,9 @@ __device__ __forceinline__ void stage_fill(const StageRuns<SLOTS>& H, T* stage, 
     }
 }
 
-// for_each_row_range over the staged copy: f(j, rec_j) for every candidate j,
-// from LDS where a staged run holds the range
+// for_each_row_range over the staged copy: f(j, rec_j, sx, sy, sz) for every
+// candidate j (with its range's image shift, as for_each_row_range), from LDS
+// where a staged run holds the range
 template <int DIM, class T, class F>
 __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
                                                    const int32_t* __restrict__ cend, const StageHdr& H,
@@ -446,12 +465,17 @@ __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, c
     if (DIM == 3) axis_span(cz, cz, s.srad[2], n2, s.periodic, lo2, hi2);
     int xa[2], xb[2];
     const int nseg = x_pieces(lo0, hi0, n0, xa, xb);
+    double sxg[2] = {0.0, 0.0};
+    if (lo0 < 0) sxg[0] = s.L[0];
+    else if (hi0 >= n0) sxg[1] = -s.L[0];
     const int64_t seg = local / n0 - H.r0;
     const bool staged = H.on && seg >= 0 && seg < kStageSegs;
     for (int qz = lo2; qz <= hi2; ++qz) {
         const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+        const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
         for (int qy = lo1; qy <= hi1; ++qy) {
             const int wy = ((qy % n1) + n1) % n1;
+            const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
             const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
             const int base = (((int)seg * H.nz + (qz - lo2)) * H.ny + (qy - lo1)) * 2;
             for (int g = 0; g < nseg; ++g) {
@@ -461,20 +485,22 @@ __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, c
                     if (H.a[base] <= a && b <= H.b[base]) k = base;
                     else if (H.a[base + 1] <= a && b <= H.b[base + 1]) k = base + 1;
                 }
+                const double sx = sxg[g];
+                auto fs = [&](int j, const T& o) { f(j, o, sx, sy, sz); };
                 if (k >= 0) {
                     const int delta = H.off[k] - H.a[k];
                     int j = a;
                     for (; j + 4 <= b; j += 4) {
                         const T o0 = stage[j + delta], o1 = stage[j + 1 + delta];
                         const T o2 = stage[j + 2 + delta], o3 = stage[j + 3 + delta];
-                        f(j, o0);
-                        f(j + 1, o1);
-                        f(j + 2, o2);
-                        f(j + 3, o3);
+                        fs(j, o0);
+                        fs(j + 1, o1);
+                        fs(j + 2, o2);
+                        fs(j + 3, o3);
                     }
-                    for (; j < b; ++j) f(j, stage[j + delta]);
+                    for (; j < b; ++j) fs(j, stage[j + delta]);
                 } else {
-                    scan_range4(glob, a, b, f);
+                    scan_range4(glob, a, b, fs);
                 }
             }
         }

@@ -13,6 +13,7 @@
 // per cell, a scan over the cells and one scattered placement per particle.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "sph_internal.h"
 
@@ -453,17 +454,25 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
-        for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh, [&](int j, const double4& o) {
-            const double dx = min_image(xi - o.x, L0, H0, per);
-            const double dy = min_image(yi - o.y, L1, H1, per);
-            const double dz = DIM == 3 ? min_image(zi - o.z, L2, H2, per) : 0.0;
-            const double r2 = r2_exact(dx, dy, dz);
-            const double rad = R * fmax(hi, o.w);
-            if (r2 < rad * rad) {
-                if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
-                ++cnt;
-            }
-        });
+        // one loop with the ranges' image shifts, one with min_image per
+        // candidate (small periodic grids); both give the same r^2 bits
+        auto scan = [&](auto shift_c) {
+            constexpr bool kShift = decltype(shift_c)::value;
+            for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh,
+                                    [&](int j, const double4& o, double sx, double sy, double sz) {
+                const double dx = kShift ? (xi - o.x) + sx : min_image(xi - o.x, L0, H0, per);
+                const double dy = kShift ? (yi - o.y) + sy : min_image(yi - o.y, L1, H1, per);
+                const double dz = DIM == 3 ? (kShift ? (zi - o.z) + sz : min_image(zi - o.z, L2, H2, per)) : 0.0;
+                const double r2 = r2_exact(dx, dy, dz);
+                const double rad = R * fmax(hi, o.w);
+                if (r2 < rad * rad) {
+                    if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
+                    ++cnt;
+                }
+            });
+        };
+        if (s.range_shift) scan(std::true_type{});
+        else scan(std::false_type{});
         nbcnt[t] = cnt;
     }
     // longest list and pair count: wave, then block, then one atomic each

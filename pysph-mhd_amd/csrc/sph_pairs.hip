@@ -13,6 +13,8 @@
 // a) is removed by packing the previous a into the force pack before the loop.
 // One thread per destination in cell order; FP64 throughout; no atomics on
 // particle data (gather form), so results are deterministic.
+#include <type_traits>
+
 #include "sph_internal.h"
 
 namespace sph {
@@ -64,21 +66,30 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
         double sw = 0.0, sg = 0.0;
-        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b) {
-            scan_range4(dpack, a, b, [&](int j, const double4& o) {
-                const double dx = min_image(me.x - o.x, L0, H0, per);
-                const double dy = min_image(me.y - o.y, L1, H1, per);
-                const double dz = DIM == 3 ? min_image(me.z - o.z, L2, H2, per) : 0.0;
-                const double r2 = r2_exact(dx, dy, dz);
-                if (r2 < Rh2) {
-                    const double q = sqrt(r2) * h1;
-                    double w, dw;
-                    shape<K, DIM>(q, w, dw);
-                    sw += o.w * w;
-                    sg += o.w * ((double)DIM * w + q * dw);
-                }
+        // one loop with the ranges' image shifts, one with min_image per
+        // candidate (small periodic grids); both give the same r^2 bits
+        auto scan = [&](auto shift_c) {
+            constexpr bool kShift = decltype(shift_c)::value;
+            for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double sx, double sy,
+                                                                          double sz) {
+                scan_range4(dpack, a, b, [&](int j, const double4& o) {
+                    const double dx = kShift ? (me.x - o.x) + sx : min_image(me.x - o.x, L0, H0, per);
+                    const double dy = kShift ? (me.y - o.y) + sy : min_image(me.y - o.y, L1, H1, per);
+                    const double dz =
+                        DIM == 3 ? (kShift ? (me.z - o.z) + sz : min_image(me.z - o.z, L2, H2, per)) : 0.0;
+                    const double r2 = r2_exact(dx, dy, dz);
+                    if (r2 < Rh2) {
+                        const double q = sqrt(r2) * h1;
+                        double w, dw;
+                        shape<K, DIM>(q, w, dw);
+                        sw += o.w * w;
+                        sg += o.w * ((double)DIM * w + q * dw);
+                    }
+                });
             });
-        });
+        };
+        if (s.range_shift) scan(std::true_type{});
+        else scan(std::false_type{});
         const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
         const double rho_sum = sw * fac;
         const double dwdh = -h1 * (fac * sg);

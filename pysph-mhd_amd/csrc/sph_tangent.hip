@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
         D3 sw(0.0), sg(0.0);
-        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b) {
+        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double, double, double) {
             for (int j = a; j < b; ++j) {
                 const double4 o = dpack[j];
                 const double ddx = me.x - o.x, ddy = me.y - o.y, ddz = me.z - o.z;
