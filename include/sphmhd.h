@@ -191,7 +191,13 @@ int sphmhd_advance_clock(sphmhd_ctx* ctx);
 /* nsteps full PEC steps with the clock advanced after each: the body of
  * MHDSolver.solve (solver.py:42-73) without dumps.  Kernels are fused
  * (stage2 of step n with initialize+stage1 of step n+1); the fields at
- * return are exactly those after the last stage2. */
+ * return are exactly those after the last stage2.  Steps after the first
+ * replay a captured hipGraph and are enqueued one ahead of the host's wait
+ * (a step whose evaluation needs more density passes or longer lists makes
+ * the step after it exit at entry, and both are finished before going on).
+ * Environment: SPHMHD_GRAPHS=0 launches every kernel plainly (no graphs, no
+ * enqueueing ahead); SPHMHD_SPEC=k launches k density passes ahead of the
+ * force pass instead of the adaptive count.  Neither changes the results. */
 int sphmhd_step(sphmhd_ctx* ctx, int64_t nsteps);
 
 /* MHDApplication._sync_initial_mhd_state (application.py:100-108):

@@ -74,9 +74,16 @@ struct sphmhd_ctx {
     int max_iter_all = 1;
     int min_iter_all = 1;
     int spec_passes = 2;  // density passes launched before the first wait
+    int spec_fixed = 0;   // SPHMHD_SPEC: a fixed count instead (results do not depend on it)
     int spec_margin = 0;       // one extra pass for a while after a redo
     int evals_since_redo = 0;
-    Ctl* hctl = nullptr;  // pinned readback
+    // Host mirror of the control block (pinned, mapped): slots 0 and 1 are
+    // written by the kernel that ends an evaluation (slot seq % 2), slot 2 by
+    // synchronous reads; hcur is the record the host logic reads.
+    Ctl* hctl = nullptr;
+    Ctl* hcur = nullptr;
+    int32_t seq_next = 0;  // seq the next enqueued evaluation end will carry
+    hipEvent_t sev = nullptr;  // end of the step being waited for (pipelined runs)
     sphmhd_stats stats{};
     // captured steps, one per speculative pass count; generation bumps on
     // bind, list growth and timing changes invalidate them
@@ -221,6 +228,7 @@ static void tune_end_step(sphmhd_ctx* c) {
 
 static int read_ctl(sphmhd_ctx* c) {
     launch_ctl_export(c->d);
+    c->hcur = &c->hctl[2];
     HIPCHK(hipStreamSynchronize(c->d.stream));
     tcollect(c);
     return 0;
@@ -259,8 +267,8 @@ static int build_lists(sphmhd_ctx* c) {
         launch_nb_build(c->d);
         int rc = read_ctl(c);
         if (rc) return rc;
-        if (c->hctl->max_nb <= c->d.nb_stride) return 0;
-        rc = grow_nb(c, c->hctl->max_nb);
+        if (c->hcur->max_nb <= c->d.nb_stride) return 0;
+        rc = grow_nb(c, c->hcur->max_nb);
         if (rc) return rc;
         launch_reset_nb_stats(c->d);
     }
@@ -289,6 +297,7 @@ static int force_pass(sphmhd_ctx* c, bool first) {
     tbegin(c, T_FORCE);
     launch_force(d);  // ends with the evaluation's end (warning bits, host mirror)
     tend(c, T_FORCE);
+    c->seq_next += 1;
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -302,7 +311,9 @@ static int force_pass(sphmhd_ctx* c, bool first) {
 // fit its slots: the remaining passes run (one wait each) or the lists grow,
 // and the force pass is redone from the kept previous accelerations.  After a
 // redo one extra speculative pass is launched for the next 16 evaluations.
-static int next_spec(const sphmhd_ctx* c) { return std::max(c->spec_passes + c->spec_margin, 1); }
+static int next_spec(const sphmhd_ctx* c) {
+    return c->spec_fixed > 0 ? c->spec_fixed : std::max(c->spec_passes + c->spec_margin, 1);
+}
 
 static const int kPhaseTimer[kSegs] = {T_STAGE, T_NNPS, T_DENSITY, T_LISTS, T_FORCE};
 
@@ -356,19 +367,33 @@ static int front_range(sphmhd_ctx* c, int from, bool first, int spec) {
 
 // a standalone evaluation: no fused stage kernel ran before it, so the
 // evaluation's start is launched (first = true)
-static int eval_front(sphmhd_ctx* c, int spec) { return front_range(c, P_NNPS, true, spec); }
+static int eval_front(sphmhd_ctx* c, int spec) {
+    const int rc = front_range(c, P_NNPS, true, spec);
+    c->seq_next += 1;  // the force pass ends the evaluation
+    return rc;
+}
 
-static int eval_finish(sphmhd_ctx* c, int spec) {
+// the record of the evaluation that ended with sequence number sq
+static int pick_record(sphmhd_ctx* c, int32_t sq) {
+    c->hcur = &c->hctl[sq & 1];
+    if (c->hcur->seq != sq) return fail(SPHMHD_ESTATE, "control record out of sequence");
+    return 0;
+}
+
+// After the evaluation's record is in c->hcur: the rest of the density
+// passes and the force pass again when the iteration needed more passes or a
+// list did not fit (the record's abort flag, which also made every kernel
+// enqueued after it exit at entry), then the statistics and warnings.
+static int eval_process(sphmhd_ctx* c, int spec) {
     Dev& d = c->d;
-    HIPCHK(hipStreamSynchronize(d.stream));
-    tcollect(c);
     int rc = 0;
     int passes = spec;
     bool redo = false;
-    if (!c->hctl->all_done) {
+    if (c->hcur->abort) launch_clear_abort(d);
+    if (!c->hcur->all_done) {
         // the prep overwrote the {x,y,z,m} density pack with {x,y,z,h}
         launch_pack_density(d);
-        while (!c->hctl->all_done) {
+        while (!c->hcur->all_done) {
             if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
             tbegin(c, T_DENSITY);
             d.pass = passes++;
@@ -383,16 +408,17 @@ static int eval_finish(sphmhd_ctx* c, int spec) {
     } else if (c->spec_margin && ++c->evals_since_redo >= 16) {
         c->spec_margin = 0;
     }
-    c->spec_passes = std::max(c->hctl->iters_max, c->min_iter_all);
-    c->stats.density_iterations += c->hctl->iters_max;
-    c->stats.last_iterations_max = c->hctl->iters_max;
+    c->spec_passes = std::max(c->hcur->iters_max, c->min_iter_all);
+    c->stats.density_iterations += c->hcur->iters_max;
+    c->stats.last_iterations_max = c->hcur->iters_max;
     for (int grows = 0;;) {
-        if (c->hctl->overflow) {
+        if (c->hcur->overflow) {
             if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
-            if ((rc = grow_nb(c, c->hctl->max_nb))) return rc;
+            if ((rc = grow_nb(c, c->hcur->max_nb))) return rc;
             redo = true;
         }
         if (!redo) break;
+        launch_clear_abort(d);
         launch_reset_nb_stats(d);
         launch_reset_force_outputs(d);
         if ((rc = force_pass(c, false))) return rc;
@@ -400,14 +426,23 @@ static int eval_finish(sphmhd_ctx* c, int spec) {
         redo = false;
         c->stats.force_redos += 1;
     }
-    c->stats.max_neighbours = c->hctl->max_nb;
-    c->stats.pairs_force = c->hctl->pairs;
-    c->stats.cells = c->hctl->total_cells;
+    c->stats.max_neighbours = c->hcur->max_nb;
+    c->stats.pairs_force = c->hcur->pairs;
+    c->stats.cells = c->hcur->total_cells;
     c->stats.evals += 1;
-    int st = c->hctl->status;
+    int st = c->hcur->status;
     if (st & 2) return SPHMHD_WNAN;
     if (st & 1) return SPHMHD_WNOCONV;
     return 0;
+}
+
+// the one host wait of an evaluation enqueued last
+static int eval_finish(sphmhd_ctx* c, int spec) {
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    tcollect(c);
+    const int rc = pick_record(c, c->seq_next - 1);
+    if (rc) return rc;
+    return eval_process(c, spec);
 }
 
 static int do_eval(sphmhd_ctx* c) {
@@ -420,7 +455,11 @@ static int do_eval(sphmhd_ctx* c) {
 // enqueue-only part of one step: the stages (the first step of a run starts
 // with initialize + stage1, later ones with the fused stage2 + initialize +
 // stage1 and the clock) and the evaluation front
-static int step_front(sphmhd_ctx* c, bool first, int spec) { return front_range(c, P_STAGE, first, spec); }
+static int step_front(sphmhd_ctx* c, bool first, int spec) {
+    const int rc = front_range(c, P_STAGE, first, spec);
+    c->seq_next += 1;  // the force pass ends the evaluation
+    return rc;
+}
 
 static void release(sphmhd_ctx::StepGraph& g) {
     for (hipGraphExec_t& e : g.seg)
@@ -498,6 +537,7 @@ static int step_front_graph(sphmhd_ctx* c, int spec) {
         if (k >= 0) tend(c, k);
     }
     c->stats.graph_replays += 1;
+    c->seq_next += 1;  // the captured force pass ends the evaluation
     return 0;
 }
 
@@ -511,7 +551,7 @@ static int count_unconverged(sphmhd_ctx* c, int64_t* out) {
     launch_count_unconverged(c->d);
     int rc = read_ctl(c);
     if (rc) return rc;
-    *out = c->hctl->count;
+    *out = c->hcur->count;
     return 0;
 }
 
@@ -555,6 +595,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         d.stream = (hipStream_t)hip_stream;
         if (const char* g = std::getenv("SPHMHD_GRAPHS")) c->use_graphs = std::strcmp(g, "0") != 0;
         for (int q = 0; q < kLaunchClasses; ++q) d.blk[q] = kBlock;
+        if (const char* g = std::getenv("SPHMHD_SPEC")) c->spec_fixed = std::atoi(g);
         if (const char* g = std::getenv("SPHMHD_TUNE"))
             if (std::strcmp(g, "0") == 0) c->tune.state = 3;
         int64_t cap = 0;
@@ -639,10 +680,12 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
         // the control block's host mirror: pinned, mapped and coherent, so the
         // kernel that ends an evaluation writes it directly (no copy)
-        hipError_t e = hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocMapped | hipHostMallocCoherent);
+        hipError_t e = hipHostMalloc((void**)&c->hctl, 3 * sizeof(Ctl), hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostMalloc failed"));
         e = hipHostGetDevicePointer((void**)&d.hctl_dev, c->hctl, 0);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostGetDevicePointer failed"));
+        std::memset(c->hctl, 0, 3 * sizeof(Ctl));
+        c->hcur = &c->hctl[2];
         e = hipMemcpy(d.inst, c->hinst.data(), sizeof(Inst) * n_instances, hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
         e = hipMemcpy(d.inst_of, inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
@@ -670,6 +713,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
+    if (c->sev) (void)hipEventDestroy(c->sev);
     void* ptrs[] = {d.inst, d.ctl, d.arrive, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
@@ -770,15 +814,46 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
             c->d.stream = c->gstream;
             swap.on = true;
         }
+        // With graphs (and no timing or tuning) step s + 1 is enqueued before
+        // the host waits for step s, so the GPU does not idle through the
+        // host's turn.  If step s needs more density passes or longer lists,
+        // its end sets the abort flag and every kernel of step s + 1 exits at
+        // entry: the host then finishes step s and enqueues step s + 1 again.
+        bool ahead = false;  // step s was enqueued during step s - 1
+        int ahead_spec = 0;
         for (int64_t s = 0; s < nsteps; ++s) {
             // tuning steps are launched one by one with events around the phases
             const bool tuning = c->tune.state <= 1;
             if (tuning && (rc = tune_begin_step(c))) return rc;
-            const int spec = next_spec(c);
-            rc = (!tuning && s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec)
-                                                                 : step_front(c, s == 0, spec);
-            if (rc < 0) return rc;
-            rc = eval_finish(c, spec);
+            int spec = ahead_spec;
+            if (!ahead) {
+                spec = next_spec(c);
+                rc = (!tuning && s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec)
+                                                                     : step_front(c, s == 0, spec);
+                if (rc < 0) return rc;
+            }
+            ahead = false;
+            if (!tuning && swap.on && c->use_graphs && !c->timing && s + 1 < nsteps) {
+                if (!c->sev) HIPCHK(hipEventCreateWithFlags(&c->sev, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(c->sev, c->d.stream));  // end of step s
+                ahead_spec = next_spec(c);
+                const int64_t replays = c->stats.graph_replays;
+                rc = step_front_graph(c, ahead_spec);  // replayed, or launched plainly if a capture failed
+                if (rc < 0) return rc;
+                ahead = true;
+                HIPCHK(hipEventSynchronize(c->sev));
+                if ((rc = pick_record(c, c->seq_next - 2))) return rc;
+                if (c->hcur->abort) {
+                    // step s + 1 exited at entry: let it drain, take back its count
+                    HIPCHK(hipStreamSynchronize(c->d.stream));
+                    c->seq_next -= 1;
+                    c->stats.graph_replays = replays;
+                    ahead = false;
+                }
+                rc = eval_process(c, spec);
+            } else {
+                rc = eval_finish(c, spec);
+            }
             if (rc < 0) return rc;
             if (rc > worst) worst = rc;
             if (tuning) tune_end_step(c);
@@ -831,7 +906,7 @@ int sphmhd_warm_start(sphmhd_ctx* c) {
     launch_warm_dt(d);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(c))) return rc;
-    if (c->hctl->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
+    if (c->hcur->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
     return 0;
 }
 
@@ -990,30 +1065,31 @@ static int t_eval(sphmhd_ctx* c) {
         launch_g0_control(d);
         HIPCHK(hipGetLastError());
         if ((rc = read_ctl(c))) return rc;
-        if (c->hctl->all_done) break;
+        if (c->hcur->all_done) break;
         if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
     }
-    c->stats.density_iterations += c->hctl->iters_max;
-    c->stats.last_iterations_max = c->hctl->iters_max;
+    c->stats.density_iterations += c->hcur->iters_max;
+    c->stats.last_iterations_max = c->hcur->iters_max;
     launch_t_prep(d, g);
     for (int grows = 0;;) {
         launch_reset_nb_stats(d);
         launch_nb_build(d);
         HIPCHK(hipGetLastError());
         if ((rc = read_ctl(c))) return rc;
-        if (!c->hctl->overflow) break;
+        if (!c->hcur->overflow) break;
         if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
-        if ((rc = grow_nb(c, c->hctl->max_nb))) return rc;
+        if ((rc = grow_nb(c, c->hcur->max_nb))) return rc;
     }
     launch_t_force(d, g);
     launch_eval_end(d);
+    c->seq_next += 1;
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(c))) return rc;
-    c->stats.max_neighbours = c->hctl->max_nb;
-    c->stats.pairs_force = c->hctl->pairs;
-    c->stats.cells = c->hctl->total_cells;
+    c->stats.max_neighbours = c->hcur->max_nb;
+    c->stats.pairs_force = c->hcur->pairs;
+    c->stats.cells = c->hcur->total_cells;
     c->stats.evals += 1;
-    const int st = c->hctl->status;
+    const int st = c->hcur->status;
     if (st & 2) return SPHMHD_WNAN;
     if (st & 1) return SPHMHD_WNOCONV;
     return 0;
@@ -1087,7 +1163,7 @@ int sphmhd_tangent_warm_start(sphmhd_ctx* c) {
     launch_t_warm_dt(d, g);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(c))) return rc;
-    if (c->hctl->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
+    if (c->hcur->status & 4) return fail(SPHMHD_EINVAL, "warm start: dt_cfl <= 0 (application.py:139)");
     return 0;
 }
 

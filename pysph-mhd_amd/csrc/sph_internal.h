@@ -91,6 +91,12 @@ struct Ctl {
     int32_t overflow;    // a neighbour list did not fit its slots
     int32_t nbig;        // cells with more than kSmallBucket particles in this sort
     int32_t pad;
+    // Steps are enqueued one ahead of the host's wait.  An evaluation that
+    // needs more density passes or longer lists sets abort when it ends, and
+    // every kernel of the step enqueued after it then exits at entry, leaving
+    // the state as that evaluation left it for the host to finish.
+    int32_t abort;
+    int32_t seq;  // evaluations ended so far (the host mirror's slot is seq % 2)
 };
 
 struct Fields {
@@ -606,7 +612,7 @@ struct Dev {
     // device arrays
     Inst* inst;
     Ctl* ctl;
-    Ctl* hctl_dev;          // device view of the pinned host mirror of ctl
+    Ctl* hctl_dev;          // device view of the pinned host mirror of ctl: 3 slots
     int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
     int32_t blk[3];         // workgroup size of the density, list (prep + build) and force launches
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
@@ -683,6 +689,7 @@ void launch_nb_build(Dev& d);
 void launch_force(Dev& d);
 void launch_eval_end(Dev& d);
 void launch_ctl_export(Dev& d);
+void launch_clear_abort(Dev& d);
 
 void launch_initialize(Dev& d);
 void launch_stage1(Dev& d);

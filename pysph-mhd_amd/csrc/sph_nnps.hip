@@ -23,7 +23,10 @@ namespace sph {
 // per-instance bookkeeping
 // ---------------------------------------------------------------------------
 
+// a standalone evaluation's start (not inside a pipelined run of steps): it
+// also clears the abort flag
 __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
+    if (threadIdx.x == 0) ctl->abort = 0;
     eval_begin_body(inst, ninst, ctl, reset_dt);
 }
 
@@ -42,6 +45,7 @@ __device__ void grid_setup(Inst* inst, int ninst, Ctl* ctl);
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
                        int64_t n, int next, int64_t chunk, int ninst, Ctl* ctl, int with_grid,
                        int32_t* arrive) {
+    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     __shared__ unsigned long long wmax[kBlock / kWave];
     __shared__ int wk[kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -176,7 +180,8 @@ __device__ __forceinline__ int run_end(unsigned long long heads, unsigned long l
 
 __global__ void k_cell_count(Fields F, const int32_t* __restrict__ inst_of, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ prev_perm, uint32_t* __restrict__ keys,
-                             int32_t* __restrict__ ccount, int64_t n) {
+                             int32_t* __restrict__ ccount, int64_t n, const Ctl* ctl) {
+    if (ctl->abort) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < n;
     uint32_t key = 0xFFFFFFFFu;
@@ -238,6 +243,7 @@ __device__ void cell_scan_top(int32_t* __restrict__ cblock, const Ctl* ctl) {
 __global__ void __launch_bounds__(kBlock) k_cell_scan(const int32_t* __restrict__ ccount,
                                                       int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
                                                       int32_t* __restrict__ cblock, Ctl* ctl, int32_t* arrive) {
+    if (ctl->abort) return;
     __shared__ int wsum[kBlock / kWave];
     const int64_t total = ctl->total_cells;
     const int64_t c0 = (int64_t)blockIdx.x * kScanCells;
@@ -269,7 +275,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_scan(const int32_t* __restrict_
 // the counts return to zero); the order inside a bucket is arbitrary here
 __global__ void k_cell_place(const int32_t* __restrict__ prev_perm, const uint32_t* __restrict__ keys,
                              int32_t* __restrict__ ccount, const int32_t* __restrict__ cstart,
-                             const int32_t* __restrict__ cblock, int32_t* __restrict__ bucket, int64_t n) {
+                             const int32_t* __restrict__ cblock, int32_t* __restrict__ bucket, int64_t n,
+                             const Ctl* ctl) {
+    if (ctl->abort) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < n;
     const int32_t i = valid ? prev_perm[t] : 0;
@@ -292,6 +300,7 @@ __global__ void k_cell_place(const int32_t* __restrict__ prev_perm, const uint32
 // final cell table; cells above kSmallBucket particles are listed for k_cell_big
 __global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
                               const int32_t* __restrict__ cblock, int32_t* __restrict__ bigc, Ctl* ctl, int64_t cap) {
+    if (ctl->abort) return;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cap || c >= ctl->total_cells) return;
     const int32_t off = cblock[c / kScanCells];
@@ -304,7 +313,8 @@ __global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict_
 // small cells: a particle's place is its rank by storage index in its bucket
 __global__ void k_cell_rank(const int32_t* __restrict__ bucket, const uint32_t* __restrict__ keys,
                             const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n) {
+                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n, const Ctl* ctl) {
+    if (ctl->abort) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int32_t v = bucket[q];
@@ -344,6 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_big(const int32_t* __restrict__
                                                      const int32_t* __restrict__ bucket, int32_t* perm,
                                                      uint32_t* __restrict__ skey, const int32_t* __restrict__ bigc,
                                                      const Ctl* ctl) {
+    if (ctl->abort) return;
     __shared__ int32_t lds[kBigLds];
     const int nbig = ctl->nbig;
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
@@ -372,7 +383,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_big(const int32_t* __restrict__
 // ---------------------------------------------------------------------------
 
 __global__ void k_pack_density(Fields F, const int32_t* __restrict__ perm, double4* __restrict__ dpack, int64_t n,
-                               int dim) {
+                               int dim, const Ctl* ctl) {
+    if (ctl->abort) return;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int32_t i = perm[t];
@@ -426,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl, int cap) {
+    if (ctl->abort) return;
     // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
     // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
     // rarely fit and the LDS would cost occupancy (cap 0)
@@ -546,16 +559,16 @@ void launch_keys_sort_ranges(Dev& d) {
     // keys_alt: keys by storage index; perm_alt: the buckets; perm: the
     // previous order on entry, the new one on exit
     hipLaunchKernelGGL(k_cell_count, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.perm,
-                       d.keys_alt, d.ccount, d.n);
+                       d.keys_alt, d.ccount, d.n, d.ctl);
     const unsigned sblocks = (unsigned)((d.cell_cap + kScanCells - 1) / kScanCells);
     hipLaunchKernelGGL(k_cell_scan, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cstart, d.cend, d.cblock,
                        d.ctl, d.arrive);
     hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
-                       d.cstart, d.cblock, d.perm_alt, d.n);
+                       d.cstart, d.cblock, d.perm_alt, d.n, d.ctl);
     hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,
                        d.bigc, d.ctl, d.cell_cap);
     hipLaunchKernelGGL(k_cell_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm_alt, d.keys_alt, d.cstart,
-                       d.cend, d.perm, d.keys, d.n);
+                       d.cend, d.perm, d.keys, d.n, d.ctl);
     hipLaunchKernelGGL(k_cell_big, dim3(kBigBlocks), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.perm_alt, d.perm,
                        d.keys, d.bigc, d.ctl);
 }
@@ -563,7 +576,7 @@ void launch_keys_sort_ranges(Dev& d) {
 void launch_pack_density(Dev& d) {
     if (d.n == 0) return;
     hipLaunchKernelGGL(k_pack_density, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.perm,
-                       (double4*)d.dpack, d.n, d.dim);
+                       (double4*)d.dpack, d.n, d.dim, d.ctl);
 }
 
 void launch_nb_build(Dev& d) {

@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
                                                     const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                     int32_t* __restrict__ act_out, int32_t* cnt_out, Ctl* ctl,
                                                     int ninst, int next_list, int32_t* arrive) {
+    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out, cnt_out);
     if (last_to_arrive(arrive)) g0_control_body(inst, ninst, ctl, next_list);
 }
@@ -283,7 +284,9 @@ __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, i
 
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
-                             double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev) {
+                             double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev,
+                             const Ctl* ctl) {
+    if (ctl->abort) return;
     int64_t t = xcd_tile((n + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
@@ -730,7 +733,11 @@ __device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) 
         Ctl v = *ctl;
         v.status |= st;
         ctl->status = v.status;
-        *hout = v;
+        // more passes or longer lists needed: the step enqueued next exits
+        v.abort = (!v.all_done || v.overflow) ? 1 : 0;
+        ctl->abort = v.abort;
+        ctl->seq = v.seq + 1;
+        hout[v.seq & 1] = v;
     }
 }
 
@@ -739,7 +746,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_end(const Inst* __restrict__ in
     eval_end_body(inst, ninst, ctl, hout);
 }
 
-__global__ void k_ctl_export(const Ctl* ctl, Ctl* hout) { *hout = *ctl; }
+// the control block into the host mirror's third slot (synchronous reads)
+__global__ void k_ctl_export(const Ctl* ctl, Ctl* hout) { hout[2] = *ctl; }
+
+__global__ void k_clear_abort(Ctl* ctl) { ctl->abort = 0; }
 
 // the fused force pass; its last workgroup ends the evaluation
 template <int K, int DIM>
@@ -748,6 +758,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride,
                                                   int ninst, Ctl* ctl, Ctl* hout, int32_t* arrive) {
+    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride);
     if (last_to_arrive(arrive)) eval_end_body(inst, ninst, ctl, hout);
 }
@@ -797,7 +808,8 @@ void launch_prep_force(Dev& d, bool keep_prev) {
     if (d.n == 0) return;
     const int B = d.blk[LC_LISTS];
     hipLaunchKernelGGL(k_prep_force, dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields, d.perm, d.inst,
-                       d.inst_of, d.fpack, (double4*)d.dpack, d.n, d.aprev, keep_prev ? 1 : 0);
+                       d.inst_of, d.fpack, (double4*)d.dpack, d.n, d.aprev, keep_prev ? 1 : 0,
+                       d.ctl);
 }
 
 void launch_eval_end(Dev& d) {
@@ -807,6 +819,8 @@ void launch_eval_end(Dev& d) {
 void launch_ctl_export(Dev& d) {
     hipLaunchKernelGGL(k_ctl_export, dim3(1), dim3(1), 0, d.stream, d.ctl, d.hctl_dev);
 }
+
+void launch_clear_abort(Dev& d) { hipLaunchKernelGGL(k_clear_abort, dim3(1), dim3(1), 0, d.stream, d.ctl); }
 
 void launch_force(Dev& d) {
     if (d.n == 0) return;

@@ -251,6 +251,7 @@ __device__ __forceinline__ void advance_clock(Inst& s) {
 // two kernels of one thread per instance
 __global__ void k_stage2_init_stage1(Fields F, Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                      int ninst, Ctl* ctl, int32_t* arrive) {
+    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) stage2_init_stage1_one(F, inst, inst_of, i);
     if (last_to_arrive(arrive)) {

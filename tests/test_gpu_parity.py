@@ -180,6 +180,42 @@ def test_graph_replays_equal_plain_launches(gpu, monkeypatch):
     assert tm["force_launches"] >= 10 and tm["force_ms"] > 0.0
 
 
+@pytest.mark.parametrize("spec", ["1", "2"])
+def test_steps_enqueued_ahead_survive_redos(gpu, monkeypatch, spec):
+    """Graph steps are enqueued one ahead of the host's wait.  With one
+    density pass launched ahead of the force pass (below the two passes every
+    evaluation takes) every step ends needing a redo: its end raises the abort
+    flag, the step enqueued after it exits at entry, and the host finishes the
+    step and enqueues the next again.  The result equals plain launches bit
+    for bit, and each step is replayed from the graph exactly once."""
+    from sphmhd.device import DeviceSystem
+
+    s = _ot(48)
+    monkeypatch.setenv("SPHMHD_GRAPHS", "0")
+    monkeypatch.delenv("SPHMHD_SPEC", raising=False)
+    plain = DeviceSystem(s.params, s.offsets)
+    monkeypatch.setenv("SPHMHD_GRAPHS", "1")
+    monkeypatch.setenv("SPHMHD_SPEC", spec)
+    ahead = DeviceSystem(s.params, s.offsets)
+    for d in (plain, ahead):
+        d.set_launch((256, 256, 256))
+        d.set_fields(init_fields(s))
+        d.warm_start()
+    redos0 = ahead.stats()["force_redos"]
+    for rnd in range(2):
+        plain.step(6)
+        ahead.step(6)
+        for nm in STATE + RATES:
+            assert np.array_equal(plain.get(nm), ahead.get(nm)), (spec, rnd, nm)
+        assert np.array_equal(plain.clock()[0], ahead.clock()[0])
+        assert np.array_equal(plain.clock()[1], ahead.clock()[1])
+    st = ahead.stats()
+    assert st["graph_replays"] == 10
+    if spec == "1":
+        assert st["force_redos"] - redos0 == 12  # every step was finished after its abort
+    assert st["evals"] == plain.stats()["evals"]
+
+
 @pytest.mark.parametrize("kernel", ["cubic", "wendland_c2", "wendland_c4", "wendland_c6", "gaussian",
                                     "super_gaussian"])
 def test_kernels_steps(gpu, kernel):
