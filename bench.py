@@ -205,7 +205,7 @@ def run_tangent(steps: int, warmup: int = 2):
             d.set_tangents(0, _ic_tangent(lambda **kw: alfven_2d(97, 168, **kw), theta, "amplitude"))
             d.seed_params({(1, "alpha_max"): 1.0, (2, "cleaning_damping_factor"): 1.0})
         d.warm_start()
-        d.step(warmup + (TUNE_STEPS if cls is DeviceSystem else 0))
+        d.step(warmup + TUNE_STEPS)
         d.set_timing(False)
         d.synchronize()
         torch.cuda.synchronize()

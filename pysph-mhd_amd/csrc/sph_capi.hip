@@ -1051,6 +1051,29 @@ static int t_check(sphmhd_ctx* c) {
 // one evaluation over values and tangents: the primal neighbour search and
 // list build on the values, dual density passes (one wait each), dual prep,
 // the list (grown if needed), the dual force pass and the dt tangent
+// the tangent prep, lists and force pass (with the dt tangent and the
+// evaluation's end); keep_prev: a redo, which reads the first prep's copy of
+// the previous accelerations
+static int t_force_pass(sphmhd_ctx* c, bool keep_prev) {
+    Dev& d = c->d;
+    TanDev& g = c->tg;
+    tbegin(c, T_LISTS);
+    launch_t_prep(d, g, keep_prev);
+    launch_reset_nb_stats(d);
+    launch_nb_build(d);
+    tend(c, T_LISTS);
+    tbegin(c, T_FORCE);
+    launch_t_force(d, g);
+    tend(c, T_FORCE);
+    launch_eval_end(d);
+    c->seq_next += 1;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One wait per evaluation, as the plain path: speculative density passes
+// and a speculative force pass, then the remaining passes and a redone force
+// pass when the iteration needed more passes or a list had to grow.
 static int t_eval(sphmhd_ctx* c) {
     Dev& d = c->d;
     TanDev& g = c->tg;
@@ -1058,33 +1081,53 @@ static int t_eval(sphmhd_ctx* c) {
     if (rc) return rc;
     launch_t_eval_begin(d, g);
     launch_pack_density(d);
+    const int spec = next_spec(c);
     int passes = 0;
-    for (;;) {
-        d.pass = passes++;
+    tbegin(c, T_DENSITY);
+    for (; passes < spec; ++passes) {
+        d.pass = passes;
         launch_t_density(d, g);
         launch_g0_control(d);
-        HIPCHK(hipGetLastError());
-        if ((rc = read_ctl(c))) return rc;
-        if (c->hcur->all_done) break;
-        if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
     }
+    tend(c, T_DENSITY);
+    if ((rc = t_force_pass(c, false))) return rc;
+    if ((rc = read_ctl(c))) return rc;
+    bool redo = false;
+    if (c->hcur->abort) launch_clear_abort(d);
+    if (!c->hcur->all_done) {
+        launch_pack_density(d);  // the prep overwrote the density pack with {x,y,z,h}
+        while (!c->hcur->all_done) {
+            if (passes > c->max_iter_all + 1) return fail(SPHMHD_ESTATE, "density iteration control did not stop");
+            d.pass = passes++;
+            launch_t_density(d, g);
+            launch_g0_control(d);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_ctl(c))) return rc;
+        }
+        redo = true;
+        c->spec_margin = 1;
+        c->evals_since_redo = 0;
+    } else if (c->spec_margin && ++c->evals_since_redo >= 16) {
+        c->spec_margin = 0;
+    }
+    c->spec_passes = std::max(c->hcur->iters_max, c->min_iter_all);
     c->stats.density_iterations += c->hcur->iters_max;
     c->stats.last_iterations_max = c->hcur->iters_max;
-    launch_t_prep(d, g);
     for (int grows = 0;;) {
-        launch_reset_nb_stats(d);
-        launch_nb_build(d);
-        HIPCHK(hipGetLastError());
+        if (c->hcur->overflow) {
+            if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
+            if ((rc = grow_nb(c, c->hcur->max_nb))) return rc;
+            redo = true;
+        }
+        if (!redo) break;
+        launch_clear_abort(d);
+        launch_reset_force_outputs(d);
+        launch_t_eval_begin(d, g);
+        if ((rc = t_force_pass(c, true))) return rc;
         if ((rc = read_ctl(c))) return rc;
-        if (!c->hcur->overflow) break;
-        if (++grows > 2) return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
-        if ((rc = grow_nb(c, c->hcur->max_nb))) return rc;
+        redo = false;
+        c->stats.force_redos += 1;
     }
-    launch_t_force(d, g);
-    launch_eval_end(d);
-    c->seq_next += 1;
-    HIPCHK(hipGetLastError());
-    if ((rc = read_ctl(c))) return rc;
     c->stats.max_neighbours = c->hcur->max_nb;
     c->stats.pairs_force = c->hcur->pairs;
     c->stats.cells = c->hcur->total_cells;
@@ -1173,11 +1216,15 @@ int sphmhd_tangent_step(sphmhd_ctx* c, int64_t nsteps) {
     if (nsteps < 0) return fail(SPHMHD_EINVAL, "nsteps < 0");
     int worst = 0;
     for (int64_t s = 0; s < nsteps; ++s) {
+        // a context that only takes tangent steps tunes its launch shapes here
+        const bool tuning = c->tune.state <= 1;
+        if (tuning && (rc = tune_begin_step(c))) return rc;
         launch_t_initialize(c->d, c->tg);
         launch_t_stage1(c->d, c->tg);
         rc = t_eval(c);
         if (rc < 0) return rc;
         if (rc > worst) worst = rc;
+        if (tuning) tune_end_step(c);
         launch_t_stage2(c->d, c->tg);
         launch_t_clock(c->d, c->tg);
         HIPCHK(hipGetLastError());

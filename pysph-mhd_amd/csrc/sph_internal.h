@@ -660,7 +660,7 @@ struct TanDev {
 
 void launch_t_eval_begin(Dev& d, TanDev& g);
 void launch_t_density(Dev& d, TanDev& g);
-void launch_t_prep(Dev& d, TanDev& g);
+void launch_t_prep(Dev& d, TanDev& g, bool keep_prev);
 void launch_t_force(Dev& d, TanDev& g);
 void launch_t_initialize(Dev& d, TanDev& g);
 void launch_t_stage1(Dev& d, TanDev& g);

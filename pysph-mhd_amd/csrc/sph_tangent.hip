@@ -14,6 +14,7 @@
 // tangents are rows of the caller's tangent storage (TFields).  Per-instance
 // tangents (dt, pending dt, parameters) live in InstTan.
 #include <climits>
+#include <type_traits>
 
 #include "sph_dual.h"
 #include "sph_internal.h"
@@ -42,6 +43,31 @@ __device__ __forceinline__ D3 parT(double v, const InstTan& it, int p) {
     return r;
 }
 
+// loads and stores of ND directions starting at direction d0
+template <int ND>
+__device__ __forceinline__ Dual<ND> ldN(const Fields& F, const TFields& G, int k, int64_t i, int d0) {
+    Dual<ND> r(F.f[k][i]);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) r.d[q] = G.f[d0 + q][k][i];
+    return r;
+}
+
+template <int ND>
+__device__ __forceinline__ void stN(const Fields& F, const TFields& G, int k, int64_t i, const Dual<ND>& x, int d0,
+                                    bool value) {
+    if (value) F.f[k][i] = x.v;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) G.f[d0 + q][k][i] = x.d[q];
+}
+
+template <int ND>
+__device__ __forceinline__ Dual<ND> parN(double v, const InstTan& it, int p, int d0) {
+    Dual<ND> r(v);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) r.d[q] = it.par[p][d0 + q];
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ T ipow_dim_t(const T& x, int dim) { return dim == 2 ? x * x : x * x * x; }
 template <class T>
@@ -62,6 +88,13 @@ __global__ void k_t_eval_begin(InstTan* it, int ninst) {
 // G0: density sum + Newton step (k_density over duals)
 // ---------------------------------------------------------------------------
 
+// Three lanes per particle, one tangent direction each (21 particles per
+// wave, the last lane idle): the lanes of a particle scan the same candidates
+// in lockstep (the candidate loads broadcast), every lane reads the old h,
+// omega and converged before any lane writes, and only the direction-0 lane
+// stores the values and joins the primal's list and reductions.
+constexpr int kTPW = kWave / kTanDirs;  // particles per wave
+
 template <int K, int DIM>
 __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const double4* __restrict__ dpack,
                                                       const int32_t* __restrict__ perm,
@@ -71,68 +104,78 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
                                                       const int32_t* __restrict__ inst_of, int64_t n,
                                                       const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                       int32_t* __restrict__ act_out, int32_t* cnt_out) {
-    const int64_t idx0 = (int64_t)blockIdx.x * kBlock;
+    using D1 = Dual<1>;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const int slot = lane / kTanDirs, dir = lane - kTanDirs * slot;
+    const int64_t idx0 = (int64_t)blockIdx.x * nw * kTPW;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
-    if (idx0 >= items) return;
-    const int64_t idx = idx0 + threadIdx.x;
-    const bool valid = idx < items;
+    if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
+    const int64_t idx = idx0 + (int64_t)w * kTPW + slot;
+    const bool valid = slot < kTPW && idx < items;
+    const bool lead = dir == 0;
     const int64_t t = valid ? (act_in ? (int64_t)act_in[idx] : idx) : 0;
     const int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
     const bool live = valid && !s.done;
     const int32_t i = valid ? perm[t] : 0;
-    D3 hi = live ? ldT(F, G, TFLD(H), i) : D3(0.0);
+    D1 hi = live ? ldN<1>(F, G, TFLD(H), i, dir) : D1(0.0);
     const bool active = live && F.f[SPHMHD_F_CONVERGED][i] != 1.0;
-    D3 hout = hi;
+    D1 hout = hi;
     bool unconv = false;
 
     if (active) {
         const double R = kernel_radius(K);
         const double4 me = dpack[t];
-        const D3 xi = ldT(F, G, TFLD(X), i), yi = ldT(F, G, TFLD(Y), i);
-        const D3 zi = DIM == 3 ? ldT(F, G, TFLD(Z), i) : D3(0.0);
-        const D3 h1 = 1.0 / hi;
+        const D1 xi = ldN<1>(F, G, TFLD(X), i, dir), yi = ldN<1>(F, G, TFLD(Y), i, dir);
+        const D1 zi = DIM == 3 ? ldN<1>(F, G, TFLD(Z), i, dir) : D1(0.0);
+        const D1 h1 = 1.0 / hi;
         const double Rh = R * hi.v;
         const double Rh2 = Rh * Rh;
         const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
-        D3 sw(0.0), sg(0.0);
-        for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double, double, double) {
-            for (int j = a; j < b; ++j) {
-                const double4 o = dpack[j];
-                const double ddx = me.x - o.x, ddy = me.y - o.y, ddz = me.z - o.z;
-                const double dxv = min_image(ddx, L0, H0, per);
-                const double dyv = min_image(ddy, L1, H1, per);
-                const double dzv = DIM == 3 ? min_image(ddz, L2, H2, per) : 0.0;
-                if (r2_exact(dxv, dyv, dzv) < Rh2) {
-                    const int32_t jj = perm[j];
-                    // the minimum image shifts by a constant: the tangent is x_i' - x_j'
-                    const D3 dx = (xi - ldT(F, G, TFLD(X), jj)) + (dxv - ddx);
-                    const D3 dy = (yi - ldT(F, G, TFLD(Y), jj)) + (dyv - ddy);
-                    const D3 dz = DIM == 3 ? (zi - ldT(F, G, TFLD(Z), jj)) + (dzv - ddz) : D3(0.0);
-                    const D3 q = sqrt(dx * dx + dy * dy + dz * dz) * h1;
-                    D3 w, dw;
-                    shape<K, DIM>(q, w, dw);
-                    sw += o.w * w;
-                    sg += o.w * ((double)DIM * w + q * dw);
+        D1 sw(0.0), sg(0.0);
+        // the value test as the primal pass (range shifts where exact)
+        auto scan = [&](auto shift_c) {
+            constexpr bool kShift = decltype(shift_c)::value;
+            for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double sx, double sy,
+                                                                          double sz) {
+                for (int j = a; j < b; ++j) {
+                    const double4 o = dpack[j];
+                    const double ddx = me.x - o.x, ddy = me.y - o.y, ddz = me.z - o.z;
+                    const double dxv = kShift ? ddx + sx : min_image(ddx, L0, H0, per);
+                    const double dyv = kShift ? ddy + sy : min_image(ddy, L1, H1, per);
+                    const double dzv = DIM == 3 ? (kShift ? ddz + sz : min_image(ddz, L2, H2, per)) : 0.0;
+                    if (r2_exact(dxv, dyv, dzv) < Rh2) {
+                        const int32_t jj = perm[j];
+                        // the minimum image shifts by a constant: the tangent is x_i' - x_j'
+                        const D1 dx = (xi - ldN<1>(F, G, TFLD(X), jj, dir)) + (dxv - ddx);
+                        const D1 dy = (yi - ldN<1>(F, G, TFLD(Y), jj, dir)) + (dyv - ddy);
+                        const D1 dz = DIM == 3 ? (zi - ldN<1>(F, G, TFLD(Z), jj, dir)) + (dzv - ddz) : D1(0.0);
+                        const D1 q = sqrt(dx * dx + dy * dy + dz * dz) * h1;
+                        D1 w1, dw;
+                        shape<K, DIM>(q, w1, dw);
+                        sw += o.w * w1;
+                        sg += o.w * ((double)DIM * w1 + q * dw);
+                    }
                 }
-            }
-        });
-        const D3 fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
-        const D3 rho_sum = sw * fac;
-        const D3 dwdh = -h1 * (fac * sg);
+            });
+        };
+        if (s.range_shift) scan(std::true_type{});
+        else scan(std::false_type{});
+        const D1 fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
+        const D1 rho_sum = sw * fac;
+        const D1 dwdh = -h1 * (fac * sg);
 
         // DensityIteration.post_loop (equations.py:54-86)
         const double mi = me.w;
-        const D3 hi0 = ldT(F, G, TFLD(H0), i);
-        const D3 rhoi = mi / ipow_dim_t(hi / s.hfact, DIM);
-        const D3 dhdrhoi = -hi / ((double)DIM * rhoi);
-        const D3 omegai = 1.0 - dhdrhoi * dwdh;
-        stT(F, G, TFLD(OMEGA), i, 1.0 / omegai);
-        const D3 func = rhoi - rho_sum;
-        const D3 dfdh1 = omegai > 0.0 ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + 2.220446049250313e-16);
-        D3 hnew = hi - func * dfdh1;
+        const D1 hi0 = ldN<1>(F, G, TFLD(H0), i, dir);
+        const D1 rhoi = mi / ipow_dim_t(hi / s.hfact, DIM);
+        const D1 dhdrhoi = -hi / ((double)DIM * rhoi);
+        const D1 omegai = 1.0 - dhdrhoi * dwdh;
+        const D1 func = rhoi - rho_sum;
+        const D1 dfdh1 = omegai > 0.0 ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + 2.220446049250313e-16);
+        D1 hnew = hi - func * dfdh1;
         if (hnew > 1.2 * hi) hnew = 1.2 * hi;
         else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
         const double diff = ::fabs(hnew.v - hi.v) / hi0.v;
@@ -140,25 +183,27 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
         if (!fin) {
             hout = hnew;
             unconv = true;
-            F.f[SPHMHD_F_CONVERGED][i] = 0.0;
         } else {
             hout = s.hfact * iroot_dim_t(mi / rho_sum, DIM);
-            F.f[SPHMHD_F_CONVERGED][i] = 1.0;
         }
-        stT(F, G, TFLD(H), i, hout);
+        // every lane of the wave has read the old values (lockstep): store
+        __builtin_amdgcn_wave_barrier();
+        stN<1>(F, G, TFLD(OMEGA), i, 1.0 / omegai, dir, lead);
+        if (lead) F.f[SPHMHD_F_CONVERGED][i] = fin ? 1.0 : 0.0;
+        stN<1>(F, G, TFLD(H), i, hout, dir, lead);
     }
 
-    // the primal's list and reductions, on the values
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    // the primal's list and reductions, on the values (direction-0 lanes)
+    const bool mine = live && lead;
     constexpr int NW = kBlock / kWave;
     __shared__ int wcnt[NW], wbase[NW], wk[NW], wuc[NW];
     __shared__ unsigned long long whc[NW], whu[NW];
-    const unsigned long long um = __ballot(unconv);
-    const bool conv_now = live && !unconv;
+    const unsigned long long um = __ballot(unconv && lead);
+    const bool conv_now = mine && !unconv;
     unsigned long long hc = conv_now ? dbits(hout.v) : 0ull;
-    unsigned long long hu = unconv ? dbits(hout.v) : 0ull;
-    int uc = unconv ? 1 : 0;
-    const int k0 = __shfl(ki, 0);
+    unsigned long long hu = unconv && lead ? dbits(hout.v) : 0ull;
+    int uc = unconv && lead ? 1 : 0;
+    const int k0 = __shfl(ki, 0);  // lane 0 is valid whenever any lane is
     const bool uniform = __all(ki == k0 || !valid);
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -168,7 +213,7 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
             hu = o > hu ? o : hu;
             uc += __shfl_xor(uc, off);
         }
-    } else if (live) {
+    } else if (mine) {
         if (hc) atomicMax(&inst[ki].hmax_conv_bits, hc);
         if (hu) atomicMax(&inst[ki].hmax_next_bits, hu);
         if (uc) atomicAdd(&inst[ki].unconv, uc);
@@ -183,20 +228,20 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
     __syncthreads();
     if (threadIdx.x == 0) {
         int tot = 0;
-        for (int q = 0; q < NW; ++q) {
+        for (int q = 0; q < nw; ++q) {
             wbase[q] = tot;
             tot += wcnt[q];
         }
         if (tot) {
             const int base = atomicAdd(cnt_out, tot);
-            for (int q = 0; q < NW; ++q) wbase[q] += base;
+            for (int q = 0; q < nw; ++q) wbase[q] += base;
         }
-        for (int q = 0; q < NW;) {
+        for (int q = 0; q < nw;) {
             const int k = wk[q];
             unsigned long long c = whc[q], u = whu[q];
             int cu = wuc[q];
             int r = q + 1;
-            for (; r < NW && wk[r] == k; ++r) {
+            for (; r < nw && wk[r] == k; ++r) {
                 c = whc[r] > c ? whc[r] : c;
                 u = whu[r] > u ? whu[r] : u;
                 cu += wuc[r];
@@ -210,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
         }
     }
     __syncthreads();
-    if (unconv) {
+    if (unconv && lead) {
         const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
         act_out[wbase[w] + __popcll(um & lt)] = (int32_t)t;
     }
@@ -223,7 +268,7 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
 
 __global__ void k_t_prep(Fields F, TFields G, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
                          const int32_t* __restrict__ inst_of, double4* __restrict__ xh, double* __restrict__ aprev,
-                         int64_t n) {
+                         int64_t n, int keep_prev) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Inst& s = inst[inst_of[t]];
@@ -249,6 +294,9 @@ __global__ void k_t_prep(Fields F, TFields G, const int32_t* __restrict__ perm, 
     stT(F, G, TFLD(BZPRED), i, bz);
     const double z = dim == 3 ? F.f[SPHMHD_F_Z][i] : 0.0;
     xh[t] = make_double4(F.f[SPHMHD_F_X][i], F.f[SPHMHD_F_Y][i], z, h.v);
+    // the previous accelerations, with tangents, as the first prep of the
+    // evaluation found them (a redone force pass reads the same ones)
+    if (keep_prev) return;
     const int ka[3] = {TFLD(AU), TFLD(AV), TFLD(AW)};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -269,31 +317,6 @@ __device__ __forceinline__ D3 ld_aprev(const double* __restrict__ aprev, int c, 
 // ---------------------------------------------------------------------------
 // G3 + G5 + G7 + G8 pair loops and G3/G5/G6/G8/G9 epilogues (k_force over duals)
 // ---------------------------------------------------------------------------
-
-// loads and stores of ND directions starting at direction d0
-template <int ND>
-__device__ __forceinline__ Dual<ND> ldN(const Fields& F, const TFields& G, int k, int64_t i, int d0) {
-    Dual<ND> r(F.f[k][i]);
-#pragma unroll
-    for (int q = 0; q < ND; ++q) r.d[q] = G.f[d0 + q][k][i];
-    return r;
-}
-
-template <int ND>
-__device__ __forceinline__ void stN(const Fields& F, const TFields& G, int k, int64_t i, const Dual<ND>& x, int d0,
-                                    bool value) {
-    if (value) F.f[k][i] = x.v;
-#pragma unroll
-    for (int q = 0; q < ND; ++q) G.f[d0 + q][k][i] = x.d[q];
-}
-
-template <int ND>
-__device__ __forceinline__ Dual<ND> parN(double v, const InstTan& it, int p, int d0) {
-    Dual<ND> r(v);
-#pragma unroll
-    for (int q = 0; q < ND; ++q) r.d[q] = it.par[p][d0 + q];
-    return r;
-}
 
 template <int ND>
 __device__ __forceinline__ Dual<ND> ld_aprevN(const double* __restrict__ aprev, int c, int64_t n, int64_t i, int d0) {
@@ -367,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
     using D3 = Dual<ND>;
     const int d0 = blockIdx.y * ND;
     const bool lead = blockIdx.y == 0;
-    const int64_t t0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;  // 64, 128 or 256 (the force pass's tuned shape)
     if (t0 >= n) return;
     const int64_t t = t0 + threadIdx.x;
     const bool valid = t < n;
@@ -617,7 +640,8 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
     }
     // the primal's min dt_adapt and NaN flag per instance
     if (!lead) return;  // the whole workgroup: uniform
-    constexpr int NW = kBlock / kWave;
+    constexpr int NW = kBlock / kWave;  // LDS slots: waves of the largest shape
+    const int nw = blockDim.x / kWave;
     __shared__ unsigned long long wdb[NW];
     __shared__ int wk[NW], wbad[NW];
     const int w = threadIdx.x / kWave;
@@ -641,12 +665,12 @@ __global__ void __launch_bounds__(kBlock) k_t_force(Fields F, TFields G, const d
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 0; q < NW;) {
+        for (int q = 0; q < nw;) {
             const int k = wk[q];
             unsigned long long m = wdb[q];
             int b = wbad[q];
             int r = q + 1;
-            for (; r < NW && wk[r] == k; ++r) {
+            for (; r < nw && wk[r] == k; ++r) {
                 m = wdb[r] < m ? wdb[r] : m;
                 b |= wbad[r];
             }
@@ -937,24 +961,28 @@ void launch_t_density(Dev& d, TanDev& g) {
     const int32_t* cin = p == 0 ? nullptr : &d.ctl->act_cnt[(p - 1) & 1];
     int32_t* aout = d.act[p & 1];
     int32_t* cout = &d.ctl->act_cnt[p & 1];
+    const int B = d.blk[LC_DENSITY];
+    const int64_t per_block = (int64_t)(B / kWave) * kTPW;
+    const unsigned grid = (unsigned)((d.n + per_block - 1) / per_block);
 #define CALL_TD(KK, DD)                                                                                       \
-    { hipLaunchKernelGGL((k_t_density<KK, DD>), dim3(tblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, g.tf,     \
+    { hipLaunchKernelGGL((k_t_density<KK, DD>), dim3(grid), dim3(B), 0, d.stream, d.fields, g.tf,              \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, ain, \
                          cin, aout, cout); }
     T_DISPATCH(d.kernel, d.dim, CALL_TD)
 #undef CALL_TD
 }
 
-void launch_t_prep(Dev& d, TanDev& g) {
+void launch_t_prep(Dev& d, TanDev& g, bool keep_prev) {
     if (d.n == 0) return;
     hipLaunchKernelGGL(k_t_prep, dim3(tblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, g.tf, d.perm, d.inst,
-                       d.inst_of, (double4*)d.dpack, g.aprev, d.n);
+                       d.inst_of, (double4*)d.dpack, g.aprev, d.n, keep_prev ? 1 : 0);
 }
 
 void launch_t_force(Dev& d, TanDev& g) {
     if (d.n == 0) return;
+    const int B = d.blk[LC_FORCE];
 #define CALL_TF(KK, DD)                                                                                      \
-    { hipLaunchKernelGGL((k_t_force<KK, DD, 1>), dim3(tblk(d.n), kTanDirs), dim3(kBlock), 0, d.stream, d.fields, \
+    { hipLaunchKernelGGL((k_t_force<KK, DD, 1>), dim3((d.n + B - 1) / B, kTanDirs), dim3(B), 0, d.stream, d.fields, \
                          g.tf, g.aprev, d.perm, d.nb, d.nbcnt, d.inst, g.itan, d.inst_of, d.n, d.nb_stride); }
     T_DISPATCH(d.kernel, d.dim, CALL_TF)
 #undef CALL_TF
