@@ -236,6 +236,15 @@ int sphmhd_get_timing(sphmhd_ctx* ctx, sphmhd_timing* out);
 /* Block until the context's stream is idle. */
 int sphmhd_synchronize(sphmhd_ctx* ctx);
 
+/* Close-packed (HCP) lattice positions generated on the GPU: the
+ * reference's close_packed_lattice (src/cases/lattice.py:4-34), bit for bit,
+ * for the initial conditions of its cases.  bounds = {xmin, xmax, ymin,
+ * ymax, zmin, zmax}; x, y, z: device arrays of capacity cap; *count receives
+ * the number of sites (SPHMHD_ECAP if it exceeds cap, nothing written).
+ * Enqueued on hip_stream (0: the null stream); no context needed. */
+int sphmhd_hcp_lattice(const double* bounds, double spacing, double* x, double* y, double* z,
+                       int64_t cap, int64_t* count, void* hip_stream);
+
 /* Launch shapes.  No counterpart in the reference (PySPH picks its own launch
  * sizes); the results of a step do not depend on them, only its speed does.
  * blocks[3] = workgroup sizes of the density passes, the prep + neighbour-list

@@ -31,6 +31,13 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+// the library's error record for the other translation units
+int sph::report_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \

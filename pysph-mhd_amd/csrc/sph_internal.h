@@ -15,6 +15,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <string>
 #include <stdint.h>
 
 #include "../../include/sphmhd.h"
@@ -687,6 +689,9 @@ void launch_g0_control(Dev& d);
 void launch_prep_force(Dev& d, bool keep_prev);
 void launch_nb_build(Dev& d);
 void launch_force(Dev& d);
+// sets the message sphmhd_last_error() returns; returns code
+int report_error(int code, const std::string& msg);
+
 void launch_eval_end(Dev& d);
 void launch_ctl_export(Dev& d);
 void launch_clear_abort(Dev& d);

@@ -239,6 +239,8 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_tangent_get_dt": ([vp, dp], ctypes.c_int),
         "sphmhd_set_timing": ([vp, i32], ctypes.c_int),
         "sphmhd_set_launch": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "sphmhd_hcp_lattice": ([ctypes.POINTER(ctypes.c_double), ctypes.c_double, vp, vp, vp, i64,
+                                ctypes.POINTER(ctypes.c_int64), vp], ctypes.c_int),
         "sphmhd_get_launch": ([vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_get_timing": ([vp, ctypes.POINTER(Timing)], ctypes.c_int),
     }

@@ -84,37 +84,62 @@ def read_plan(groups) -> dict:
     )
 
 
+def domain_box(pa, domain):
+    """(lo, hi, periodic) of the cell grid for a particle array and a domain.
+
+    A periodic domain gives its bounds.  An open domain (the reference's
+    DomainManager() without bounds) gives the particles' starting box plus a
+    margin of its size on each side: a particle that later leaves it is
+    clamped into an edge cell, which only enlarges candidate sets (the radius
+    test decides), so the box sets the speed of the search, never its result.
+    """
+    if domain is not None and getattr(domain, "is_periodic", False):
+        b = domain.bounds
+        return (b[0], b[2], b[4]), (b[1], b[3], b[5]), True
+    lo, hi = [], []
+    for name in ("x", "y", "z"):
+        c = np.asarray(pa.properties[name], dtype=np.float64) if name in pa.properties else np.zeros(1)
+        c0, c1 = (float(c.min()), float(c.max())) if len(c) else (0.0, 0.0)
+        pad = max(0.5 * (c1 - c0), 1.0e-12 * max(abs(c0), abs(c1), 1.0))
+        lo.append(c0 - pad)
+        hi.append(c1 + pad)
+    return tuple(lo), tuple(hi), False
+
+
+def plan_params(pa, groups, kernel, domain, scheme=None, solver=None):
+    """Validate the plan and return (Params, plan constants) for the library:
+    constants from the equations, the grid box from the domain, the
+    dissipation switch range from the scheme and tf / adaptivity from the
+    solver."""
+    plan = read_plan(groups)
+    dim = plan["dim"]
+    if kernel is not None and kernel.dim != dim:
+        raise PlanError("kernel dimension differs from the plan's")
+    lo, hi, periodic = domain_box(pa, domain)
+    p = make_params(
+        dim=dim, kernel=kernel.kernel_id, lo=lo, hi=hi, gamma=plan["gamma"], mu0=plan["mu0"], hfact=plan["hfact"],
+        artificial_viscosity=plan["artificial_viscosity"],
+        artificial_thermal_conductivity=plan["artificial_thermal_conductivity"],
+        artificial_magnetic_dissipation=plan["artificial_magnetic_dissipation"],
+        cleaning_speed_factor=plan["cleaning_speed_factor"], cleaning_damping_factor=plan["cleaning_damping_factor"],
+        timestep_factor=plan["timestep_factor"],
+        alpha_min=getattr(scheme, "alpha_min", 0.0), alpha_max=getattr(scheme, "alpha_max", 1.0),
+        tf=float(getattr(solver, "tf", 1.0e30)), density="iterate" if plan["iterate"] else "single",
+        periodic=periodic, adaptive_timestep=bool(getattr(solver, "adaptive_timestep", True)),
+        max_iterations=plan["max_iterations"])
+    p.min_iterations = plan["min_iterations"]
+    return p, plan
+
+
 class HIPAccelerationEval:
     def __init__(self, pa, groups, kernel, domain, scheme=None, solver=None):
         from .device import DeviceSystem
 
         self.pa = pa
         self.groups = groups
-        self.plan = read_plan(groups)
+        self.params, self.plan = plan_params(pa, groups, kernel, domain, scheme, solver)
         self.density_eq = groups[0].equations[0]
-        dim = self.plan["dim"]
-        if kernel is not None and kernel.dim != dim:
-            raise PlanError("kernel dimension differs from the plan's")
-        if domain is None or not getattr(domain, "is_periodic", False):
-            raise PlanError("the device path needs a periodic domain (DomainManager with periodic bounds)")
-        b = domain.bounds
-        lo = (b[0], b[2], b[4])
-        hi = (b[1], b[3], b[5])
-        pl = self.plan
-        p = make_params(
-            dim=dim, kernel=kernel.kernel_id, lo=lo, hi=hi, gamma=pl["gamma"], mu0=pl["mu0"], hfact=pl["hfact"],
-            artificial_viscosity=pl["artificial_viscosity"],
-            artificial_thermal_conductivity=pl["artificial_thermal_conductivity"],
-            artificial_magnetic_dissipation=pl["artificial_magnetic_dissipation"],
-            cleaning_speed_factor=pl["cleaning_speed_factor"], cleaning_damping_factor=pl["cleaning_damping_factor"],
-            timestep_factor=pl["timestep_factor"],
-            alpha_min=getattr(scheme, "alpha_min", 0.0), alpha_max=getattr(scheme, "alpha_max", 1.0),
-            tf=float(getattr(solver, "tf", 1.0e30)), density="iterate" if pl["iterate"] else "single",
-            periodic=True, adaptive_timestep=bool(getattr(solver, "adaptive_timestep", True)),
-            max_iterations=pl["max_iterations"])
-        p.min_iterations = pl["min_iterations"]
-        self.params = p
-        self.system = DeviceSystem(p, n=len(pa))
+        self.system = DeviceSystem(self.params, n=len(pa))
         pa.gpu.attach(self.system)
         pa.gpu.push(*[n for n in FIELDS if n in pa.properties], "itype")
 

@@ -336,10 +336,9 @@ class Application:
                 continue
             i += 1
 
-    def setup(self, argv=None):
-        """Build scheme, solver, particles and the device evaluator."""
-        from .backend import HIPAccelerationEval
-
+    def prepare(self, argv=None):
+        """Build scheme, solver, particles and domain on the host (no device
+        work)."""
         self._parse(argv)
         self.initialize()
         self.scheme = self.create_scheme()
@@ -352,6 +351,13 @@ class Application:
         self.particles = self.create_particles()
         self.domain = self.create_domain()
         self.solver.particles = self.particles
+        return self
+
+    def setup(self, argv=None):
+        """prepare(), then bind the particles to the device evaluator."""
+        from .backend import HIPAccelerationEval
+
+        self.prepare(argv)
         self.evaluator = HIPAccelerationEval(self.particles[0], self.scheme.get_equations(), self.solver.kernel,
                                              self.domain, self.scheme, self.solver)
         self.solver.integrator.evaluator = self.evaluator

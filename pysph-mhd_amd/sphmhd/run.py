@@ -6,9 +6,13 @@ import argparse
 from pathlib import Path
 
 from .cases.apps import AlfvenWave2D, MHDBlast3D, MHDRotor2D, OrszagTang2D
+from .cases.reference import CASES as REFERENCE_CASES
 from .config import load_config, pysph_arguments
 
+# the reference's twelve case names (its TOMLs run unchanged) and the
+# benchmark workloads' 2-D and cubic-lattice variants
 CASES = {
+    **REFERENCE_CASES,
     "orstang2d": OrszagTang2D,
     "mhdrotor2d": MHDRotor2D,
     "mhdblast3d": MHDBlast3D,

@@ -1,0 +1,157 @@
+"""The reference's twelve cases on the HIP path: the GPU lattice generator
+against the reference's positions, and each case's warm start and first
+steps against the oracle.
+
+Cases start from exact close-packed lattices, where some outputs are
+analytically zero and hold only round-off (div v of a uniform or
+incompressible flow, w in a slab symmetric in z, the acceleration of a
+static gas); those are measured against 1e-3 of their physical scale
+(physical_scales), so the bar stays relative L2 <= 1e-10 everywhere.  The
+frozen walls of the shock tube are left out of the rate comparison: their
+dissipation switch is a 0/0 of round-off-level velocities and is never
+integrated.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import CASE_NX, RATES, STATE, case_app, case_setup, compare, init_fields, physical_scales
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-10
+GOLDEN = Path(__file__).resolve().parent / "golden" / "lattice_golden.npz"
+
+
+@pytest.mark.parametrize("case", range(11))
+def test_device_lattice_matches_reference_positions(gpu, case):
+    import torch
+
+    from sphmhd.cases.lattice import close_packed_lattice_device
+
+    z = np.load(GOLDEN, allow_pickle=False)
+    a, b = z["offsets"][case], z["offsets"][case + 1]
+    x, y, zz = close_packed_lattice_device(z["bounds"][case], float(z["spacing"][case]), device=gpu)
+    torch.cuda.synchronize()
+    for t, k in ((x, "x"), (y, "y"), (zz, "z")):
+        assert np.array_equal(t.cpu().numpy(), z[k][a:b]), k
+
+
+def test_device_lattice_rejects_small_capacity(gpu):
+    import ctypes
+
+    import torch
+
+    from sphmhd._abi import load_library
+    from sphmhd.cases.lattice import lattice_shape
+
+    lib = load_library()
+    nx, ny, nz, *_ = lattice_shape((-0.5, 0.5, -0.5, 0.5, -0.5, 0.5), 0.125)
+    out = torch.empty((3, 10), dtype=torch.float64, device=gpu)
+    bounds = (ctypes.c_double * 6)(-0.5, 0.5, -0.5, 0.5, -0.5, 0.5)
+    got = ctypes.c_int64(0)
+    rc = lib.sphmhd_hcp_lattice(bounds, ctypes.c_double(0.125), ctypes.c_void_p(out[0].data_ptr()),
+                                ctypes.c_void_p(out[1].data_ptr()), ctypes.c_void_p(out[2].data_ptr()), 10,
+                                ctypes.byref(got), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc != 0 and got.value == nx * ny * nz > 10
+
+
+def _pair(s):
+    from pyoracle import OracleSystem
+    from sphmhd.device import DeviceSystem
+
+    dev = DeviceSystem(s.params, s.offsets)
+    orc = OracleSystem(s.params, s.offsets, threads=16)
+    f = init_fields(s)
+    dev.set_fields(f)
+    orc.set_fields(f)
+    return dev, orc
+
+
+@pytest.mark.parametrize("name", sorted(CASE_NX))
+def test_reference_case_matches_oracle(gpu, name):
+    s = case_setup(name, CASE_NX[name])
+    p = s.params[0]
+    live = (s.itype != 3) if s.itype is not None else None
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    sc = physical_scales(orc, p.mu0, p.alpha_max, live)
+    compare(dev, orc, STATE + RATES, TOL, f"{name} warm start", scales=sc, mask=live)
+    for k in range(3):
+        dev.step(1)
+        orc.step(1)
+        sc = physical_scales(orc, p.mu0, p.alpha_max, live)
+        compare(dev, orc, STATE + RATES + ("z",), TOL, f"{name} step {k + 1}", scales=sc, mask=live)
+    td, dtd = dev.clock()
+    to, dto = orc.clock()
+    assert td[0] == pytest.approx(to[0], rel=1e-12) and dtd[0] == pytest.approx(dto[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("name", ["1jet", "12jet"])
+def test_open_domain_neighbours_bit_exact(gpu, name):
+    """Open-domain cell search on the GPU == the oracle's, also with
+    particles far outside the grid box."""
+    s = case_setup(name, CASE_NX[name])
+    x = s.fields["x"].copy()
+    x[:40] += 5.0
+    x[40:45] -= 3.0
+    s.fields["x"] = x
+    dev, orc = _pair(s)
+    kd, pd = dev.cells()
+    ko, po = orc.cells()
+    assert np.array_equal(kd, ko) and np.array_equal(pd, po)
+    od, idd = dev.neighbours()
+    oo, io = orc.neighbours()
+    assert np.array_equal(od, oo) and np.array_equal(idd, io)
+
+
+class _Wrap:
+    def __init__(self, pa):
+        self.pa = pa
+
+    def get(self, name):
+        self.pa.gpu.pull(name)
+        return self.pa.properties[name].copy()
+
+
+@pytest.mark.parametrize("name", ["mhdshock", "1jet"])
+def test_reference_application_runs(gpu, tmp_path, name):
+    """The application of a reference case (TOML, warm start, initial dt,
+    solver loop, dumps) on the HIP path ends where the oracle run through
+    the same sequence ends."""
+    from pyoracle import OracleSystem
+
+    app = case_app(name, CASE_NX[name], directory=str(tmp_path / "out"))
+    app.run(["-d", str(tmp_path / "out"), "--max-steps", "4"])
+    assert app.solver.count == 4
+    s = case_setup(name, CASE_NX[name])
+    p = s.params[0]
+    orc = OracleSystem(s.params, s.offsets, threads=16)
+    orc.set_fields(init_fields(s))
+    orc.warm_start()
+    orc.step(4)
+    live = (s.itype != 3) if s.itype is not None else None
+    compare(_Wrap(app.particles[0]), orc, STATE, TOL, name,
+            scales=physical_scales(orc, p.mu0, p.alpha_max, live), mask=live)
+    assert sorted((tmp_path / "out").glob("*.npz"))
+
+
+def test_vortex_refreshes_energy_after_first_density(gpu, tmp_path):
+    """mhdvortex's hook: after the warm start e = p(x, y) / ((gamma - 1) rho)
+    with rho the summed density, not the initial rho = 1."""
+    from sphmhd.cases.reference import mhdvortex_pressure
+
+    app = case_app("mhdvortex", CASE_NX["mhdvortex"], directory=str(tmp_path / "out"))
+    app.setup(["-d", str(tmp_path / "out"), "--max-steps", "0"])
+    app._warm_initial_derivatives(app.solver)
+    pa = app.particles[0]
+    pa.gpu.pull("e", "rho", "x", "y")
+    # the two evaluations after the refresh re-sum rho from the same h, so
+    # rho may move by round-off since e was set
+    p = mhdvortex_pressure(pa.x, pa.y)
+    assert np.allclose(pa.e, p / ((app.gamma - 1.0) * pa.rho), rtol=1e-12, atol=0)
+    assert np.max(np.abs(pa.e / (p / (app.gamma - 1.0)) - 1.0)) > 1e-9
