@@ -34,7 +34,9 @@ constexpr int kBigLds = 4096;    // larger cells up to this size are sorted in L
 
 // Force-pack record: the FP64 values the force pair loop reads from a
 // neighbour, 18 in 2-D (144 B) and 20 in 3-D (160 B), packed densely so that
-// a record is a whole number of 16-byte units.  The pair loop is bound by the
+// a record is a whole number of 16-byte units.  (A unit-major layout, one
+// array per unit, was measured too: it gathers no faster and is slower on the
+// 40-neighbour quintic workloads.)  The pair loop is bound by the
 // bytes it gathers, so whatever the loop can rebuild exactly from the record
 // is left out: p = (gamma - 1) rho e and 1/rho, P/rho^2, B/rho are recomputed
 // per pair with the same operations the prologue uses.  au, av, aw are the

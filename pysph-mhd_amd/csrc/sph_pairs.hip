@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
 // ---------------------------------------------------------------------------
 
 template <int DIM>
-__device__ __forceinline__ void write_force_record(double* __restrict__ fpack, int64_t t, const Fields& F, int32_t i,
+__device__ __forceinline__ void write_force_record(double2* stage, const Fields& F, int32_t i,
                                                    double x, double y, double z, double h, double m, double rho,
                                                    double p, double e, double bx, double by, double bz, double vwave,
                                                    double chf, double au, double av, double aw) {
@@ -277,18 +277,16 @@ __device__ __forceinline__ void write_force_record(double* __restrict__ fpack, i
     if constexpr (DIM == 3) r[R::AW] = aw;
     r[R::PSICH] = f[SPHMHD_F_PSIPRED][i] * (chf * vwave);
     r[R::VW] = vwave;
-    double2* out = (double2*)(fpack + t * R::N);
 #pragma unroll
-    for (int q = 0; q < R::UNITS; ++q) out[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    for (int q = 0; q < R::UNITS; ++q) stage[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
-__global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
-                             const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
-                             double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev,
-                             const Ctl* ctl) {
-    if (ctl->abort) return;
-    int64_t t = xcd_tile((n + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+// one particle of k_prep_force: the group G1/G2/G4 epilogue values into the
+// property rows, its force record into stage
+__device__ __forceinline__ void prep_force_one(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
+                                               const int32_t* __restrict__ inst_of, double2* stage,
+                                               double4* __restrict__ xh, int64_t t, int64_t n,
+                                               double* __restrict__ aprev, int keep_prev) {
     const Inst& s = inst[inst_of[t]];
     const int32_t i = perm[t];
     const int dim = s.dim;
@@ -335,9 +333,30 @@ __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const I
         aw = aprev[2 * n + i];
     }
     if (dim == 2)
-        write_force_record<2>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
+        write_force_record<2>(stage, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
     else
-        write_force_record<3>(fpack, t, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
+        write_force_record<3>(stage, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
+}
+
+__global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,
+                             const int32_t* __restrict__ inst_of, double* __restrict__ fpack,
+                             double4* __restrict__ xh, int64_t n, double* __restrict__ aprev, int keep_prev,
+                             const Ctl* ctl) {
+    if (ctl->abort) return;
+    const int64_t t0 = xcd_tile((n + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    if (t0 >= n) return;  // whole block past the end
+    // Records are built in LDS and stored as one contiguous run per block:
+    // each lane writing its own 144/160-byte record would leave every store
+    // instruction touching 64 partial lines.
+    __shared__ double2 stage[kBlock * kFRecMax / 2];
+    const int64_t t = t0 + threadIdx.x;
+    const int dim = inst[inst_of[t0]].dim;  // the same for every instance
+    const int units = dim == 3 ? FRec<3>::UNITS : FRec<2>::UNITS;
+    if (t < n) prep_force_one(F, perm, inst, inst_of, stage + threadIdx.x * units, xh, t, n, aprev, keep_prev);
+    __syncthreads();
+    const int64_t cnt = min((int64_t)blockDim.x, n - t0) * units;
+    double2* out = (double2*)fpack + t0 * units;
+    for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x) out[q] = stage[q];
 }
 
 // ---------------------------------------------------------------------------
