@@ -282,12 +282,15 @@ static int build_lists(sphmhd_ctx* c) {
     return fail(SPHMHD_ECAP, "neighbour list did not fit after growing");
 }
 
-// begun: the evaluation's start (eval_begin_body with reset_dt) already ran
-// in the tail of the fused stage kernel
-static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions, bool begun = false) {
-    if (!begun) launch_eval_begin(c->d, reset_clock_reductions);
-    launch_hmax_grid(c->d);
-    launch_keys_sort_ranges(c->d);
+// begun: the evaluation's start (eval_begin_body with reset_dt), the h_max
+// reduction and the grid set-up already ran in the fused stage kernel;
+// pack: the sort also writes the density pack
+static int nnps_rebuild(sphmhd_ctx* c, bool reset_clock_reductions, bool begun = false, bool pack = false) {
+    if (!begun) {
+        launch_eval_begin(c->d, reset_clock_reductions);
+        launch_hmax_grid(c->d);
+    }
+    launch_keys_sort_ranges(c->d, pack);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -337,9 +340,8 @@ static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
         }
         break;
     case P_NNPS: {
-        const int rc = nnps_rebuild(c, true, !first);
+        const int rc = nnps_rebuild(c, true, !first, true);
         if (rc) return rc;
-        launch_pack_density(d);
         break;
     }
     case P_DENSITY:
@@ -666,6 +668,8 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.inst, n_instances))) return cleanup(rc);
         if ((rc = dalloc(&d.ctl, 1))) return cleanup(rc);
         if ((rc = dalloc(&d.arrive, kArriveInts))) return cleanup(rc);
+        if ((rc = dalloc(&d.hblk, (n + kBlock - 1) / kBlock))) return cleanup(rc);
+        if ((rc = dalloc(&d.kblk, (n + kBlock - 1) / kBlock))) return cleanup(rc);
         if ((rc = dalloc(&d.inst_of, n))) return cleanup(rc);
         if ((rc = dalloc(&d.keys, n))) return cleanup(rc);
         if ((rc = dalloc(&d.keys_alt, n))) return cleanup(rc);
@@ -721,7 +725,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
     if (c->sev) (void)hipEventDestroy(c->sev);
-    void* ptrs[] = {d.inst, d.ctl, d.arrive, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
+    void* ptrs[] = {d.inst, d.ctl, d.arrive, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)

@@ -81,6 +81,7 @@ struct Inst {
     unsigned long long hmax_conv_bits;  // max h of converged particles (frozen for the evaluation)
     unsigned long long dtp_bits;        // min dt_adapt of the last force pass
     int32_t iters, unconv, done, status;
+    unsigned long long hmax_stage_bits; // max h after the fused stage kernel (zero between steps)
 };
 
 struct Ctl {
@@ -244,6 +245,94 @@ __device__ inline void stencil_from_hmax(Inst& s) {
         }
     }
     s.range_shift = shift_ok;
+}
+
+// Grid set-up (one workgroup): one thread per instance, then a serial prefix
+// of cell bases.
+__device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
+    int k = threadIdx.x;
+    for (; k < ninst; k += blockDim.x) {
+        Inst& s = inst[k];
+        int64_t nk = s.off1 - s.off0;
+        int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
+        double R = kernel_radius(s.kernel);
+        double c = R * bitsd(ld_dev(&s.hmax_bits));  // reduced by this launch's atomics
+        double lmax = 0.0;
+        for (int a = 0; a < s.dim; ++a) lmax = ::fmax(lmax, s.L[a]);
+        if (!(c > 0.0) || !(c <= lmax)) c = lmax;  // h_max of 0, NaN or above the box: one cell
+        for (int guard = 0; guard < 80; ++guard) {
+            int64_t prod = 1;
+            for (int a = 0; a < 3; ++a) {
+                if (a < s.dim) {
+                    double nf = floor(s.L[a] / c);
+                    int na = nf < 1.0 ? 1 : (nf > 1.0e9 ? 1000000000 : (int)nf);
+                    s.n[a] = na;
+                    prod *= na;
+                    if (prod > cap) break;
+                } else {
+                    s.n[a] = 1;
+                }
+            }
+            if (prod <= cap) break;
+            c = 2.0 * c;
+        }
+        for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
+        s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
+        stencil_from_hmax(s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t base = 0;
+        for (int q = 0; q < ninst; ++q) {
+            inst[q].cell_base = base;
+            base += inst[q].ncell;
+        }
+        ctl->total_cells = base;
+        ctl->nbig = 0;
+    }
+}
+
+// Max of the bit patterns b (positive doubles order like their bits) over a
+// block of kBlock threads, per run of equal instances: waves reduce by
+// shuffles, then thread 0 calls emit(k, m, whole) once per run, whole
+// telling that the run is the whole block.  kcur < 0: the thread holds
+// nothing.
+template <class Emit>
+__device__ __forceinline__ void block_max_runs(int kcur, unsigned long long b, Emit emit) {
+    __shared__ unsigned long long wmax[kBlock / kWave];
+    __shared__ int wk[kBlock / kWave];
+    __shared__ int wmixed;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (threadIdx.x == 0) wmixed = 0;
+    int k0 = kcur;
+    for (int off = 32; off > 0; off >>= 1) k0 = max(k0, __shfl_xor(k0, off));
+    const bool uniform = __all(kcur == k0 || kcur < 0);
+    __syncthreads();
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(b, off);
+            b = o > b ? o : b;
+        }
+    } else {
+        if (kcur >= 0) emit(kcur, b, false);  // rare: a wave straddling instances
+        if (lane == 0) wmixed = 1;
+    }
+    if (lane == 0) {
+        wmax[wave] = b;
+        wk[wave] = uniform ? k0 : -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < nw;) {
+            const int k = wk[q];
+            unsigned long long m = wmax[q];
+            int r = q + 1;
+            for (; r < nw && wk[r] == k; ++r) m = wmax[r] > m ? wmax[r] : m;
+            if (k >= 0) emit(k, m, q == 0 && r == nw && !wmixed);
+            q = r;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -618,6 +707,8 @@ struct Dev {
     Ctl* ctl;
     Ctl* hctl_dev;          // device view of the pinned host mirror of ctl: 3 slots
     int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
+    unsigned long long* hblk;  // per block of the fused stage kernel: its h_max bits (instance in kblk)
+    int32_t* kblk;          // that block's instance, or -1 when it straddles instances
     int32_t blk[3];         // workgroup size of the density, list (prep + build) and force launches
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
     uint32_t *keys, *keys_alt;
@@ -682,7 +773,7 @@ void launch_reset_nb_stats(Dev& d);
 void launch_reset_force_outputs(Dev& d);
 void launch_count_unconverged(Dev& d);
 void launch_hmax_grid(Dev& d);
-void launch_keys_sort_ranges(Dev& d);
+void launch_keys_sort_ranges(Dev& d, bool pack);  // pack: also write the density pack
 void launch_pack_density(Dev& d);
 void launch_density(Dev& d);
 void launch_g0_control(Dev& d);

@@ -40,7 +40,6 @@ __global__ void k_eval_begin(Inst* inst, int ninst, Ctl* ctl, int reset_dt) {
 constexpr int kHmaxBlocks = 128;
 constexpr int kHmaxBatch = 8;
 
-__device__ void grid_setup(Inst* inst, int ninst, Ctl* ctl);
 
 __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
                        int64_t n, int next, int64_t chunk, int ninst, Ctl* ctl, int with_grid,
@@ -102,51 +101,6 @@ __global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__
         }
     }
     if (with_grid && last_to_arrive(arrive)) grid_setup(inst, ninst, ctl);
-}
-
-// Grid set-up (one workgroup): one thread per instance, then a serial prefix
-// of cell bases.
-__device__ void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
-    int k = threadIdx.x;
-    for (; k < ninst; k += blockDim.x) {
-        Inst& s = inst[k];
-        int64_t nk = s.off1 - s.off0;
-        int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
-        double R = kernel_radius(s.kernel);
-        double c = R * bitsd(ld_dev(&s.hmax_bits));  // reduced by this launch's atomics
-        double lmax = 0.0;
-        for (int a = 0; a < s.dim; ++a) lmax = fmax(lmax, s.L[a]);
-        if (!(c > 0.0) || !(c <= lmax)) c = lmax;  // h_max of 0, NaN or above the box: one cell
-        for (int guard = 0; guard < 80; ++guard) {
-            int64_t prod = 1;
-            for (int a = 0; a < 3; ++a) {
-                if (a < s.dim) {
-                    double nf = floor(s.L[a] / c);
-                    int na = nf < 1.0 ? 1 : (nf > 1.0e9 ? 1000000000 : (int)nf);
-                    s.n[a] = na;
-                    prod *= na;
-                    if (prod > cap) break;
-                } else {
-                    s.n[a] = 1;
-                }
-            }
-            if (prod <= cap) break;
-            c = 2.0 * c;
-        }
-        for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
-        s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
-        stencil_from_hmax(s);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t base = 0;
-        for (int q = 0; q < ninst; ++q) {
-            inst[q].cell_base = base;
-            base += inst[q].ncell;
-        }
-        ctl->total_cells = base;
-        ctl->nbig = 0;
-    }
 }
 
 __global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) { grid_setup(inst, ninst, ctl); }
@@ -310,10 +264,23 @@ __global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict_
     if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)c;
 }
 
-// small cells: a particle's place is its rank by storage index in its bucket
+// the density pack {x, y, z, m} of storage index i
+__device__ __forceinline__ double4 density_record(const Fields& F, int32_t i, int dim) {
+    double4 r;
+    r.x = F.f[SPHMHD_F_X][i];
+    r.y = F.f[SPHMHD_F_Y][i];
+    r.z = dim == 3 ? F.f[SPHMHD_F_Z][i] : 0.0;
+    r.w = F.f[SPHMHD_F_M][i];
+    return r;
+}
+
+// small cells: a particle's place is its rank by storage index in its bucket;
+// with dpack, its density record goes to that place too (k_pack_density's
+// work, without a launch of its own)
 __global__ void k_cell_rank(const int32_t* __restrict__ bucket, const uint32_t* __restrict__ keys,
                             const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n, const Ctl* ctl) {
+                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n, const Ctl* ctl,
+                            Fields F, double4* __restrict__ dpack, int dim) {
     if (ctl->abort) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -325,6 +292,7 @@ __global__ void k_cell_rank(const int32_t* __restrict__ bucket, const uint32_t* 
     for (int32_t k = s; k < e; ++k) r += bucket[k] < v ? 1 : 0;
     perm[s + r] = v;
     skey[s + r] = c;
+    if (dpack) dpack[s + r] = density_record(F, v, dim);
 }
 
 // ascending bitonic network over P = 2^k slots of which the first m are
@@ -353,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_big(const int32_t* __restrict__
                                                      const int32_t* __restrict__ cend,
                                                      const int32_t* __restrict__ bucket, int32_t* perm,
                                                      uint32_t* __restrict__ skey, const int32_t* __restrict__ bigc,
-                                                     const Ctl* ctl) {
+                                                     const Ctl* ctl, Fields F, double4* __restrict__ dpack, int dim) {
     if (ctl->abort) return;
     __shared__ int32_t lds[kBigLds];
     const int nbig = ctl->nbig;
@@ -375,6 +343,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_big(const int32_t* __restrict__
             bitonic_sort(perm + s, m, P);
         }
         __syncthreads();
+        if (dpack)
+            for (int64_t q = threadIdx.x; q < m; q += blockDim.x) dpack[s + q] = density_record(F, perm[s + q], dim);
     }
 }
 
@@ -387,13 +357,7 @@ __global__ void k_pack_density(Fields F, const int32_t* __restrict__ perm, doubl
     if (ctl->abort) return;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    int32_t i = perm[t];
-    double4 r;
-    r.x = F.f[SPHMHD_F_X][i];
-    r.y = F.f[SPHMHD_F_Y][i];
-    r.z = dim == 3 ? F.f[SPHMHD_F_Z][i] : 0.0;
-    r.w = F.f[SPHMHD_F_M][i];
-    dpack[t] = r;
+    dpack[t] = density_record(F, perm[t], dim);
 }
 
 // positions and h only, for neighbour exports outside an evaluation
@@ -554,7 +518,7 @@ void launch_hmax_grid(Dev& d) {
 
 constexpr int kBigBlocks = 64;
 
-void launch_keys_sort_ranges(Dev& d) {
+void launch_keys_sort_ranges(Dev& d, bool pack) {
     if (d.n == 0) return;
     // keys_alt: keys by storage index; perm_alt: the buckets; perm: the
     // previous order on entry, the new one on exit
@@ -567,10 +531,11 @@ void launch_keys_sort_ranges(Dev& d) {
                        d.cstart, d.cblock, d.perm_alt, d.n, d.ctl);
     hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,
                        d.bigc, d.ctl, d.cell_cap);
+    double4* dp = pack ? (double4*)d.dpack : nullptr;
     hipLaunchKernelGGL(k_cell_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm_alt, d.keys_alt, d.cstart,
-                       d.cend, d.perm, d.keys, d.n, d.ctl);
+                       d.cend, d.perm, d.keys, d.n, d.ctl, d.fields, dp, d.dim);
     hipLaunchKernelGGL(k_cell_big, dim3(kBigBlocks), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.perm_alt, d.perm,
-                       d.keys, d.bigc, d.ctl);
+                       d.keys, d.bigc, d.ctl, d.fields, dp, d.dim);
 }
 
 void launch_pack_density(Dev& d) {

@@ -131,8 +131,8 @@ __global__ void k_stage2(Fields F, const Inst* __restrict__ inst, const int32_t*
 // expression (so it rounds identically), but every field is loaded once into
 // registers and every output stored once: through the property pointers the
 // compiler cannot tell the fields apart and would reload and re-store them.
-__device__ __forceinline__ void stage2_init_stage1_one(const Fields& F, const Inst* __restrict__ inst,
-                                                       const int32_t* __restrict__ inst_of, int64_t i) {
+__device__ __forceinline__ double stage2_init_stage1_one(const Fields& F, const Inst* __restrict__ inst,
+                                                         const int32_t* __restrict__ inst_of, int64_t i) {
     const Inst& s = inst[inst_of[i]];
     const double dt = s.dt;
     const double dtn = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
@@ -237,6 +237,7 @@ __device__ __forceinline__ void stage2_init_stage1_one(const Fields& F, const In
     FLD(BEVOLYPRED) = byp;
     FLD(BEVOLZPRED) = bzp;
     FLD(PSIPRED) = psip;
+    return h;
 }
 
 // t += dt; dt = the pending step clipped to tf (solver.py:57-61)
@@ -246,18 +247,67 @@ __device__ __forceinline__ void advance_clock(Inst& s) {
     s.t = t;
 }
 
-// the last workgroup then commits the clock and starts the next evaluation
-// (eval_begin_body with the pending dt cleared), which would otherwise be
-// two kernels of one thread per instance
-__global__ void k_stage2_init_stage1(Fields F, Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
-                                     int ninst, Ctl* ctl, int32_t* arrive) {
+// The predicted h of every particle is reduced to its instance's h_max on the
+// way (the reduction k_hmax would otherwise launch for): each block leaves
+// its maximum in its own slot (hblk, kblk; same-address atomics from every
+// block would serialise), and the last workgroup combines the slots, commits
+// the clock, starts the next evaluation (eval_begin_body with the pending dt
+// cleared) and sets up its cell grid: work that would otherwise be three more
+// launches.
+__global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* inst,
+                                                               const int32_t* __restrict__ inst_of, int64_t n,
+                                                               int ninst, Ctl* ctl, int32_t* arrive,
+                                                               unsigned long long* hblk, int32_t* kblk) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) stage2_init_stage1_one(F, inst, inst_of, i);
+    int k = -1;
+    unsigned long long hb = 0ull;
+    if (i < n) {
+        hb = dbits(stage2_init_stage1_one(F, inst, inst_of, i));
+        k = inst_of[i];
+    }
+    bool slot = false;
+    block_max_runs(k, hb, [&](int q, unsigned long long m, bool whole) {
+        if (whole) {
+            st_dev(&hblk[blockIdx.x], m);
+            st_dev(&kblk[blockIdx.x], q);
+            slot = true;
+        } else {
+            atomicMax(&inst[q].hmax_stage_bits, m);  // a block straddling instances
+        }
+    });
+    if (threadIdx.x == 0 && !slot) st_dev(&kblk[blockIdx.x], -1);
     if (last_to_arrive(arrive)) {
-        for (int k = threadIdx.x; k < ninst; k += blockDim.x) advance_clock(inst[k]);
+        // the blocks' slots, each thread a contiguous chunk
+        const int nb = (int)gridDim.x;
+        const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int q0 = (int)threadIdx.x * per, q1 = min(nb, q0 + per);
+        int kc = -1;
+        unsigned long long mc = 0ull;
+        for (int q = q0; q < q1; ++q) {
+            const int kq = ld_dev(&kblk[q]);
+            if (kq < 0) continue;
+            const unsigned long long v = ld_dev(&hblk[q]);
+            if (kq != kc) {
+                if (kc >= 0) atomicMax(&inst[kc].hmax_stage_bits, mc);
+                kc = kq;
+                mc = 0ull;
+            }
+            mc = v > mc ? v : mc;
+        }
+        block_max_runs(kc, mc, [&](int q, unsigned long long m, bool) { atomicMax(&inst[q].hmax_stage_bits, m); });
+        __syncthreads();
+        for (int q = threadIdx.x; q < ninst; q += blockDim.x) advance_clock(inst[q]);
         __syncthreads();
         eval_begin_body(inst, ninst, ctl, 1);
+        __syncthreads();
+        // the reduced h_max, cleared for the next step; the same thread then
+        // sets up that instance's grid
+        for (int q = threadIdx.x; q < ninst; q += blockDim.x) {
+            inst[q].hmax_bits = ld_dev(&inst[q].hmax_stage_bits);
+            inst[q].hmax_stage_bits = 0ull;
+        }
+        grid_setup(inst, ninst, ctl);
     }
 }
 
@@ -373,7 +423,7 @@ void launch_stage2(Dev& d) {
 void launch_stage2_init_stage1(Dev& d) {
     if (d.n)
         hipLaunchKernelGGL(k_stage2_init_stage1, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
-                           d.inst_of, d.n, d.ninst, d.ctl, d.arrive);
+                           d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
 }
 void launch_clock(Dev& d) {
     hipLaunchKernelGGL(k_clock, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);
