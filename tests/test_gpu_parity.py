@@ -77,12 +77,21 @@ def test_cells_crowded_bit_exact(gpu, nx, hbig):
     assert np.array_equal(pd, po)
 
 
-@pytest.mark.parametrize("which", ["ot2d", "blast3d", "ensemble"])
+@pytest.mark.parametrize("which", ["ot2d", "ot2d_large", "rotor2d", "blast3d", "ensemble", "ensemble_large"])
 def test_neighbours_bit_exact(gpu, which):
-    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered
+    """2-D at 40^2 (LDS-staged blocks spanning whole cell rows), 160^2 and
+    the 128^2 rotor (staged single-row blocks), 3-D, and ensembles small and
+    large (blocks straddling instances scan from global memory)."""
+    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered, rotor_2d
 
     if which == "ot2d":
         s = _ot(40)
+    elif which == "ot2d_large":
+        s = _ot(160)
+    elif which == "rotor2d":
+        s = jittered(rotor_2d(128), 0.1, seed=9)
+    elif which == "ensemble_large":
+        s = jittered(ensemble_ot(3, sizes=[96, 131, 160]), 0.1, seed=6)
     elif which == "blast3d":
         s = jittered(blast_3d(14), 0.1, seed=3)
     else:
