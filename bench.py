@@ -26,6 +26,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "pysph-mhd_amd"))
 
@@ -297,6 +299,47 @@ def cpu_baseline(name: str, steps: int):
     }
 
 
+def cpu_c1(steps: int = 100, steps_one_thread: int = 20):
+    """The CPU measurement of the C1 configuration (2-D Orszag-Tang, 128^2
+    particles, quintic, PEC steps after the warm start) on the oracle at one
+    thread (a bounded sample of steps_one_thread steps, about 10 s) and at
+    every host core (the full 100 steps), with the conserved totals at steps
+    0 and 100 of the latter: particle-steps/s, the all-core/1-thread ratio
+    and the relative drift of mass, momentum and total energy."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyoracle import OracleSystem
+
+    from sphmhd.cases.baseline import orszag_tang_2d
+
+    cores = len(os.sched_getaffinity(0))
+    s = orszag_tang_2d(128)
+    out = {"config": "C1: Orszag-Tang 2-D, 128^2 = 16,384 particles, quintic, h = 1.2 dx, FP64",
+           "cores_available": cores}
+    for label, threads, nst in (("one_thread", 1, steps_one_thread), ("all_cores", min(cores, 16), steps)):
+        orc = OracleSystem(s.params, s.offsets, threads=threads)
+        orc.set_fields(dict(s.fields))
+        orc.warm_start()
+        t0 = orc.totals()[0]
+        tic = time.perf_counter()
+        orc.step(nst)
+        wall = time.perf_counter() - tic
+        t1 = orc.totals()[0]
+        orc.close()
+        out[label] = {"threads": threads, "steps": nst, "particle_steps_per_s": round(s.n * nst / wall, 1),
+                      "seconds": round(wall, 2)}
+    e0 = t0[4] + t0[5] + t0[6]
+    e1 = t1[4] + t1[5] + t1[6]
+    out["speedup_all_cores"] = round(out["all_cores"]["particle_steps_per_s"] / out["one_thread"]["particle_steps_per_s"], 2)
+    out["totals_drift"] = {  # over the all-core run's steps
+        "mass": float(abs(t1[0] - t0[0]) / abs(t0[0])),
+        "momentum": float(np.hypot(t1[1] - t0[1], t1[2] - t0[2]) / np.sqrt(2.0 * t0[0] * t0[4])),
+        "energy": float(abs(e1 - e0) / abs(e0)),
+        "divb_start": float(t0[7]), "divb_end": float(t1[7]),
+        "note": "momentum against sqrt(2 M E_kin); energy = kinetic + thermal + magnetic",
+    }
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -356,6 +399,8 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)
+        if not args.no_extras:
+            line["cpu_c1"] = cpu_c1()
     print(json.dumps(line))
 
 
