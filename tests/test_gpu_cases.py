@@ -155,3 +155,29 @@ def test_vortex_refreshes_energy_after_first_density(gpu, tmp_path):
     p = mhdvortex_pressure(pa.x, pa.y)
     assert np.allclose(pa.e, p / ((app.gamma - 1.0) * pa.rho), rtol=1e-12, atol=0)
     assert np.max(np.abs(pa.e / (p / (app.gamma - 1.0)) - 1.0)) > 1e-9
+
+
+@pytest.mark.parametrize("name,nx", [("orstang", 48), ("mhdrotor", 32), ("alfven", 16), ("1jet", 12)])
+def test_reference_case_100_steps(gpu, name, nx):
+    """A hundred steps of a reference case against the oracle: every field to
+    1e-10 (physical floors as above) at steps 50 and 100, and the conserved
+    totals to 1e-8 (momentum against sqrt(2 M E_kin), the total being zero
+    up to round-off)."""
+    s = case_setup(name, nx)
+    p = s.params[0]
+    live = (s.itype != 3) if s.itype is not None else None
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    for k in range(2):
+        dev.step(50)
+        orc.step(50)
+        compare(dev, orc, STATE + RATES + ("z",), TOL, f"{name} step {50 * (k + 1)}",
+                scales=physical_scales(orc, p.mu0, p.alpha_max, live), mask=live)
+    td, to = dev.totals()[0], orc.totals()[0]
+    mass, ekin = to[0], to[4]
+    assert abs(td[0] - to[0]) <= 1e-8 * abs(mass)
+    pscale = np.sqrt(2.0 * mass * ekin)
+    assert np.linalg.norm(td[1:4] - to[1:4]) <= 1e-8 * pscale
+    e_d, e_o = td[4] + td[5] + td[6], to[4] + to[5] + to[6]
+    assert abs(e_d - e_o) <= 1e-8 * abs(e_o)
