@@ -299,9 +299,9 @@ def cpu_baseline(name: str, steps: int):
     }
 
 
-def cpu_c1(steps: int = 100, steps_one_thread: int = 20):
-    """The CPU measurement of the C1 configuration (2-D Orszag-Tang, 128^2
-    particles, quintic, PEC steps after the warm start) on the oracle at one
+def cpu_ot128(steps: int = 100, steps_one_thread: int = 20):
+    """The oracle on the 2-D Orszag-Tang vortex at 128^2 particles (quintic,
+    PEC steps after the warm start) on the oracle at one
     thread (a bounded sample of steps_one_thread steps, about 10 s) and at
     every host core (the full 100 steps), with the conserved totals at steps
     0 and 100 of the latter: particle-steps/s, the all-core/1-thread ratio
@@ -313,7 +313,7 @@ def cpu_c1(steps: int = 100, steps_one_thread: int = 20):
 
     cores = len(os.sched_getaffinity(0))
     s = orszag_tang_2d(128)
-    out = {"config": "C1: Orszag-Tang 2-D, 128^2 = 16,384 particles, quintic, h = 1.2 dx, FP64",
+    out = {"config": "Orszag-Tang 2-D, 128^2 = 16,384 particles, quintic, h = 1.2 dx, FP64",
            "cores_available": cores}
     for label, threads, nst in (("one_thread", 1, steps_one_thread), ("all_cores", min(cores, 16), steps)):
         orc = OracleSystem(s.params, s.offsets, threads=threads)
@@ -400,7 +400,7 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)
         if not args.no_extras:
-            line["cpu_c1"] = cpu_c1()
+            line["cpu_ot128"] = cpu_ot128()
     print(json.dumps(line))
 
 
