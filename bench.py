@@ -301,8 +301,8 @@ def cpu_baseline(name: str, steps: int):
 
 def cpu_ot128(steps: int = 100, steps_one_thread: int = 20):
     """The oracle on the 2-D Orszag-Tang vortex at 128^2 particles (quintic,
-    PEC steps after the warm start) on the oracle at one
-    thread (a bounded sample of steps_one_thread steps, about 10 s) and at
+    PEC steps after the warm start) at one thread (a bounded sample of
+    steps_one_thread steps, about 10 s) and at
     every host core (the full 100 steps), with the conserved totals at steps
     0 and 100 of the latter: particle-steps/s, the all-core/1-thread ratio
     and the relative drift of mass, momentum and total energy."""
