@@ -19,9 +19,15 @@
  *             RHOIJ = (rho_i + rho_j)/2, WI/GHI/DWI with h_i, DWJ with h_j,
  *             self pairs included.
  *   neighbours j is a neighbour of i iff r^2 < (R max(h_i, h_j))^2.
- *   cells     built once per evaluation from h_max (positions do not move
- *             during an evaluation): c = R h_max, n_a = max(1, floor(L_a/c)),
- *             c doubled while prod n_a > max(1, 2 N); c_eff_a = L_a / n_a;
+ *   cells     built once per evaluation from h_max and h_min (positions do
+ *             not move during an evaluation): c = R h_max / k with
+ *             k = min(floor(h_max/h_min), 4) when h_max >= 2 h_min, else 1
+ *             (cells split where h varies, so the small-h particles do not
+ *             scan cells sized for the largest h); n_a = max(1, floor(L_a/c)),
+ *             while prod n_a > max(1, 2 N) the split is undone first and c
+ *             then doubled; c_eff_a = L_a / n_a over the box: the domain on
+ *             periodic axes, else the particles' box at this evaluation
+ *             widened by box_pad on each side;
  *             key = ix + n_x (iy + n_y iz) with i_a = floor((x_a - lo_a)/c_eff_a)
  *             clamped to [0, n_a - 1]; particles stably sorted by key.
  *   stencil   per pass, s_a = max(1, ceil(R h_max_now / c_eff_a)); an axis
@@ -78,6 +84,7 @@ typedef struct {
 
 typedef struct {
     int n[3];
+    double lo[3]; /* the grid's origin: the domain's on periodic axes, else the particles' box */
     double ceff[3], L[3], halfL[3];
     int64_t ncell;
     int64_t base; /* global key offset of this instance */
@@ -848,22 +855,63 @@ static double inst_hmax(orc_t* o, int k) {
     return hm;
 }
 
-static void grid_setup(orc_t* o, int k, double hmax) {
+static double inst_hmin(orc_t* o, int k) {
+    double hm = 0.0;
+    for (int64_t i = o->off[k]; i < o->off[k + 1]; ++i)
+        if (i == o->off[k] || o->p[P_h][i] < hm) hm = o->p[P_h][i];
+    return hm;
+}
+
+/* cells per h_max cell along an axis: k of the cell rule above */
+static double cell_split(double hmax, double hmin) {
+    if (hmin > 0.0 && hmax / hmin >= 2.0) return fmin(floor(hmax / hmin), 4.0);
+    return 1.0;
+}
+
+/* margin of an open domain's grid box around the particles' extent
+   [mn, mx]: 1% of the extent, at least 1e-9 of its largest coordinate
+   (and of 1), so the box is never empty */
+static double box_pad(double mn, double mx) {
+    return fmax(0.01 * (mx - mn), 1.0e-9 * fmax(fmax(fabs(mn), fabs(mx)), 1.0));
+}
+
+static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
     const sphmhd_params* P = &o->prm[k];
     grid_t* g = &o->grid[k];
     int64_t nk = o->off[k + 1] - o->off[k];
     int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
+    double lo[3] = {0.0, 0.0, 0.0}, L[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < P->dim; ++a) {
+        lo[a] = P->lo[a];
+        L[a] = P->hi[a] - P->lo[a];
+        if (!P->periodic && nk > 0) {
+            /* open domain: the particles' box at this evaluation */
+            double mn = o->p[P_x + a][o->off[k]], mx = mn;
+            for (int64_t i = o->off[k] + 1; i < o->off[k + 1]; ++i) {
+                double v = o->p[P_x + a][i];
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+            double pad = box_pad(mn, mx);
+            double hi = mx + pad;
+            lo[a] = mn - pad;
+            L[a] = hi - lo[a];
+        }
+    }
     double R = orc_kernel_radius(P->kernel);
-    double c = R * hmax;
+    double split = cell_split(hmax, hmin);
+    double c = R * hmax / split;
     double lmax = 0.0;
-    for (int a = 0; a < P->dim; ++a) lmax = dmax(lmax, P->hi[a] - P->lo[a]);
-    if (!(c > 0.0) || !(c <= lmax)) c = lmax; /* h_max of 0, NaN or above the box: one cell */
+    for (int a = 0; a < P->dim; ++a) lmax = dmax(lmax, L[a]);
+    if (!(c > 0.0) || !(c <= lmax)) { /* h_max of 0, NaN or above the box: one cell */
+        c = lmax;
+        split = 1.0;
+    }
     for (int guard = 0; guard < 80; ++guard) {
         int64_t prod = 1;
         for (int a = 0; a < 3; ++a) {
             if (a < P->dim) {
-                double L = P->hi[a] - P->lo[a];
-                double nf = floor(L / c);
+                double nf = floor(L[a] / c);
                 int na = nf < 1.0 ? 1 : (nf > 1.0e9 ? 1000000000 : (int)nf);
                 g->n[a] = na;
                 prod *= na;
@@ -873,14 +921,22 @@ static void grid_setup(orc_t* o, int k, double hmax) {
             }
         }
         if (prod <= cap) break;
-        c = 2.0 * c;
+        /* too many cells: first undo the split, then coarsen */
+        if (split > 1.0) {
+            split -= 1.0;
+            c = R * hmax / split;
+        } else {
+            c = 2.0 * c;
+        }
     }
     for (int a = 0; a < 3; ++a) {
         if (a < P->dim) {
-            g->L[a] = P->hi[a] - P->lo[a];
+            g->lo[a] = lo[a];
+            g->L[a] = L[a];
             g->ceff[a] = g->L[a] / g->n[a];
             g->halfL[a] = 0.5 * g->L[a];
         } else {
+            g->lo[a] = 0.0;
             g->L[a] = 0.0;
             g->ceff[a] = 1.0;
             g->halfL[a] = 0.0;
@@ -891,7 +947,7 @@ static void grid_setup(orc_t* o, int k, double hmax) {
 
 static int cell_coord(const grid_t* g, const sphmhd_params* P, int a, double x) {
     if (a >= P->dim) return 0;
-    double f = floor((x - P->lo[a]) / g->ceff[a]);
+    double f = floor((x - g->lo[a]) / g->ceff[a]);
     if (f < 0.0) return 0;
     if (f > (double)(g->n[a] - 1)) return g->n[a] - 1;
     return (int)f;
@@ -901,7 +957,7 @@ static int cell_coord(const grid_t* g, const sphmhd_params* P, int a, double x) 
 static void nnps_build(orc_t* o) {
     int64_t base = 0;
     for (int k = 0; k < o->ninst; ++k) {
-        grid_setup(o, k, inst_hmax(o, k));
+        grid_setup(o, k, inst_hmax(o, k), inst_hmin(o, k));
         o->grid[k].base = base;
         base += o->grid[k].ncell;
     }

@@ -600,6 +600,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         d.n = n;
         d.ninst = n_instances;
         d.dim = p0.dim;
+        d.periodic = p0.periodic;
         d.kernel = p0.kernel;
         d.stream = (hipStream_t)hip_stream;
         if (const char* g = std::getenv("SPHMHD_GRAPHS")) c->use_graphs = std::strcmp(g, "0") != 0;
@@ -668,7 +669,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.inst, n_instances))) return cleanup(rc);
         if ((rc = dalloc(&d.ctl, 1))) return cleanup(rc);
         if ((rc = dalloc(&d.arrive, kArriveInts))) return cleanup(rc);
-        if ((rc = dalloc(&d.hblk, (n + kBlock - 1) / kBlock))) return cleanup(rc);
+        if ((rc = dalloc(&d.hblk, kRed * ((n + kBlock - 1) / kBlock)))) return cleanup(rc);
         if ((rc = dalloc(&d.kblk, (n + kBlock - 1) / kBlock))) return cleanup(rc);
         if ((rc = dalloc(&d.inst_of, n))) return cleanup(rc);
         if ((rc = dalloc(&d.keys, n))) return cleanup(rc);

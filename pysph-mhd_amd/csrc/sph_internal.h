@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <string>
 #include <stdint.h>
 
@@ -81,7 +82,11 @@ struct Inst {
     unsigned long long hmax_conv_bits;  // max h of converged particles (frozen for the evaluation)
     unsigned long long dtp_bits;        // min dt_adapt of the last force pass
     int32_t iters, unconv, done, status;
-    unsigned long long hmax_stage_bits; // max h after the fused stage kernel (zero between steps)
+    // the evaluation's start: h_max, h_min and the particles' box, reduced as
+    // maxima of kRed bit patterns (see red_values); red_stage accumulates
+    // them in the fused stage kernel (zero between steps)
+    unsigned long long red[8];
+    unsigned long long red_stage[8];
 };
 
 struct Ctl {
@@ -214,6 +219,7 @@ __device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int rese
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
         s.hmax_bits = 0ull;
+        for (int q = 0; q < 8; ++q) s.red[q] = 0ull;
         s.hmax_next_bits = 0ull;
         s.hmax_conv_bits = 0ull;
         if (reset_dt) s.dtp_bits = 0x7FF0000000000000ull;  // +inf
@@ -247,6 +253,48 @@ __device__ inline void stencil_from_hmax(Inst& s) {
     s.range_shift = shift_ok;
 }
 
+// Reductions at an evaluation's start, all as maxima of 64-bit patterns (zero
+// = nothing yet): 0 h_max (positive doubles order like their bits), 1 the
+// complement of h_min's bits, 2-4 the ordered bits of max x, y, z, 5-7 the
+// complements of the ordered bits of min x, y, z.
+constexpr int kRed = 8;
+
+// a double's bits mapped so that unsigned order is numeric order
+__host__ __device__ inline unsigned long long ord_bits(double x) {
+    unsigned long long b;
+    memcpy(&b, &x, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double ord_value(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double x;
+    memcpy(&x, &b, sizeof x);
+    return x;
+}
+__device__ __forceinline__ void red_values(unsigned long long (&v)[kRed], double h, double x, double y, double z) {
+    v[0] = dbits(h);
+    v[1] = ~dbits(h);
+    v[2] = ord_bits(x);
+    v[3] = ord_bits(y);
+    v[4] = ord_bits(z);
+    v[5] = ~ord_bits(x);
+    v[6] = ~ord_bits(y);
+    v[7] = ~ord_bits(z);
+}
+
+// margin of an open domain's grid box around the particles' extent [mn, mx]:
+// 1% of the extent, at least 1e-9 of its largest coordinate (and of 1)
+__host__ __device__ inline double box_pad(double mn, double mx) {
+    return ::fmax(0.01 * (mx - mn), 1.0e-9 * ::fmax(::fmax(::fabs(mn), ::fabs(mx)), 1.0));
+}
+
+// cells per R h_max along an axis: min(floor(h_max / h_min), 4) when h_max
+// >= 2 h_min, else 1
+__host__ __device__ inline double cell_split(double hmax, double hmin) {
+    if (hmin > 0.0 && hmax / hmin >= 2.0) return ::fmin(::floor(hmax / hmin), 4.0);
+    return 1.0;
+}
+
 // Grid set-up (one workgroup): one thread per instance, then a serial prefix
 // of cell bases.
 __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
@@ -256,10 +304,34 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         int64_t nk = s.off1 - s.off0;
         int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
         double R = kernel_radius(s.kernel);
-        double c = R * bitsd(ld_dev(&s.hmax_bits));  // reduced by this launch's atomics
+        // h_max, h_min and the box were reduced by this launch's atomics
+        unsigned long long rv[kRed];
+        for (int q = 0; q < kRed; ++q) rv[q] = ld_dev(&s.red[q]);
+        s.hmax_bits = rv[0];
+        const double hm = bitsd(rv[0]);
+        const double hn = rv[1] ? bitsd(~rv[1]) : 0.0;
+        // an open domain's grid spans the particles' box at this evaluation
+        if (!s.periodic && nk > 0) {
+            for (int a = 0; a < s.dim; ++a) {
+                const double mn = ord_value(~rv[5 + a]), mx = ord_value(rv[2 + a]);
+                const double pad = box_pad(mn, mx);
+                s.hi[a] = mx + pad;
+                s.lo[a] = mn - pad;
+                s.L[a] = s.hi[a] - s.lo[a];
+                s.halfL[a] = 0.5 * s.L[a];
+            }
+        }
+        // where h varies by 2x or more the cells are split below R h_max (at
+        // most 4 per axis), so the small-h particles do not scan cells sized
+        // for the largest h
+        double split = cell_split(hm, hn);
+        double c = R * hm / split;
         double lmax = 0.0;
         for (int a = 0; a < s.dim; ++a) lmax = ::fmax(lmax, s.L[a]);
-        if (!(c > 0.0) || !(c <= lmax)) c = lmax;  // h_max of 0, NaN or above the box: one cell
+        if (!(c > 0.0) || !(c <= lmax)) {  // h_max of 0, NaN or above the box: one cell
+            c = lmax;
+            split = 1.0;
+        }
         for (int guard = 0; guard < 80; ++guard) {
             int64_t prod = 1;
             for (int a = 0; a < 3; ++a) {
@@ -274,7 +346,13 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
                 }
             }
             if (prod <= cap) break;
-            c = 2.0 * c;
+            // too many cells: first undo the split, then coarsen
+            if (split > 1.0) {
+                split -= 1.0;
+                c = R * hm / split;
+            } else {
+                c = 2.0 * c;
+            }
         }
         for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
         s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
@@ -292,15 +370,17 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
     }
 }
 
-// Max of the bit patterns b (positive doubles order like their bits) over a
-// block of kBlock threads, per run of equal instances: waves reduce by
-// shuffles, then thread 0 calls emit(k, m, whole) once per run, whole
-// telling that the run is the whole block.  kcur < 0: the thread holds
-// nothing.
-template <class Emit>
-__device__ __forceinline__ void block_max_runs(int kcur, unsigned long long b, Emit emit) {
-    __shared__ unsigned long long wmax[kBlock / kWave];
-    __shared__ int wk[kBlock / kWave];
+// Maxima of the first NR of the kRed patterns v over a block of kBlock
+// threads, per run of equal instances: waves reduce by shuffles, then thread
+// 0 calls emit(k, m, whole) once per run (m: the maxima; entries from NR on
+// are zero), whole telling that the run is the whole block.  kcur < 0: the
+// thread holds nothing (v all zero).  Periodic domains need only h_max and
+// h_min (NR = 2), open ones the box too (NR = kRed).
+template <int NR, class Emit>
+__device__ __forceinline__ void block_max_runs(int kcur, unsigned long long (&v)[kRed], Emit emit) {
+    constexpr int NW = kBlock / kWave;
+    __shared__ unsigned long long wv[NW][kRed];
+    __shared__ int wk[NW];
     __shared__ int wmixed;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
@@ -311,26 +391,31 @@ __device__ __forceinline__ void block_max_runs(int kcur, unsigned long long b, E
     __syncthreads();
     if (uniform) {
         for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long o = __shfl_xor(b, off);
-            b = o > b ? o : b;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                const unsigned long long o = __shfl_xor(v[q], off);
+                v[q] = o > v[q] ? o : v[q];
+            }
         }
     } else {
-        if (kcur >= 0) emit(kcur, b, false);  // rare: a wave straddling instances
+        if (kcur >= 0) emit(kcur, v, false);  // rare: a wave straddling instances
         if (lane == 0) wmixed = 1;
     }
     if (lane == 0) {
-        wmax[wave] = b;
+        for (int q = 0; q < NR; ++q) wv[wave][q] = v[q];
         wk[wave] = uniform ? k0 : -1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 0; q < nw;) {
-            const int k = wk[q];
-            unsigned long long m = wmax[q];
-            int r = q + 1;
-            for (; r < nw && wk[r] == k; ++r) m = wmax[r] > m ? wmax[r] : m;
-            if (k >= 0) emit(k, m, q == 0 && r == nw && !wmixed);
-            q = r;
+        for (int w = 0; w < nw;) {
+            const int k = wk[w];
+            unsigned long long m[kRed] = {};
+            for (int q = 0; q < NR; ++q) m[q] = wv[w][q];
+            int r = w + 1;
+            for (; r < nw && wk[r] == k; ++r)
+                for (int q = 0; q < NR; ++q) m[q] = wv[r][q] > m[q] ? wv[r][q] : m[q];
+            if (k >= 0) emit(k, m, w == 0 && r == nw && !wmixed);
+            w = r;
         }
     }
 }
@@ -368,7 +453,8 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t tiles) {
 // minimum image of each of its candidates when s.range_shift is set.
 template <int DIM, class F>
 __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
-                                                   const int32_t* __restrict__ cend, F&& f) {
+                                                   const int32_t* __restrict__ cend, F&& f,
+                                                   const int* srad = nullptr) {
     const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
     const int64_t local = key - s.cell_base;
     const int cx = (int)(local % n0);
@@ -377,7 +463,7 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
     int lo[3], hi[3];
     const int c[3] = {cx, cy, cz};
     for (int a = 0; a < 3; ++a) {
-        const int na = s.n[a], sr = s.srad[a];
+        const int na = s.n[a], sr = srad ? srad[a] : s.srad[a];
         if (a >= DIM) { lo[a] = 0; hi[a] = 0; }
         else if (s.periodic && 2 * sr + 1 >= na) { lo[a] = 0; hi[a] = na - 1; }
         else if (s.periodic) { lo[a] = c[a] - sr; hi[a] = c[a] + sr; }
@@ -698,6 +784,7 @@ struct Dev {
     int64_t n;
     int32_t ninst;
     int32_t dim, kernel;
+    int32_t periodic;       // every instance's domain is periodic (they agree)
     int64_t cell_cap;
     int32_t nb_stride;      // neighbour-list capacity per particle
     int32_t pass;           // density pass index within the evaluation
@@ -707,7 +794,7 @@ struct Dev {
     Ctl* ctl;
     Ctl* hctl_dev;          // device view of the pinned host mirror of ctl: 3 slots
     int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
-    unsigned long long* hblk;  // per block of the fused stage kernel: its h_max bits (instance in kblk)
+    unsigned long long* hblk;  // per block of the fused stage kernel: its kRed maxima (instance in kblk)
     int32_t* kblk;          // that block's instance, or -1 when it straddles instances
     int32_t blk[3];         // workgroup size of the density, list (prep + build) and force launches
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)

@@ -41,66 +41,51 @@ constexpr int kHmaxBlocks = 128;
 constexpr int kHmaxBatch = 8;
 
 
-__global__ void k_hmax(const double* __restrict__ h, const int32_t* __restrict__ inst_of, Inst* inst,
-                       int64_t n, int next, int64_t chunk, int ninst, Ctl* ctl, int with_grid,
-                       int32_t* arrive) {
+// h_max, h_min and the particles' box per instance (the kRed maxima of
+// red_values), then (the launch's last workgroup) the cell grid
+__global__ void k_hmax(Fields F, const int32_t* __restrict__ inst_of, Inst* inst, int64_t n, int64_t chunk,
+                       int ninst, Ctl* ctl, int32_t* arrive) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
-    __shared__ unsigned long long wmax[kBlock / kWave];
-    __shared__ int wk[kBlock / kWave];
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const double* __restrict__ h = F.f[SPHMHD_F_H];
+    const double* __restrict__ x = F.f[SPHMHD_F_X];
+    const double* __restrict__ y = F.f[SPHMHD_F_Y];
+    const double* __restrict__ z = F.f[SPHMHD_F_Z];
     const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = min(c0 + chunk, n);
     int kcur = -1;
-    unsigned long long b = 0ull;
+    unsigned long long v[kRed] = {};
+    auto flush = [&](int k, const unsigned long long* m) {
+        for (int q = 0; q < kRed; ++q) atomicMax(&inst[k].red[q], m[q]);
+    };
     for (int64_t i0 = c0 + threadIdx.x; i0 < c1; i0 += (int64_t)kHmaxBatch * blockDim.x) {
         int kk[kHmaxBatch];
-        unsigned long long vv[kHmaxBatch];
+        double hh[kHmaxBatch], xx[kHmaxBatch], yy[kHmaxBatch], zz[kHmaxBatch];
 #pragma unroll
         for (int u = 0; u < kHmaxBatch; ++u) {
             const int64_t i = i0 + (int64_t)u * blockDim.x;
             const bool ok = i < c1;
             kk[u] = ok ? inst_of[i] : -1;
-            vv[u] = ok ? dbits(h[i]) : 0ull;
+            hh[u] = ok ? h[i] : 0.0;
+            xx[u] = ok ? x[i] : 0.0;
+            yy[u] = ok ? y[i] : 0.0;
+            zz[u] = ok && z ? z[i] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kHmaxBatch; ++u) {
             const int k = kk[u];
             if (k < 0) continue;
             if (k != kcur) {
-                if (kcur >= 0) atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
+                if (kcur >= 0) flush(kcur, v);
                 kcur = k;
-                b = 0ull;
+                for (int q = 0; q < kRed; ++q) v[q] = 0ull;
             }
-            b = vv[u] > b ? vv[u] : b;
+            unsigned long long w[kRed];
+            red_values(w, hh[u], xx[u], yy[u], zz[u]);
+            for (int q = 0; q < kRed; ++q) v[q] = w[q] > v[q] ? w[q] : v[q];
         }
     }
-    // threads with no element carry kcur = -1 and b = 0
-    int k0 = kcur;
-    for (int off = 32; off > 0; off >>= 1) k0 = max(k0, __shfl_xor(k0, off));
-    const bool uniform = __all(kcur == k0 || kcur < 0);
-    if (uniform) {
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long o = __shfl_xor(b, off);
-            b = o > b ? o : b;
-        }
-    } else if (kcur >= 0) {
-        atomicMax(next ? &inst[kcur].hmax_next_bits : &inst[kcur].hmax_bits, b);
-    }
-    if (lane == 0) {
-        wmax[wave] = b;
-        wk[wave] = uniform ? k0 : -1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < kBlock / kWave;) {
-            const int k = wk[q];
-            unsigned long long m = wmax[q];
-            int r = q + 1;
-            for (; r < kBlock / kWave && wk[r] == k; ++r) m = wmax[r] > m ? wmax[r] : m;
-            if (k >= 0) atomicMax(next ? &inst[k].hmax_next_bits : &inst[k].hmax_bits, m);
-            q = r;
-        }
-    }
-    if (with_grid && last_to_arrive(arrive)) grid_setup(inst, ninst, ctl);
+    // threads with no element carry kcur = -1 and v = 0
+    block_max_runs<kRed>(kcur, v, [&](int k, const unsigned long long* m, bool) { flush(k, m); });
+    if (last_to_arrive(arrive)) grid_setup(inst, ninst, ctl);
 }
 
 __global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) { grid_setup(inst, ninst, ctl); }
@@ -512,8 +497,8 @@ void launch_hmax_grid(Dev& d) {
     }
     const int64_t blocks = std::min<int64_t>(kHmaxBlocks, nblk(d.n));
     const int64_t chunk = (d.n + blocks - 1) / blocks;
-    hipLaunchKernelGGL(k_hmax, dim3((unsigned)blocks), dim3(kBlock), 0, d.stream, d.fields.f[SPHMHD_F_H], d.inst_of,
-                       d.inst, d.n, 0, chunk, d.ninst, d.ctl, 1, d.arrive);
+    hipLaunchKernelGGL(k_hmax, dim3((unsigned)blocks), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.n,
+                       chunk, d.ninst, d.ctl, d.arrive);
 }
 
 constexpr int kBigBlocks = 64;

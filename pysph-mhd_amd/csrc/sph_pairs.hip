@@ -65,6 +65,15 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
         const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
+        // this particle's own stencil: every j with r < R h_i lies within
+        // ceil(R h_i / c_eff) cells of it (at most the pass's stencil, which
+        // is sized for the largest h), so where h varies the small-h
+        // particles scan fewer cells
+        int sr[3] = {0, 0, 0};
+        for (int a = 0; a < DIM; ++a) {
+            const double q = ceil(Rh / s.ceff[a]);
+            sr[a] = q < 1.0 ? 1 : (q < (double)s.srad[a] ? (int)q : s.srad[a]);
+        }
         double sw = 0.0, sg = 0.0;
         // one loop with the ranges' image shifts, one with min_image per
         // candidate (small periodic grids); both give the same r^2 bits
@@ -86,7 +95,7 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
                         sg += o.w * ((double)DIM * w + q * dw);
                     }
                 });
-            });
+            }, sr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});

@@ -131,8 +131,8 @@ __global__ void k_stage2(Fields F, const Inst* __restrict__ inst, const int32_t*
 // expression (so it rounds identically), but every field is loaded once into
 // registers and every output stored once: through the property pointers the
 // compiler cannot tell the fields apart and would reload and re-store them.
-__device__ __forceinline__ double stage2_init_stage1_one(const Fields& F, const Inst* __restrict__ inst,
-                                                         const int32_t* __restrict__ inst_of, int64_t i) {
+__device__ __forceinline__ double4 stage2_init_stage1_one(const Fields& F, const Inst* __restrict__ inst,
+                                                          const int32_t* __restrict__ inst_of, int64_t i) {
     const Inst& s = inst[inst_of[i]];
     const double dt = s.dt;
     const double dtn = next_dt(s, s.t + s.dt, s.dt, bitsd(s.dtp_bits));
@@ -237,7 +237,7 @@ __device__ __forceinline__ double stage2_init_stage1_one(const Fields& F, const 
     FLD(BEVOLYPRED) = byp;
     FLD(BEVOLZPRED) = bzp;
     FLD(PSIPRED) = psip;
-    return h;
+    return make_double4(x, y, z, h);  // the evaluation's positions and h
 }
 
 // t += dt; dt = the pending step clipped to tf (solver.py:57-61)
@@ -247,13 +247,14 @@ __device__ __forceinline__ void advance_clock(Inst& s) {
     s.t = t;
 }
 
-// The predicted h of every particle is reduced to its instance's h_max on the
-// way (the reduction k_hmax would otherwise launch for): each block leaves
-// its maximum in its own slot (hblk, kblk; same-address atomics from every
-// block would serialise), and the last workgroup combines the slots, commits
-// the clock, starts the next evaluation (eval_begin_body with the pending dt
-// cleared) and sets up its cell grid: work that would otherwise be three more
-// launches.
+// The predicted h and positions of every particle are reduced to its
+// instance's h_max, h_min and box on the way (the reduction k_hmax would
+// otherwise launch for): each block leaves its kRed maxima in its own slot
+// (hblk, kblk; same-address atomics from every block would serialise), and
+// the last workgroup combines the slots, commits the clock, starts the next
+// evaluation (eval_begin_body with the pending dt cleared) and sets up its
+// cell grid: work that would otherwise be three more launches.
+template <int NR>
 __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* inst,
                                                                const int32_t* __restrict__ inst_of, int64_t n,
                                                                int ninst, Ctl* ctl, int32_t* arrive,
@@ -261,19 +262,20 @@ __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* i
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int k = -1;
-    unsigned long long hb = 0ull;
+    unsigned long long v[kRed] = {};
     if (i < n) {
-        hb = dbits(stage2_init_stage1_one(F, inst, inst_of, i));
+        const double4 r = stage2_init_stage1_one(F, inst, inst_of, i);
+        red_values(v, r.w, r.x, r.y, r.z);
         k = inst_of[i];
     }
     bool slot = false;
-    block_max_runs(k, hb, [&](int q, unsigned long long m, bool whole) {
+    block_max_runs<NR>(k, v, [&](int q, const unsigned long long* m, bool whole) {
         if (whole) {
-            st_dev(&hblk[blockIdx.x], m);
+            for (int c = 0; c < NR; ++c) st_dev(&hblk[(int64_t)blockIdx.x * kRed + c], m[c]);
             st_dev(&kblk[blockIdx.x], q);
             slot = true;
-        } else {
-            atomicMax(&inst[q].hmax_stage_bits, m);  // a block straddling instances
+        } else {  // a block straddling instances
+            for (int c = 0; c < NR; ++c) atomicMax(&inst[q].red_stage[c], m[c]);
         }
     });
     if (threadIdx.x == 0 && !slot) st_dev(&kblk[blockIdx.x], -1);
@@ -283,29 +285,36 @@ __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* i
         const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
         const int q0 = (int)threadIdx.x * per, q1 = min(nb, q0 + per);
         int kc = -1;
-        unsigned long long mc = 0ull;
+        unsigned long long mc[kRed] = {};
+        auto flush = [&](int q, const unsigned long long* m) {
+            for (int c = 0; c < NR; ++c) atomicMax(&inst[q].red_stage[c], m[c]);
+        };
         for (int q = q0; q < q1; ++q) {
             const int kq = ld_dev(&kblk[q]);
             if (kq < 0) continue;
-            const unsigned long long v = ld_dev(&hblk[q]);
             if (kq != kc) {
-                if (kc >= 0) atomicMax(&inst[kc].hmax_stage_bits, mc);
+                if (kc >= 0) flush(kc, mc);
                 kc = kq;
-                mc = 0ull;
+                for (int c = 0; c < kRed; ++c) mc[c] = 0ull;
             }
-            mc = v > mc ? v : mc;
+            for (int c = 0; c < NR; ++c) {
+                const unsigned long long x = ld_dev(&hblk[(int64_t)q * kRed + c]);
+                mc[c] = x > mc[c] ? x : mc[c];
+            }
         }
-        block_max_runs(kc, mc, [&](int q, unsigned long long m, bool) { atomicMax(&inst[q].hmax_stage_bits, m); });
+        block_max_runs<NR>(kc, mc, [&](int q, const unsigned long long* m, bool) { flush(q, m); });
         __syncthreads();
         for (int q = threadIdx.x; q < ninst; q += blockDim.x) advance_clock(inst[q]);
         __syncthreads();
         eval_begin_body(inst, ninst, ctl, 1);
         __syncthreads();
-        // the reduced h_max, cleared for the next step; the same thread then
+        // the reductions, cleared for the next step; the same thread then
         // sets up that instance's grid
         for (int q = threadIdx.x; q < ninst; q += blockDim.x) {
-            inst[q].hmax_bits = ld_dev(&inst[q].hmax_stage_bits);
-            inst[q].hmax_stage_bits = 0ull;
+            for (int c = 0; c < kRed; ++c) {
+                inst[q].red[c] = ld_dev(&inst[q].red_stage[c]);
+                inst[q].red_stage[c] = 0ull;
+            }
         }
         grid_setup(inst, ninst, ctl);
     }
@@ -422,8 +431,15 @@ void launch_stage2(Dev& d) {
 }
 void launch_stage2_init_stage1(Dev& d) {
     if (d.n)
-        hipLaunchKernelGGL(k_stage2_init_stage1, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
-                           d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
+    {
+        // periodic domains reduce h_max and h_min, open ones their box too
+        if (d.periodic)
+            hipLaunchKernelGGL(k_stage2_init_stage1<2>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
+                               d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
+        else
+            hipLaunchKernelGGL(k_stage2_init_stage1<kRed>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields,
+                               d.inst, d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
+    }
 }
 void launch_clock(Dev& d) {
     hipLaunchKernelGGL(k_clock, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);

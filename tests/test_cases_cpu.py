@@ -230,3 +230,27 @@ def test_open_domain_neighbours_match_brute_force(name):
     for i in range(len(x)):
         assert np.array_equal(np.sort(ic[oc[i]:oc[i + 1]]), np.sort(ib[ob[i]:ob[i + 1]])), i
     assert oc[-1] > 20 * len(x)
+
+
+@pytest.mark.parametrize("name,nx,hbig", [("mhdrotor", 16, None), ("mhdrotor", 16, 0.2), ("mhdshock", 8, None)])
+def test_split_cells_neighbours_match_brute_force(name, nx, hbig):
+    """Where h varies by 2x or more the cells are split below R h_max (the
+    rotor's disk has h 10^(-1/3) of the rest, the shock tube's left half
+    half the right's; one particle with a huge h drives the split to its
+    cap of 4): the cell search still equals the all-pairs search."""
+    from pyoracle import OracleSystem
+
+    s = case_setup(name, nx)
+    f = dict(s.fields)
+    if hbig is not None:
+        f["h"] = f["h"].copy()
+        f["h"][3] = hbig
+    o = OracleSystem(s.params, s.offsets)
+    o.set_fields(f)
+    if s.itype is not None:
+        o.set_fields({"itype": s.itype})
+    oc, ic = o.neighbours()
+    ob, ib = o.neighbours_brute()
+    assert np.array_equal(oc, ob)
+    for i in range(len(f["x"])):
+        assert np.array_equal(np.sort(ic[oc[i]:oc[i + 1]]), np.sort(ib[ob[i]:ob[i + 1]])), i
