@@ -30,7 +30,7 @@ constexpr int kWave = 64;
 enum { LC_DENSITY = 0, LC_LISTS, LC_FORCE, kLaunchClasses };
 constexpr int kScanItems = 4;    // cells per thread of the cell-count scan
 constexpr int kScanCells = 256 * kScanItems;  // cells per scan block
-constexpr int kSmallBucket = 64; // cells up to this size are ordered by rank, one thread per particle
+constexpr int kSmallBucket = 128; // cells up to this size are ordered by rank, one thread per particle
 constexpr int kBigLds = 4096;    // larger cells up to this size are sorted in LDS
 
 // Force-pack record: the FP64 values the force pair loop reads from a

@@ -501,7 +501,7 @@ void launch_hmax_grid(Dev& d) {
                        chunk, d.ninst, d.ctl, d.arrive);
 }
 
-constexpr int kBigBlocks = 64;
+constexpr int kBigBlocks = 256;
 
 void launch_keys_sort_ranges(Dev& d, bool pack) {
     if (d.n == 0) return;

@@ -181,3 +181,20 @@ def test_reference_case_100_steps(gpu, name, nx):
     assert np.linalg.norm(td[1:4] - to[1:4]) <= 1e-8 * pscale
     e_d, e_o = td[4] + td[5] + td[6], to[4] + to[5] + to[6]
     assert abs(e_d - e_o) <= 1e-8 * abs(e_o)
+
+
+@pytest.mark.parametrize("name,nx", [("jadvect", 32), ("mhdvortex", 24), ("mhdrotor", 24)])
+def test_reference_case_cells_and_neighbours_bit_exact(gpu, name, nx):
+    """Crowded cells (jadvect and mhdvortex put 60-200 particles in a cell:
+    rank-ordered and bitonic-sorted buckets) and split cells (the rotor's
+    disk): keys, permutation and neighbour lists bit for bit."""
+    s = case_setup(name, nx)
+    dev, orc = _pair(s)
+    kd, pd = dev.cells()
+    ko, po = orc.cells()
+    assert np.array_equal(kd, ko) and np.array_equal(pd, po)
+    counts = np.bincount(ko.astype(np.int64))
+    assert counts.max() > 64 or name == "mhdrotor"
+    od, idd = dev.neighbours()
+    oo, io = orc.neighbours()
+    assert np.array_equal(od, oo) and np.array_equal(idd, io)
