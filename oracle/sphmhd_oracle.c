@@ -24,7 +24,7 @@
  *             k = min(floor(h_max/h_min), 4) when h_max >= 2 h_min, else 1
  *             (cells split where h varies, so the small-h particles do not
  *             scan cells sized for the largest h); n_a = max(1, floor(L_a/c)),
- *             while prod n_a > max(1, 2 N) the split is undone first and c
+ *             while prod n_a > max(1, 8 N) the split is undone first and c
  *             then doubled; c_eff_a = L_a / n_a over the box: the domain on
  *             periodic axes, else the particles' box at this evaluation
  *             widened by box_pad on each side;
@@ -56,6 +56,9 @@
 #define PAIR_DISTANCE_EPSILON 1.0e-12
 #define INTERACTION_DISTANCE_EPSILON 1.0e-8
 #define FLOAT_TINY 2.2250738585072014e-308
+
+/* cell grid: at most this many cells per particle (max(1, 8 N)) */
+#define kCellsPerParticle 8
 
 /* particles.py:1-129 (itype is held separately as int32). */
 #define ORC_PROPS(X)                                                                        \
@@ -879,7 +882,7 @@ static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
     const sphmhd_params* P = &o->prm[k];
     grid_t* g = &o->grid[k];
     int64_t nk = o->off[k + 1] - o->off[k];
-    int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
+    int64_t cap = kCellsPerParticle * nk > 1 ? kCellsPerParticle * nk : 1;
     double lo[3] = {0.0, 0.0, 0.0}, L[3] = {0.0, 0.0, 0.0};
     for (int a = 0; a < P->dim; ++a) {
         lo[a] = P->lo[a];

@@ -611,7 +611,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         int64_t cap = 0;
         for (int k = 0; k < n_instances; ++k) {
             int64_t nk = offsets[k + 1] - offsets[k];
-            cap += 2 * nk > 1 ? 2 * nk : 1;
+            cap += kCellsPerParticle * nk > 1 ? kCellsPerParticle * nk : 1;
         }
         d.cell_cap = cap;
         d.nb_stride = p0.dim == 2 ? 64 : 96;

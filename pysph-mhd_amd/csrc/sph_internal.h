@@ -30,6 +30,7 @@ constexpr int kWave = 64;
 enum { LC_DENSITY = 0, LC_LISTS, LC_FORCE, kLaunchClasses };
 constexpr int kScanItems = 4;    // cells per thread of the cell-count scan
 constexpr int kScanCells = 256 * kScanItems;  // cells per scan block
+constexpr int kCellsPerParticle = 8;  // a grid has at most this many cells per particle (max(1, 8 N))
 constexpr int kSmallBucket = 128; // cells up to this size are ordered by rank, one thread per particle
 constexpr int kBigLds = 4096;    // larger cells up to this size are sorted in LDS
 
@@ -302,7 +303,7 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
     for (; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
         int64_t nk = s.off1 - s.off0;
-        int64_t cap = 2 * nk > 1 ? 2 * nk : 1;
+        int64_t cap = kCellsPerParticle * nk > 1 ? kCellsPerParticle * nk : 1;
         double R = kernel_radius(s.kernel);
         // h_max, h_min and the box were reduced by this launch's atomics
         unsigned long long rv[kRed];
