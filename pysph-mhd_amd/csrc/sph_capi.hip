@@ -87,8 +87,8 @@ struct sphmhd_ctx {
     // Host mirror of the control block (pinned, mapped): slots 0 and 1 are
     // written by the kernel that ends an evaluation (slot seq % 2), slot 2 by
     // synchronous reads; hcur is the record the host logic reads.
-    Ctl* hctl = nullptr;
-    Ctl* hcur = nullptr;
+    CtlView* hctl = nullptr;
+    CtlView* hcur = nullptr;
     int32_t seq_next = 0;  // seq the next enqueued evaluation end will carry
     hipEvent_t sev = nullptr;  // end of the step being waited for (pipelined runs)
     sphmhd_stats stats{};
@@ -106,6 +106,7 @@ struct sphmhd_ctx {
     };
     std::vector<StepGraph> graphs;
     uint64_t graph_gen = 1;
+    int prune_mode = -1;  // list kernel: -1 by the last evaluation's grids, 0 plain, 1 pruned
     bool use_graphs = true;
     bool capturing = false;
     // Multi-step runs go on a stream of the library's own (a capture cannot
@@ -233,12 +234,25 @@ static void tune_end_step(sphmhd_ctx* c) {
     c->graph_gen += 1;  // captured steps hold the old shapes
 }
 
+// The list kernel follows the last evaluation's grids: the pruned one where
+// some grid was split (both give the same lists; the pruning state costs the
+// unsplit case occupancy).  Captured steps hold the kernel, so a change
+// re-captures them once the queue has drained.
+static int sync_prune(sphmhd_ctx* c) {
+    const bool p = c->prune_mode >= 0 ? c->prune_mode != 0 : c->hcur->any_split != 0;
+    if (p == c->d.prune_lists) return 0;
+    HIPCHK(hipStreamSynchronize(c->d.stream));
+    c->d.prune_lists = p;
+    c->graph_gen += 1;
+    return 0;
+}
+
 static int read_ctl(sphmhd_ctx* c) {
     launch_ctl_export(c->d);
     c->hcur = &c->hctl[2];
     HIPCHK(hipStreamSynchronize(c->d.stream));
     tcollect(c);
-    return 0;
+    return sync_prune(c);
 }
 
 static int pull_inst(sphmhd_ctx* c) {
@@ -271,7 +285,7 @@ static int grow_nb(sphmhd_ctx* c, int need) {
 // rebuilds when a particle has more neighbours than slots
 static int build_lists(sphmhd_ctx* c) {
     for (int attempt = 0; attempt < 2; ++attempt) {
-        launch_nb_build(c->d);
+        launch_nb_build(c->d, true);
         int rc = read_ctl(c);
         if (rc) return rc;
         if (c->hcur->max_nb <= c->d.nb_stride) return 0;
@@ -302,7 +316,7 @@ static int force_pass(sphmhd_ctx* c, bool first) {
     Dev& d = c->d;
     tbegin(c, T_LISTS);
     launch_prep_force(d, first);
-    launch_nb_build(d);
+    launch_nb_build(d, true);
     tend(c, T_LISTS);
     tbegin(c, T_FORCE);
     launch_force(d);  // ends with the evaluation's end (warning bits, host mirror)
@@ -352,7 +366,7 @@ static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
         break;
     case P_LISTS:
         launch_prep_force(d, true);
-        launch_nb_build(d);
+        launch_nb_build(d, true);
         break;
     default:
         launch_force(d);  // ends with the evaluation's end (warning bits, host mirror)
@@ -386,7 +400,7 @@ static int eval_front(sphmhd_ctx* c, int spec) {
 static int pick_record(sphmhd_ctx* c, int32_t sq) {
     c->hcur = &c->hctl[sq & 1];
     if (c->hcur->seq != sq) return fail(SPHMHD_ESTATE, "control record out of sequence");
-    return 0;
+    return sync_prune(c);
 }
 
 // After the evaluation's record is in c->hcur: the rest of the density
@@ -606,6 +620,9 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if (const char* g = std::getenv("SPHMHD_GRAPHS")) c->use_graphs = std::strcmp(g, "0") != 0;
         for (int q = 0; q < kLaunchClasses; ++q) d.blk[q] = kBlock;
         if (const char* g = std::getenv("SPHMHD_SPEC")) c->spec_fixed = std::atoi(g);
+        // SPHMHD_PRUNE_LISTS=1 / 0: always / never the pruned list kernel
+        if (const char* g = std::getenv("SPHMHD_PRUNE_LISTS")) c->prune_mode = std::strcmp(g, "0") != 0;
+        d.prune_lists = c->prune_mode == 1;
         if (const char* g = std::getenv("SPHMHD_TUNE"))
             if (std::strcmp(g, "0") == 0) c->tune.state = 3;
         int64_t cap = 0;
@@ -681,6 +698,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.bigc, n / (kSmallBucket + 1) + 1))) return cleanup(rc);
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.hcell, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.fpack, (int64_t)kFRecMax * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nb, (int64_t)d.nb_stride * n))) return cleanup(rc);
@@ -692,11 +710,11 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
         // the control block's host mirror: pinned, mapped and coherent, so the
         // kernel that ends an evaluation writes it directly (no copy)
-        hipError_t e = hipHostMalloc((void**)&c->hctl, 3 * sizeof(Ctl), hipHostMallocMapped | hipHostMallocCoherent);
+        hipError_t e = hipHostMalloc((void**)&c->hctl, 3 * sizeof(CtlView), hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostMalloc failed"));
         e = hipHostGetDevicePointer((void**)&d.hctl_dev, c->hctl, 0);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, "hipHostGetDevicePointer failed"));
-        std::memset(c->hctl, 0, 3 * sizeof(Ctl));
+        std::memset(c->hctl, 0, 3 * sizeof(CtlView));
         c->hcur = &c->hctl[2];
         e = hipMemcpy(d.inst, c->hinst.data(), sizeof(Inst) * n_instances, hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(e)));
@@ -727,7 +745,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (c->gev) (void)hipEventDestroy(c->gev);
     if (c->sev) (void)hipEventDestroy(c->sev);
     void* ptrs[] = {d.inst, d.ctl, d.arrive, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
-                    d.cstart, d.cend, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
+                    d.cstart, d.cend, d.hcell, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -984,6 +1002,7 @@ int sphmhd_neighbours(sphmhd_ctx* c, int64_t* offs, int32_t* idx, int64_t cap, i
     if (!offs || !total || (cap > 0 && !idx)) return fail(SPHMHD_EINVAL, "null argument");
     try {
         if ((rc = nnps_rebuild(c, false))) return rc;
+        if ((rc = read_ctl(c))) return rc;  // this grid picks the list kernel
         launch_pack_xh(c->d);
         if ((rc = build_lists(c))) return rc;
         const int64_t n = c->d.n;

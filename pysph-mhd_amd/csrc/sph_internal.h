@@ -59,12 +59,20 @@ struct FRec {
 static_assert(FRec<2>::N == 18 && FRec<3>::N == 20, "force record sizes");
 constexpr int kFRecMax = 20;  // doubles allocated per particle
 
+// One instance's parameters, grid and clock, then the targets of its
+// reductions.  Fields that kernels update with device-scope atomics while
+// other workgroups of the same launch read the instance sit on 128-byte lines
+// of their own, one line per launch family: an atomic on a line that also
+// holds fields every workgroup loads made those loads miss the cache (the
+// force pass ran 20-25% slower with dt_adapt's target beside the fields its
+// epilogue reads).
 struct Inst {
     // static parameters (filled at create)
     int64_t off0, off1;
     int32_t dim, kernel, density_iterate, min_iter;
     int32_t max_iter, periodic, adaptive;
     int32_t range_shift;  // the stencil's per-range image shifts equal the minimum image (see stencil_from_hmax)
+    int32_t split;        // cells per R h_max along an axis (grid_setup); above 1 the lists prune by cell h
     double lo[3], hi[3], L[3], halfL[3];
     double gamma, gamma1, mu0, hfact;
     double av, ac, ar, chf;
@@ -77,38 +85,78 @@ struct Inst {
     int64_t cell_base, ncell;
     // clock
     double t, dt;
-    // reduction targets (bit patterns of positive doubles order like integers)
+    // density-iteration state (the group control's; read by the passes)
     unsigned long long hmax_bits;       // max h for the current pass
-    unsigned long long hmax_next_bits;  // max h of still-unconverged particles after the running pass
-    unsigned long long hmax_conv_bits;  // max h of converged particles (frozen for the evaluation)
-    unsigned long long dtp_bits;        // min dt_adapt of the last force pass
-    int32_t iters, unconv, done, status;
+    int32_t iters, done;
+    // reduction targets (bit patterns of positive doubles order like integers)
+    // force pass: min dt_adapt and the warning bits (1: density hit
+    // max_iterations, 2: NaN)
+    alignas(128) unsigned long long dtp_bits;  // min dt_adapt of the last force pass
+    int32_t status;
+    // density passes
+    alignas(128) unsigned long long hmax_next_bits;  // max h of still-unconverged particles after the running pass
+    unsigned long long hmax_conv_bits;               // max h of converged particles (frozen for the evaluation)
+    int32_t unconv;
     // the evaluation's start: h_max, h_min and the particles' box, reduced as
     // maxima of kRed bit patterns (see red_values); red_stage accumulates
     // them in the fused stage kernel (zero between steps)
-    unsigned long long red[8];
+    alignas(128) unsigned long long red[8];
     unsigned long long red_stage[8];
 };
 
+// The evaluation's control block.  The first line holds what every
+// workgroup reads (written between launches or by a launch's last
+// workgroup); each family of device-scope atomics has a 128-byte line of its
+// own, so they do not make those reads miss.
 struct Ctl {
     int64_t total_cells;
     int32_t all_done;
     int32_t status;
-    int32_t max_nb;
     int32_t iters_max;
-    int64_t pairs;
-    int64_t count;
-    int32_t act_cnt[2];  // lengths of the two active-particle lists of the density passes
-    int32_t overflow;    // a neighbour list did not fit its slots
-    int32_t nbig;        // cells with more than kSmallBucket particles in this sort
-    int32_t pad;
+    int32_t any_split;  // some instance's grid has split cells (grid_setup)
     // Steps are enqueued one ahead of the host's wait.  An evaluation that
     // needs more density passes or longer lists sets abort when it ends, and
     // every kernel of the step enqueued after it then exits at entry, leaving
     // the state as that evaluation left it for the host to finish.
     int32_t abort;
     int32_t seq;  // evaluations ended so far (the host mirror's slot is seq % 2)
+    // list build
+    alignas(128) int64_t pairs;
+    int32_t max_nb;
+    int32_t overflow;  // a neighbour list did not fit its slots
+    // lengths of the two active-particle lists of the density passes (a pass
+    // reads one and appends to the other)
+    alignas(128) int32_t act0;
+    alignas(128) int32_t act1;
+    alignas(128) int32_t nbig;   // cells with more than kSmallBucket particles in this sort
+    alignas(128) int64_t count;  // particles counted by k_count
 };
+
+__host__ __device__ inline int32_t* act_count(Ctl* c, int k) { return k ? &c->act1 : &c->act0; }
+
+// the host's copy of the control block (pinned, written by the device)
+struct CtlView {
+    int64_t total_cells, pairs, count;
+    int32_t all_done, status, max_nb, iters_max, overflow, nbig, any_split, abort, seq, pad;
+};
+
+__device__ inline CtlView ctl_view(const Ctl& c) {
+    CtlView v;
+    v.total_cells = c.total_cells;
+    v.pairs = c.pairs;
+    v.count = c.count;
+    v.all_done = c.all_done;
+    v.status = c.status;
+    v.max_nb = c.max_nb;
+    v.iters_max = c.iters_max;
+    v.overflow = c.overflow;
+    v.nbig = c.nbig;
+    v.any_split = c.any_split;
+    v.abort = c.abort;
+    v.seq = c.seq;
+    v.pad = 0;
+    return v;
+}
 
 struct Fields {
     double* f[SPHMHD_NFIELDS];
@@ -213,8 +261,8 @@ __device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int rese
         ctl->iters_max = 0;
         ctl->pairs = 0;
         ctl->count = 0;
-        ctl->act_cnt[0] = 0;
-        ctl->act_cnt[1] = 0;
+        ctl->act0 = 0;
+        ctl->act1 = 0;
         ctl->overflow = 0;
     }
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
@@ -357,6 +405,7 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         }
         for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
         s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
+        s.split = (int)split;
         stencil_from_hmax(s);
     }
     __syncthreads();
@@ -368,6 +417,9 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         }
         ctl->total_cells = base;
         ctl->nbig = 0;
+        int split = 0;
+        for (int q = 0; q < ninst; ++q) split |= inst[q].split > 1;
+        ctl->any_split = split;
     }
 }
 
@@ -421,6 +473,37 @@ __device__ __forceinline__ void block_max_runs(int kcur, unsigned long long (&v)
     }
 }
 
+__device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool valid) {
+    const uint32_t prev = __shfl_up(key, 1);
+    const int lane = threadIdx.x & (kWave - 1);
+    return __ballot(valid && (lane == 0 || prev != key));
+}
+
+// one past the last lane of the run that starts at head lane `lane`
+__device__ __forceinline__ int run_end(unsigned long long heads, unsigned long long vmask, int lane) {
+    const unsigned long long above = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
+    return above ? __ffsll((long long)above) - 1 : kWave - __clzll(vmask);
+}
+
+// per-cell largest h of cell-ordered particles (lanes of a wave in sorted
+// order, key their cell): each run of equal keys issues one atomicMax of its
+// largest h's bits (positive doubles order like their bits; h that is not
+// positive counts as 0, as fmax(h_i, h) then never exceeds h_i either).
+// Lanes with use == false contribute nothing.
+__device__ __forceinline__ void cell_hmax(unsigned long long* hcell, uint32_t key, double h, bool use) {
+    if (!__ballot(use)) return;
+    const unsigned long long heads = run_heads(key, use);
+    const unsigned long long vmask = __ballot(use);
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long v = use && h > 0.0 ? dbits(h) : 0ull;
+    const int end = use ? run_end(heads, vmask, lane) : lane;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned long long o = __shfl_down(v, off);
+        if (lane + off < end && o > v) v = o;
+    }
+    if (use && ((heads >> lane) & 1ull) && v) atomicMax(&hcell[key], v);
+}
+
 // ---------------------------------------------------------------------------
 // XCD-aware tiles
 // ---------------------------------------------------------------------------
@@ -452,10 +535,107 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t tiles) {
 // periodic boundary; (sx, sy, sz) is the range's image shift (0 or +-L per
 // axis: +L where the stencil wrapped below 0, -L above n), which is the
 // minimum image of each of its candidates when s.range_shift is set.
+// stencil span of the cells c0..c1 of one axis; for c0 == c1 exactly the span
+// for_each_row_range forms
+__device__ __forceinline__ void axis_span(int c0, int c1, int sr, int na, int periodic, int& lo, int& hi) {
+    if (periodic && 2 * sr + 1 + (c1 - c0) >= na) { lo = 0; hi = na - 1; }
+    else if (periodic) { lo = c0 - sr; hi = c1 + sr; }
+    else { lo = max(c0 - sr, 0); hi = min(c1 + sr, na - 1); }
+}
+
+// the x pieces of [lo, hi] (-n0 <= lo <= hi < 2 n0) after periodic wrapping
+// (at most two), with their image shifts: +L for cells wrapped from below 0,
+// -L for cells wrapped from n0 and above
+__device__ __forceinline__ int x_pieces(int lo, int hi, int n0, int* xa, int* xb) {
+    if (hi < 0) { xa[0] = lo + n0; xb[0] = hi + n0; return 1; }
+    if (lo >= n0) { xa[0] = lo - n0; xb[0] = hi - n0; return 1; }
+    if (lo < 0) { xa[0] = lo + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi; return 2; }
+    if (hi >= n0) { xa[0] = lo; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi - n0; return 2; }
+    xa[0] = lo; xb[0] = hi; return 1;
+}
+
+__device__ __forceinline__ void x_shifts(int lo, int hi, int n0, double L, double* sx) {
+    sx[0] = lo < 0 ? L : (lo >= n0 ? -L : 0.0);
+    sx[1] = hi >= n0 ? -L : 0.0;
+}
+
+// Geometric pruning of one particle's stencil.  A cell is passed over when
+// its box, in the frame of the stencil's image shifts, lies farther from the
+// particle than R max(h_i, h_c), h_c the cell's largest h where per-cell
+// values are given (hcell) and otherwise a bound hb on every h.  Such a cell
+// holds no pair of the particle, so every sum and list comes out exactly as
+// from the full scan.  Whole stencil rows are dropped and row ends trimmed,
+// so a row stays one contiguous range.  The first and last cells of a
+// periodic axis can hold particles binned there by clamping (outside the box
+// before the first wrap), so their gap along that axis counts as zero; gaps
+// are shrunk by a margin far above the rounding of the binning and of the
+// distance arithmetic.  Only for grids with range shifts (the unwrapped
+// stencil frame is then the minimum image).
+struct Prune {
+    double x[3], lo[3], c[3], m[3];
+    int ci[3], n[3];
+    int per;
+    double R, hi, hb;
+    const unsigned long long* hcell;
+
+    // gap along axis a between the particle and stencil cell q (unwrapped)
+    __device__ __forceinline__ double gap(int a, int q) const {
+        if (q == ci[a]) return 0.0;
+        if (per) {
+            const int w = q < 0 ? q + n[a] : (q >= n[a] ? q - n[a] : q);
+            if (w == 0 || w == n[a] - 1) return 0.0;
+        }
+        const double g = q > ci[a] ? (lo[a] + q * c[a]) - x[a] : x[a] - (lo[a] + (q + 1) * c[a]);
+        return g > m[a] ? g - m[a] : 0.0;
+    }
+    __device__ __forceinline__ bool far(double g2, double h) const {
+        const double r = R * ::fmax(hi, h);
+        return g2 > r * r * (1.0 + 1.0e-12);
+    }
+    // trims [xl, xh] of the stencil row starting at global cell `row`;
+    // false when nothing is left
+    __device__ __forceinline__ bool trim(double g2yz, int64_t row, int& xl, int& xh) const {
+        if (far(g2yz, hb)) return false;
+        auto skip = [&](int q) {
+            const double g = gap(0, q);
+            if (!hcell) return far(g2yz + g * g, hb);
+            const int w = q < 0 ? q + n[0] : (q >= n[0] ? q - n[0] : q);
+            return far(g2yz + g * g, bitsd(hcell[row + w]));
+        };
+        while (xl <= xh && skip(xl)) ++xl;
+        while (xh >= xl && skip(xh)) --xh;
+        return xl <= xh;
+    }
+};
+
+// hb: the bound on every h for cells without values (0 for a reach of R h_i)
+template <int DIM>
+__device__ __forceinline__ void prune_init(Prune& p, const Inst& s, int64_t key, double x, double y, double z,
+                                           double R, double hi, double hb, const unsigned long long* hcell) {
+    const int64_t local = key - s.cell_base;
+    p.ci[0] = (int)(local % s.n[0]);
+    p.ci[1] = (int)((local / s.n[0]) % s.n[1]);
+    p.ci[2] = DIM == 3 ? (int)(local / ((int64_t)s.n[0] * s.n[1])) : 0;
+    p.x[0] = x;
+    p.x[1] = y;
+    p.x[2] = z;
+    for (int a = 0; a < 3; ++a) {
+        p.lo[a] = s.lo[a];
+        p.c[a] = s.ceff[a];
+        p.n[a] = s.n[a];
+        p.m[a] = 1.0e-9 * s.ceff[a] + 1.0e-12 * (::fabs(s.lo[a]) + ::fabs(s.L[a]));
+    }
+    p.per = s.periodic;
+    p.R = R;
+    p.hi = hi;
+    p.hb = hb;
+    p.hcell = hcell;
+}
+
 template <int DIM, class F>
 __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
                                                    const int32_t* __restrict__ cend, F&& f,
-                                                   const int* srad = nullptr) {
+                                                   const int* srad = nullptr, const Prune* pr = nullptr) {
     const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
     const int64_t local = key - s.cell_base;
     const int cx = (int)(local % n0);
@@ -470,22 +650,31 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
         else if (s.periodic) { lo[a] = c[a] - sr; hi[a] = c[a] + sr; }
         else { lo[a] = max(c[a] - sr, 0); hi[a] = min(c[a] + sr, na - 1); }
     }
-    // x segments (at most two after wrapping)
-    int xa[2], xb[2], nseg;
-    if (lo[0] < 0) { xa[0] = lo[0] + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0]; nseg = 2; }
-    else if (hi[0] >= n0) { xa[0] = lo[0]; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi[0] - n0; nseg = 2; }
-    else { xa[0] = lo[0]; xb[0] = hi[0]; nseg = 1; }
-    double sxg[2] = {0.0, 0.0};
-    if (lo[0] < 0) sxg[0] = s.L[0];
-    else if (hi[0] >= n0) sxg[1] = -s.L[0];
+    // x pieces of the full row; a pruned row recomputes its own
+    int xa0[2], xb0[2];
+    double sx0[2];
+    const int nseg0 = x_pieces(lo[0], hi[0], n0, xa0, xb0);
+    x_shifts(lo[0], hi[0], n0, s.L[0], sx0);
     for (int qz = lo[2]; qz <= hi[2]; ++qz) {
         const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
         const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
+        const double gz = pr && DIM == 3 ? pr->gap(2, qz) : 0.0;
         for (int qy = lo[1]; qy <= hi[1]; ++qy) {
             const int wy = ((qy % n1) + n1) % n1;
             const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
             const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
-            for (int g = 0; g < nseg; ++g) f(cstart[row + xa[g]], cend[row + xb[g]], sxg[g], sy, sz);
+            int xa[2] = {xa0[0], xa0[1]}, xb[2] = {xb0[0], xb0[1]};
+            double sx[2] = {sx0[0], sx0[1]};
+            int nseg = nseg0;
+            if (pr) {
+                int xl = lo[0], xh = hi[0];
+                const double gy = pr->gap(1, qy);
+                if (!pr->trim(gy * gy + gz * gz, row, xl, xh)) continue;
+                nseg = x_pieces(xl, xh, n0, xa, xb);
+                x_shifts(xl, xh, n0, s.L[0], sx);
+            }
+            f(cstart[row + xa[0]], cend[row + xb[0]], sx[0], sy, sz);
+            if (nseg == 2) f(cstart[row + xa[1]], cend[row + xb[1]], sx[1], sy, sz);
         }
     }
 }
@@ -530,21 +719,6 @@ struct StageRuns {
     int a[SLOTS], b[SLOTS], off[SLOTS + 1];
 };
 using StageHdr = StageRuns<kWave>;
-
-// stencil span of the cells c0..c1 of one axis; for c0 == c1 exactly the span
-// for_each_row_range forms
-__device__ __forceinline__ void axis_span(int c0, int c1, int sr, int na, int periodic, int& lo, int& hi) {
-    if (periodic && 2 * sr + 1 + (c1 - c0) >= na) { lo = 0; hi = na - 1; }
-    else if (periodic) { lo = c0 - sr; hi = c1 + sr; }
-    else { lo = max(c0 - sr, 0); hi = min(c1 + sr, na - 1); }
-}
-
-// the x pieces of [lo, hi] after periodic wrapping (at most two)
-__device__ __forceinline__ int x_pieces(int lo, int hi, int n0, int* xa, int* xb) {
-    if (lo < 0) { xa[0] = lo + n0; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi; return 2; }
-    if (hi >= n0) { xa[0] = lo; xb[0] = n0 - 1; xa[1] = 0; xb[1] = hi - n0; return 2; }
-    xa[0] = lo; xb[0] = hi; return 1;
-}
 
 // Plans the runs of the group of particles [t0, t0 + count) (clipped to n).
 // Called by all 64 lanes of one wave; lane k (< SLOTS) computes run k.  The
@@ -635,11 +809,13 @@ __device__ __forceinline__ void stage_fill(const StageRuns<SLOTS>& H, T* stage, 
 
 // for_each_row_range over the staged copy: f(j, rec_j, sx, sy, sz) for every
 // candidate j (with its range's image shift, as for_each_row_range), from LDS
-// where a staged run holds the range
+// where a staged run holds the range (pruned ranges lie inside the full ones,
+// so a run that holds a full range holds its pruned part too)
 template <int DIM, class T, class F>
 __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
                                                    const int32_t* __restrict__ cend, const StageHdr& H,
-                                                   const T* stage, const T* __restrict__ glob, F&& f) {
+                                                   const T* stage, const T* __restrict__ glob, F&& f,
+                                                   const Prune* pr = nullptr) {
     const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
     const int64_t local = key - s.cell_base;
     const int cx = (int)(local % n0);
@@ -649,20 +825,30 @@ __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, c
     axis_span(cx, cx, s.srad[0], n0, s.periodic, lo0, hi0);
     axis_span(cy, cy, s.srad[1], n1, s.periodic, lo1, hi1);
     if (DIM == 3) axis_span(cz, cz, s.srad[2], n2, s.periodic, lo2, hi2);
-    int xa[2], xb[2];
-    const int nseg = x_pieces(lo0, hi0, n0, xa, xb);
-    double sxg[2] = {0.0, 0.0};
-    if (lo0 < 0) sxg[0] = s.L[0];
-    else if (hi0 >= n0) sxg[1] = -s.L[0];
+    int xa0[2], xb0[2];
+    double sx0[2];
+    const int nseg0 = x_pieces(lo0, hi0, n0, xa0, xb0);
+    x_shifts(lo0, hi0, n0, s.L[0], sx0);
     const int64_t seg = local / n0 - H.r0;
     const bool staged = H.on && seg >= 0 && seg < kStageSegs;
     for (int qz = lo2; qz <= hi2; ++qz) {
         const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
         const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
+        const double gz = pr && DIM == 3 ? pr->gap(2, qz) : 0.0;
         for (int qy = lo1; qy <= hi1; ++qy) {
             const int wy = ((qy % n1) + n1) % n1;
             const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
             const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+            int xa[2] = {xa0[0], xa0[1]}, xb[2] = {xb0[0], xb0[1]};
+            double sxg[2] = {sx0[0], sx0[1]};
+            int nseg = nseg0;
+            if (pr) {
+                int xl = lo0, xh = hi0;
+                const double gy = pr->gap(1, qy);
+                if (!pr->trim(gy * gy + gz * gz, row, xl, xh)) continue;
+                nseg = x_pieces(xl, xh, n0, xa, xb);
+                x_shifts(xl, xh, n0, s.L[0], sxg);
+            }
             const int base = (((int)seg * H.nz + (qz - lo2)) * H.ny + (qy - lo1)) * 2;
             for (int g = 0; g < nseg; ++g) {
                 const int a = cstart[row + xa[g]], b = cend[row + xb[g]];
@@ -789,11 +975,12 @@ struct Dev {
     int64_t cell_cap;
     int32_t nb_stride;      // neighbour-list capacity per particle
     int32_t pass;           // density pass index within the evaluation
+    bool prune_lists;       // list builds use the pruned kernel (some grid was split)
     hipStream_t stream;
     // device arrays
     Inst* inst;
     Ctl* ctl;
-    Ctl* hctl_dev;          // device view of the pinned host mirror of ctl: 3 slots
+    CtlView* hctl_dev;      // device view of the pinned host mirror of ctl: 3 slots
     int32_t* arrive;        // kArriveInts arrival counters of last_to_arrive (zero between launches)
     unsigned long long* hblk;  // per block of the fused stage kernel: its kRed maxima (instance in kblk)
     int32_t* kblk;          // that block's instance, or -1 when it straddles instances
@@ -805,6 +992,7 @@ struct Dev {
     int32_t* cblock;        // per scan block of kScanCells cells: exclusive offset
     int32_t* bigc;          // cells with more than kSmallBucket particles
     int32_t *cstart, *cend;
+    unsigned long long* hcell;  // per cell (cell_cap): bits of its largest h, for split grids
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build
     int32_t* act[2];        // active-particle lists of the density passes
@@ -868,7 +1056,7 @@ void launch_g0_control(Dev& d);
 // keep_prev: the prep reads au, av, aw from the fields and keeps a copy in
 // aprev; otherwise (a redone force pass) it reads them back from aprev
 void launch_prep_force(Dev& d, bool keep_prev);
-void launch_nb_build(Dev& d);
+void launch_nb_build(Dev& d, bool cell_h = false);
 void launch_force(Dev& d);
 // sets the message sphmhd_last_error() returns; returns code
 int report_error(int code, const std::string& msg);

@@ -105,18 +105,6 @@ __global__ void k_grid(Inst* inst, int ninst, Ctl* ctl) { grid_setup(inst, ninst
 // Particles are visited in the previous sort's order (perm holds it; the
 // identity before the first sort), where the particles of one cell are
 // mostly adjacent: each run of equal keys in a wave does one atomic.
-__device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool valid) {
-    const uint32_t prev = __shfl_up(key, 1);
-    const int lane = threadIdx.x & (kWave - 1);
-    return __ballot(valid && (lane == 0 || prev != key));
-}
-
-// one past the last lane of the run that starts at head lane `lane`
-__device__ __forceinline__ int run_end(unsigned long long heads, unsigned long long vmask, int lane) {
-    const unsigned long long above = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
-    return above ? __ffsll((long long)above) - 1 : kWave - __clzll(vmask);
-}
-
 __global__ void k_cell_count(Fields F, const int32_t* __restrict__ inst_of, const Inst* __restrict__ inst,
                              const int32_t* __restrict__ prev_perm, uint32_t* __restrict__ keys,
                              int32_t* __restrict__ ccount, int64_t n, const Ctl* ctl) {
@@ -238,10 +226,12 @@ __global__ void k_cell_place(const int32_t* __restrict__ prev_perm, const uint32
 
 // final cell table; cells above kSmallBucket particles are listed for k_cell_big
 __global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
-                              const int32_t* __restrict__ cblock, int32_t* __restrict__ bigc, Ctl* ctl, int64_t cap) {
+                              const int32_t* __restrict__ cblock, int32_t* __restrict__ bigc,
+                              unsigned long long* __restrict__ hcell, Ctl* ctl, int64_t cap) {
     if (ctl->abort) return;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cap || c >= ctl->total_cells) return;
+    hcell[c] = 0ull;  // filled from the final h before the lists are built
     const int32_t off = cblock[c / kScanCells];
     const int32_t s = cstart[c] + off, e = cend[c] + off;
     cstart[c] = s;
@@ -358,6 +348,18 @@ __global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double4* _
     xh[t] = r;
 }
 
+// split grids: each cell's largest h from the cell-ordered {x,y,z,h} pack,
+// which the pruned list build passes cells over by (hcell was cleared by the
+// sort's k_cell_finish)
+__global__ void k_cell_h(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
+                         const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
+                         unsigned long long* __restrict__ hcell, int64_t n, const Ctl* ctl) {
+    if (ctl->abort) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool use = t < n && inst[inst_of[t]].split > 1;
+    cell_hmax(hcell, use ? skey[t] : 0xFFFFFFFFu, use ? xh[t].w : 0.0, use);
+}
+
 // before re-running a force pass: forget the partial pass's dt minimum and
 // warning bits (density warnings are kept)
 __global__ void k_reset_force_outputs(Inst* inst, int ninst, Ctl* ctl) {
@@ -381,12 +383,13 @@ __global__ void k_reset_nb_stats(Ctl* ctl) {
 // (measured faster than per-thread rows, whose lines do not stay in L1)
 // ---------------------------------------------------------------------------
 
-template <int DIM>
+template <int DIM, bool PRUNE>
 __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
                                                      const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
-                                                     int stride, Ctl* ctl, int cap) {
+                                                     int stride, Ctl* ctl, int cap,
+                                                     const unsigned long long* __restrict__ hcell) {
     if (ctl->abort) return;
     // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
     // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
@@ -418,6 +421,14 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         const int per = s.periodic;
         // one loop with the ranges' image shifts, one with min_image per
         // candidate (small periodic grids); both give the same r^2 bits
+        // PRUNE (chosen where some grid is split: the pruning state costs
+        // occupancy): with range shifts the stencil is pruned by cell against
+        // h_max, or on split grids against each cell's largest h (hcell, when
+        // the caller filled it from the current h)
+        Prune pr;
+        if (PRUNE && s.range_shift)
+            prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, bitsd(s.hmax_bits),
+                            hcell && s.split > 1 ? hcell : nullptr);
         auto scan = [&](auto shift_c) {
             constexpr bool kShift = decltype(shift_c)::value;
             for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh,
@@ -431,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                     if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
                     ++cnt;
                 }
-            });
+            }, PRUNE && kShift ? &pr : nullptr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});
@@ -515,7 +526,7 @@ void launch_keys_sort_ranges(Dev& d, bool pack) {
     hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
                        d.cstart, d.cblock, d.perm_alt, d.n, d.ctl);
     hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,
-                       d.bigc, d.ctl, d.cell_cap);
+                       d.bigc, d.hcell, d.ctl, d.cell_cap);
     double4* dp = pack ? (double4*)d.dpack : nullptr;
     hipLaunchKernelGGL(k_cell_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm_alt, d.keys_alt, d.cstart,
                        d.cend, d.perm, d.keys, d.n, d.ctl, d.fields, dp, d.dim);
@@ -529,18 +540,30 @@ void launch_pack_density(Dev& d) {
                        (double4*)d.dpack, d.n, d.dim, d.ctl);
 }
 
-void launch_nb_build(Dev& d) {
-    if (d.n == 0) return;
+template <int DIM, bool PRUNE>
+static void nb_build(Dev& d, const unsigned long long* hc) {
     const int B = d.blk[LC_LISTS];
     const unsigned grid = xcd_grid((d.n + B - 1) / B);
+    // 2-D blocks stage candidate runs in LDS, cap records sized with the block
+    const int cap = DIM == 2 ? kStageCap * B / kBlock : 0;
+    hipLaunchKernelGGL((k_nb_build<DIM, PRUNE>), dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
+                       (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
+                       d.nb_stride, d.ctl, cap, hc);
+}
+
+void launch_nb_build(Dev& d, bool cell_h) {
+    if (d.n == 0) return;
+    const bool per_cell = cell_h && d.prune_lists;
+    if (per_cell)
+        hipLaunchKernelGGL(k_cell_h, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
+                           d.inst, d.inst_of, d.hcell, d.n, d.ctl);
+    const unsigned long long* hc = per_cell ? d.hcell : nullptr;
     if (d.dim == 2) {
-        const int cap = kStageCap * B / kBlock;
-        hipLaunchKernelGGL(k_nb_build<2>, dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
-                           (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
-                           d.nb_stride, d.ctl, cap);
+        if (d.prune_lists) nb_build<2, true>(d, hc);
+        else nb_build<2, false>(d, hc);
     } else {
-        hipLaunchKernelGGL(k_nb_build<3>, dim3(grid), dim3(B), 0, d.stream, (const double4*)d.dpack, d.keys, d.cstart,
-                           d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n, d.nb_stride, d.ctl, 0);
+        if (d.prune_lists) nb_build<3, true>(d, hc);
+        else nb_build<3, false>(d, hc);
     }
 }
 

@@ -210,7 +210,7 @@ __device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) 
     __shared__ int alldone;
     if (threadIdx.x == 0) {
         alldone = 1;
-        ctl->act_cnt[next_list] = 0;
+        *act_count(ctl, next_list) = 0;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
@@ -749,7 +749,7 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
 // end of an evaluation (one workgroup): the instances' warning bits (1:
 // density hit max_iterations, 2: NaN) into ctl->status, then ctl into its
 // host mirror, which the host reads after the stream's wait
-__device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) {
+__device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, CtlView* hout) {
     __shared__ int wst[kBlock / kWave];
     int st = 0;
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) st |= ld_dev(&inst[k].status);  // force pass's atomics
@@ -758,7 +758,7 @@ __device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) 
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < (int)blockDim.x / kWave; ++w) st |= wst[w];
-        Ctl v = *ctl;
+        CtlView v = ctl_view(*ctl);
         v.status |= st;
         ctl->status = v.status;
         // more passes or longer lists needed: the step enqueued next exits
@@ -770,12 +770,12 @@ __device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, Ctl* hout) 
 }
 
 __global__ void __launch_bounds__(kBlock) k_eval_end(const Inst* __restrict__ inst, int ninst, Ctl* ctl,
-                                                     Ctl* hout) {
+                                                     CtlView* hout) {
     eval_end_body(inst, ninst, ctl, hout);
 }
 
 // the control block into the host mirror's third slot (synchronous reads)
-__global__ void k_ctl_export(const Ctl* ctl, Ctl* hout) { hout[2] = *ctl; }
+__global__ void k_ctl_export(const Ctl* ctl, CtlView* hout) { hout[2] = ctl_view(*ctl); }
 
 __global__ void k_clear_abort(Ctl* ctl) { ctl->abort = 0; }
 
@@ -785,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride,
-                                                  int ninst, Ctl* ctl, Ctl* hout, int32_t* arrive) {
+                                                  int ninst, Ctl* ctl, CtlView* hout, int32_t* arrive) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride);
     if (last_to_arrive(arrive)) eval_end_body(inst, ninst, ctl, hout);
@@ -815,9 +815,9 @@ void launch_density(Dev& d) {
     if (d.n == 0) return;
     const int p = d.pass;
     const int32_t* ain = p == 0 ? nullptr : d.act[(p - 1) & 1];
-    const int32_t* cin = p == 0 ? nullptr : &d.ctl->act_cnt[(p - 1) & 1];
+    const int32_t* cin = p == 0 ? nullptr : act_count(d.ctl, (p - 1) & 1);
     int32_t* aout = d.act[p & 1];
-    int32_t* cout = &d.ctl->act_cnt[p & 1];
+    int32_t* cout = act_count(d.ctl, p & 1);
     const int B = d.blk[LC_DENSITY];
 #define CALL_DENS(KK, DD)                                                                                   \
     { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields,   \

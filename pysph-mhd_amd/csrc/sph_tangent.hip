@@ -958,9 +958,9 @@ void launch_t_density(Dev& d, TanDev& g) {
     if (d.n == 0) return;
     const int p = d.pass;
     const int32_t* ain = p == 0 ? nullptr : d.act[(p - 1) & 1];
-    const int32_t* cin = p == 0 ? nullptr : &d.ctl->act_cnt[(p - 1) & 1];
+    const int32_t* cin = p == 0 ? nullptr : act_count(d.ctl, (p - 1) & 1);
     int32_t* aout = d.act[p & 1];
-    int32_t* cout = &d.ctl->act_cnt[p & 1];
+    int32_t* cout = act_count(d.ctl, p & 1);
     const int B = d.blk[LC_DENSITY];
     const int64_t per_block = (int64_t)(B / kWave) * kTPW;
     const unsigned grid = (unsigned)((d.n + per_block - 1) / per_block);

@@ -77,12 +77,18 @@ def test_cells_crowded_bit_exact(gpu, nx, hbig):
     assert np.array_equal(pd, po)
 
 
+@pytest.mark.parametrize("prune", ["auto", "forced"])
 @pytest.mark.parametrize("which", ["ot2d", "ot2d_large", "rotor2d", "blast3d", "ensemble", "ensemble_large"])
-def test_neighbours_bit_exact(gpu, which):
+def test_neighbours_bit_exact(gpu, which, prune, monkeypatch):
     """2-D at 40^2 (LDS-staged blocks spanning whole cell rows), 160^2 and
     the 128^2 rotor (staged single-row blocks), 3-D, and ensembles small and
-    large (blocks straddling instances scan from global memory)."""
+    large (blocks straddling instances scan from global memory); "forced"
+    builds the lists with the pruned kernel (stencil cells beyond R h_max of
+    the particle passed over) on these unsplit grids."""
     from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered, rotor_2d
+
+    if prune == "forced":
+        monkeypatch.setenv("SPHMHD_PRUNE_LISTS", "1")
 
     if which == "ot2d":
         s = _ot(40)
