@@ -346,8 +346,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is a design (see DESIGN.md), "
-                                   "so --gpus must be 1"}))
+        print(json.dumps({"error": "this library runs one GPU per context; sharding a workload over several GPUs "
+                                   "is not implemented, so --gpus must be 1"}))
         sys.exit(2)
     head = run_gpu(args.workload, args.steps, args.warmup)
     extras = {}
