@@ -699,6 +699,8 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.hcell, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.cls, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.cle, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.fpack, (int64_t)kFRecMax * n))) return cleanup(rc);
         if ((rc = dalloc(&d.nb, (int64_t)d.nb_stride * n))) return cleanup(rc);
@@ -745,7 +747,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (c->gev) (void)hipEventDestroy(c->gev);
     if (c->sev) (void)hipEventDestroy(c->sev);
     void* ptrs[] = {d.inst, d.ctl, d.arrive, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
-                    d.cstart, d.cend, d.hcell, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
+                    d.cstart, d.cend, d.hcell, d.cls, d.cle, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -992,6 +992,7 @@ struct Dev {
     int32_t* cblock;        // per scan block of kScanCells cells: exclusive offset
     int32_t* bigc;          // cells with more than kSmallBucket particles
     int32_t *cstart, *cend;
+    int32_t *cls, *cle;     // the scan's block-local cell starts and ends (cell_cap)
     unsigned long long* hcell;  // per cell (cell_cap): bits of its largest h, for split grids
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build

@@ -224,21 +224,6 @@ __global__ void k_cell_place(const int32_t* __restrict__ prev_perm, const uint32
     if (valid) bucket[base + (lane - hl)] = i;
 }
 
-// final cell table; cells above kSmallBucket particles are listed for k_cell_big
-__global__ void k_cell_finish(int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
-                              const int32_t* __restrict__ cblock, int32_t* __restrict__ bigc,
-                              unsigned long long* __restrict__ hcell, Ctl* ctl, int64_t cap) {
-    if (ctl->abort) return;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap || c >= ctl->total_cells) return;
-    hcell[c] = 0ull;  // filled from the final h before the lists are built
-    const int32_t off = cblock[c / kScanCells];
-    const int32_t s = cstart[c] + off, e = cend[c] + off;
-    cstart[c] = s;
-    cend[c] = e;
-    if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)c;
-}
-
 // the density pack {x, y, z, m} of storage index i
 __device__ __forceinline__ double4 density_record(const Fields& F, int32_t i, int dim) {
     double4 r;
@@ -249,19 +234,37 @@ __device__ __forceinline__ double4 density_record(const Fields& F, int32_t i, in
     return r;
 }
 
-// small cells: a particle's place is its rank by storage index in its bucket;
-// with dpack, its density record goes to that place too (k_pack_density's
-// work, without a launch of its own)
-__global__ void k_cell_rank(const int32_t* __restrict__ bucket, const uint32_t* __restrict__ keys,
-                            const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                            int32_t* __restrict__ perm, uint32_t* __restrict__ skey, int64_t n, const Ctl* ctl,
-                            Fields F, double4* __restrict__ dpack, int dim) {
+// One launch for the cell table and the small buckets.  Thread q finishes
+// cell q (its start and end in the sorted order from the scan's block-local
+// prefixes plus its block's offset; cells above kSmallBucket particles are
+// listed for k_cell_big; its largest-h slot cleared) and places the particle
+// in bucket slot q: in a small cell its place is its rank by storage index,
+// and with dpack its density record goes to that place too (k_pack_density's
+// work, without a launch of its own).  Both read the scan's prefixes (cls,
+// cle) and write the final table (cstart, cend), so neither waits on the
+// other.
+__global__ void k_cell_finish_rank(const int32_t* __restrict__ cls, const int32_t* __restrict__ cle,
+                                   const int32_t* __restrict__ cblock, int32_t* __restrict__ cstart,
+                                   int32_t* __restrict__ cend, int32_t* __restrict__ bigc,
+                                   unsigned long long* __restrict__ hcell, const int32_t* __restrict__ bucket,
+                                   const uint32_t* __restrict__ keys, int32_t* __restrict__ perm,
+                                   uint32_t* __restrict__ skey, int64_t n, int64_t cap, Ctl* ctl, Fields F,
+                                   double4* __restrict__ dpack, int dim) {
     if (ctl->abort) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < cap && q < ctl->total_cells) {
+        const int32_t off = cblock[q / kScanCells];
+        const int32_t s = cls[q] + off, e = cle[q] + off;
+        cstart[q] = s;
+        cend[q] = e;
+        hcell[q] = 0ull;  // filled from the final h before pruned lists are built
+        if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)q;
+    }
     if (q >= n) return;
     const int32_t v = bucket[q];
     const uint32_t c = keys[v];
-    const int32_t s = cstart[c], e = cend[c];
+    const int32_t off = cblock[c / kScanCells];
+    const int32_t s = cls[c] + off, e = cle[c] + off;
     if (e - s > kSmallBucket) return;
     int32_t r = 0;
     for (int32_t k = s; k < e; ++k) r += bucket[k] < v ? 1 : 0;
@@ -350,7 +353,7 @@ __global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double4* _
 
 // split grids: each cell's largest h from the cell-ordered {x,y,z,h} pack,
 // which the pruned list build passes cells over by (hcell was cleared by the
-// sort's k_cell_finish)
+// sort's k_cell_finish_rank)
 __global__ void k_cell_h(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
                          const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                          unsigned long long* __restrict__ hcell, int64_t n, const Ctl* ctl) {
@@ -521,15 +524,15 @@ void launch_keys_sort_ranges(Dev& d, bool pack) {
     hipLaunchKernelGGL(k_cell_count, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.perm,
                        d.keys_alt, d.ccount, d.n, d.ctl);
     const unsigned sblocks = (unsigned)((d.cell_cap + kScanCells - 1) / kScanCells);
-    hipLaunchKernelGGL(k_cell_scan, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cstart, d.cend, d.cblock,
+    hipLaunchKernelGGL(k_cell_scan, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cls, d.cle, d.cblock,
                        d.ctl, d.arrive);
     hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
-                       d.cstart, d.cblock, d.perm_alt, d.n, d.ctl);
-    hipLaunchKernelGGL(k_cell_finish, dim3(nblk(d.cell_cap)), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.cblock,
-                       d.bigc, d.hcell, d.ctl, d.cell_cap);
+                       d.cls, d.cblock, d.perm_alt, d.n, d.ctl);
     double4* dp = pack ? (double4*)d.dpack : nullptr;
-    hipLaunchKernelGGL(k_cell_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm_alt, d.keys_alt, d.cstart,
-                       d.cend, d.perm, d.keys, d.n, d.ctl, d.fields, dp, d.dim);
+    const int64_t slots = std::max<int64_t>(d.cell_cap, d.n);
+    hipLaunchKernelGGL(k_cell_finish_rank, dim3(nblk(slots)), dim3(kBlock), 0, d.stream, d.cls, d.cle, d.cblock,
+                       d.cstart, d.cend, d.bigc, d.hcell, d.perm_alt, d.keys_alt, d.perm, d.keys, d.n, d.cell_cap,
+                       d.ctl, d.fields, dp, d.dim);
     hipLaunchKernelGGL(k_cell_big, dim3(kBigBlocks), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.perm_alt, d.perm,
                        d.keys, d.bigc, d.ctl, d.fields, dp, d.dim);
 }
