@@ -479,7 +479,8 @@ __device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool valid
     return __ballot(valid && (lane == 0 || prev != key));
 }
 
-// one past the last lane of the run that starts at head lane `lane`
+// one past the last lane of the run that holds lane `lane` (the next head
+// above it, else one past the last valid lane)
 __device__ __forceinline__ int run_end(unsigned long long heads, unsigned long long vmask, int lane) {
     const unsigned long long above = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
     return above ? __ffsll((long long)above) - 1 : kWave - __clzll(vmask);
