@@ -134,6 +134,13 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
         const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
         const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
         const int per = s.periodic;
+        // this particle's own stencil, as the primal pass: every j with
+        // r < R h_i lies within ceil(R h_i / c_eff) cells of it
+        int sr[3] = {0, 0, 0};
+        for (int a = 0; a < DIM; ++a) {
+            const double q = ceil(Rh / s.ceff[a]);
+            sr[a] = q < 1.0 ? 1 : (q < (double)s.srad[a] ? (int)q : s.srad[a]);
+        }
         D1 sw(0.0), sg(0.0);
         // the value test as the primal pass (range shifts where exact)
         auto scan = [&](auto shift_c) {
@@ -159,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_t_density(Fields F, TFields G, const
                         sg += o.w * ((double)DIM * w1 + q * dw);
                     }
                 }
-            });
+            }, sr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});

@@ -87,3 +87,25 @@ def test_autograd_energy_rollout(gpu):
     e_ref, g_ref = energy_rollout(_alfven, THETA, NAMES, 4)
     assert float(e.detach()) == pytest.approx(e_ref, rel=1e-14)
     assert np.allclose(theta.grad.numpy(), 2.0 * g_ref, rtol=1e-14, atol=0)
+
+
+def test_tangent_values_equal_plain_step_split_grid(gpu):
+    """The reference rotor (3-D, its dense disk splits the cells): the
+    tangent step's values, whose density pass scans per-particle stencils
+    as the plain pass does, are the plain step's to the parity standard."""
+    from helpers import CASE_NX, case_setup, init_fields, physical_scales
+    from sphmhd.device import DeviceSystem
+    from sphmhd.tangent import TangentSystem
+
+    s = case_setup("mhdrotor", CASE_NX["mhdrotor"])
+    p = s.params[0]
+    plain = DeviceSystem(s.params, s.offsets)
+    tan = TangentSystem(s.params, s.offsets)
+    for d in (plain, tan):
+        d.set_fields(init_fields(s))
+    plain.warm_start()
+    tan.warm_start()
+    plain.step(3)
+    tan.step(3)
+    compare(tan, plain, STATE, 1e-10, "tangent values, split grid",
+            scales=physical_scales(plain, p.mu0, p.alpha_max, None))
