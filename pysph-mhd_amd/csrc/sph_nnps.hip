@@ -164,17 +164,18 @@ __device__ void cell_scan_top(int32_t* __restrict__ cblock, const Ctl* ctl) {
     }
 }
 
-// per block of kScanCells cells: local exclusive/inclusive prefix into
-// cstart/cend and the block total into cblock; the last workgroup to finish
-// scans the block totals
+// per chunk of kScanCells cells: local exclusive/inclusive prefix into
+// cstart/cend and the chunk total into cblock; workgroups stride over the
+// chunks of this sort's cells (the launch is sized for the particles, not
+// for the capacity); the last workgroup to finish scans the chunk totals
 __global__ void __launch_bounds__(kBlock) k_cell_scan(const int32_t* __restrict__ ccount,
                                                       int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
                                                       int32_t* __restrict__ cblock, Ctl* ctl, int32_t* arrive) {
     if (ctl->abort) return;
     __shared__ int wsum[kBlock / kWave];
     const int64_t total = ctl->total_cells;
-    const int64_t c0 = (int64_t)blockIdx.x * kScanCells;
-    if (c0 < total) {  // whole workgroup: uniform
+    for (int64_t chunk = blockIdx.x; chunk * kScanCells < total; chunk += gridDim.x) {  // uniform per workgroup
+        const int64_t c0 = chunk * kScanCells;
         const int64_t cb = c0 + (int64_t)threadIdx.x * kScanItems;
         int v[kScanItems];
         int sum = 0;
@@ -193,7 +194,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_scan(const int32_t* __restrict_
             }
             run += v[q];
         }
-        if (threadIdx.x == 0) st_dev(&cblock[blockIdx.x], btot);  // read by the tail
+        if (threadIdx.x == 0) st_dev(&cblock[chunk], btot);  // read by the tail
+        __syncthreads();  // wsum is reused by the next chunk
     }
     if (last_to_arrive(arrive)) cell_scan_top(cblock, ctl);
 }
@@ -235,14 +237,14 @@ __device__ __forceinline__ double4 density_record(const Fields& F, int32_t i, in
 }
 
 // One launch for the cell table and the small buckets.  Thread q finishes
-// cell q (its start and end in the sorted order from the scan's block-local
-// prefixes plus its block's offset; cells above kSmallBucket particles are
-// listed for k_cell_big; its largest-h slot cleared) and places the particle
-// in bucket slot q: in a small cell its place is its rank by storage index,
-// and with dpack its density record goes to that place too (k_pack_density's
-// work, without a launch of its own).  Both read the scan's prefixes (cls,
-// cle) and write the final table (cstart, cend), so neither waits on the
-// other.
+// cells q, q + threads, ... (start and end in the sorted order from the
+// scan's chunk-local prefixes plus the chunk's offset; cells above
+// kSmallBucket particles are listed for k_cell_big; their largest-h slots
+// cleared) and places the particle in bucket slot q: in a small cell its
+// place is its rank by storage index, and with dpack its density record goes
+// to that place too (k_pack_density's work, without a launch of its own).
+// Both read the scan's prefixes (cls, cle) and write the final table
+// (cstart, cend), so neither waits on the other.
 __global__ void k_cell_finish_rank(const int32_t* __restrict__ cls, const int32_t* __restrict__ cle,
                                    const int32_t* __restrict__ cblock, int32_t* __restrict__ cstart,
                                    int32_t* __restrict__ cend, int32_t* __restrict__ bigc,
@@ -252,13 +254,14 @@ __global__ void k_cell_finish_rank(const int32_t* __restrict__ cls, const int32_
                                    double4* __restrict__ dpack, int dim) {
     if (ctl->abort) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < cap && q < ctl->total_cells) {
-        const int32_t off = cblock[q / kScanCells];
-        const int32_t s = cls[q] + off, e = cle[q] + off;
-        cstart[q] = s;
-        cend[q] = e;
-        hcell[q] = 0ull;  // filled from the final h before pruned lists are built
-        if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)q;
+    const int64_t ncell = min(cap, ctl->total_cells);
+    for (int64_t c = q; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t off = cblock[c / kScanCells];
+        const int32_t s = cls[c] + off, e = cle[c] + off;
+        cstart[c] = s;
+        cend[c] = e;
+        hcell[c] = 0ull;  // filled from the final h before pruned lists are built
+        if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)c;
     }
     if (q >= n) return;
     const int32_t v = bucket[q];
@@ -523,14 +526,16 @@ void launch_keys_sort_ranges(Dev& d, bool pack) {
     // previous order on entry, the new one on exit
     hipLaunchKernelGGL(k_cell_count, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst_of, d.inst, d.perm,
                        d.keys_alt, d.ccount, d.n, d.ctl);
-    const unsigned sblocks = (unsigned)((d.cell_cap + kScanCells - 1) / kScanCells);
+    // chunks of kScanCells cells: as many workgroups as the particles would
+    // fill (cells beyond are strided over), at most the capacity's chunks
+    const int64_t chunks = (d.cell_cap + kScanCells - 1) / kScanCells;
+    const unsigned sblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(chunks, (d.n + kScanCells - 1) / kScanCells));
     hipLaunchKernelGGL(k_cell_scan, dim3(sblocks), dim3(kBlock), 0, d.stream, d.ccount, d.cls, d.cle, d.cblock,
                        d.ctl, d.arrive);
     hipLaunchKernelGGL(k_cell_place, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.perm, d.keys_alt, d.ccount,
                        d.cls, d.cblock, d.perm_alt, d.n, d.ctl);
     double4* dp = pack ? (double4*)d.dpack : nullptr;
-    const int64_t slots = std::max<int64_t>(d.cell_cap, d.n);
-    hipLaunchKernelGGL(k_cell_finish_rank, dim3(nblk(slots)), dim3(kBlock), 0, d.stream, d.cls, d.cle, d.cblock,
+    hipLaunchKernelGGL(k_cell_finish_rank, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.cls, d.cle, d.cblock,
                        d.cstart, d.cend, d.bigc, d.hcell, d.perm_alt, d.keys_alt, d.perm, d.keys, d.n, d.cell_cap,
                        d.ctl, d.fields, dp, d.dim);
     hipLaunchKernelGGL(k_cell_big, dim3(kBigBlocks), dim3(kBlock), 0, d.stream, d.cstart, d.cend, d.perm_alt, d.perm,
