@@ -107,6 +107,7 @@ struct sphmhd_ctx {
     std::vector<StepGraph> graphs;
     uint64_t graph_gen = 1;
     int prune_mode = -1;  // list kernel: -1 by the last evaluation's grids, 0 plain, 1 pruned
+    bool stage_began = false;  // the stage launch of this front began the evaluation
     bool use_graphs = true;
     bool capturing = false;
     // Multi-step runs go on a stream of the library's own (a capture cannot
@@ -346,15 +347,14 @@ static int front_phase(sphmhd_ctx* c, int ph, bool first, int spec) {
     Dev& d = c->d;
     switch (ph) {
     case P_STAGE:
-        if (first) {
-            launch_initialize(d);
-            launch_stage1(d);
-        } else {
-            launch_stage2_init_stage1(d);  // also commits the clock and begins the evaluation
-        }
+        // both kernels end by beginning the evaluation and setting up its grid
+        if (first) launch_init_stage1(d);
+        else launch_stage2_init_stage1(d);  // also commits the clock
+        c->stage_began = true;
         break;
     case P_NNPS: {
-        const int rc = nnps_rebuild(c, true, !first, true);
+        const int rc = nnps_rebuild(c, true, !first || c->stage_began, true);
+        c->stage_began = false;
         if (rc) return rc;
         break;
     }

@@ -247,27 +247,16 @@ __device__ __forceinline__ void advance_clock(Inst& s) {
     s.t = t;
 }
 
-// The predicted h and positions of every particle are reduced to its
-// instance's h_max, h_min and box on the way (the reduction k_hmax would
-// otherwise launch for): each block leaves its kRed maxima in its own slot
-// (hblk, kblk; same-address atomics from every block would serialise), and
-// the last workgroup combines the slots, commits the clock, starts the next
-// evaluation (eval_begin_body with the pending dt cleared) and sets up its
-// cell grid: work that would otherwise be three more launches.
+// The end of both stage launches: each block leaves its kRed maxima of
+// red_values in its own slot (hblk, kblk; a block straddling instances uses
+// atomics), and the last workgroup combines the slots, commits the clock
+// (advance: after a step's stage2) or clears the abort flag (the first step
+// of a call, as k_eval_begin), starts the evaluation (eval_begin_body with
+// the pending dt cleared) and sets up its cell grid.
 template <int NR>
-__global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* inst,
-                                                               const int32_t* __restrict__ inst_of, int64_t n,
-                                                               int ninst, Ctl* ctl, int32_t* arrive,
-                                                               unsigned long long* hblk, int32_t* kblk) {
-    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int k = -1;
-    unsigned long long v[kRed] = {};
-    if (i < n) {
-        const double4 r = stage2_init_stage1_one(F, inst, inst_of, i);
-        red_values(v, r.w, r.x, r.y, r.z);
-        k = inst_of[i];
-    }
+__device__ __forceinline__ void stage_reduce_begin(int k, unsigned long long (&v)[kRed], Inst* inst, int ninst,
+                                                   Ctl* ctl, int32_t* arrive, unsigned long long* hblk,
+                                                   int32_t* kblk, bool advance) {
     bool slot = false;
     block_max_runs<NR>(k, v, [&](int q, const unsigned long long* m, bool whole) {
         if (whole) {
@@ -304,7 +293,11 @@ __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* i
         }
         block_max_runs<NR>(kc, mc, [&](int q, const unsigned long long* m, bool) { flush(q, m); });
         __syncthreads();
-        for (int q = threadIdx.x; q < ninst; q += blockDim.x) advance_clock(inst[q]);
+        if (advance) {
+            for (int q = threadIdx.x; q < ninst; q += blockDim.x) advance_clock(inst[q]);
+        } else if (threadIdx.x == 0) {
+            ctl->abort = 0;
+        }
         __syncthreads();
         eval_begin_body(inst, ninst, ctl, 1);
         __syncthreads();
@@ -318,6 +311,53 @@ __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* i
         }
         grid_setup(inst, ninst, ctl);
     }
+}
+
+// The predicted h and positions of every particle are reduced to its
+// instance's h_max, h_min and box on the way (the reduction k_hmax would
+// otherwise launch for): each block leaves its kRed maxima in its own slot
+// (hblk, kblk; same-address atomics from every block would serialise), and
+// the last workgroup combines the slots, commits the clock, starts the next
+// evaluation (eval_begin_body with the pending dt cleared) and sets up its
+// cell grid: work that would otherwise be three more launches.
+template <int NR>
+__global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* inst,
+                                                               const int32_t* __restrict__ inst_of, int64_t n,
+                                                               int ninst, Ctl* ctl, int32_t* arrive,
+                                                               unsigned long long* hblk, int32_t* kblk) {
+    if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k = -1;
+    unsigned long long v[kRed] = {};
+    if (i < n) {
+        const double4 r = stage2_init_stage1_one(F, inst, inst_of, i);
+        red_values(v, r.w, r.x, r.y, r.z);
+        k = inst_of[i];
+    }
+    stage_reduce_begin<NR>(k, v, inst, ninst, ctl, arrive, hblk, kblk, true);
+}
+
+// The first step of a call: initialize and stage1 (k_initialize and
+// k_stage1's bodies, in that order), the h and box reductions of the
+// evaluation that follows, and its start (as k_eval_begin, then k_hmax's
+// tail): three launches fewer than the unfused path.
+template <int NR>
+__global__ void __launch_bounds__(kBlock) k_init_stage1(Fields F, Inst* inst, const int32_t* __restrict__ inst_of,
+                                                        int64_t n, int ninst, Ctl* ctl, int32_t* arrive,
+                                                        unsigned long long* hblk, int32_t* kblk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k = -1;
+    unsigned long long v[kRed] = {};
+    if (i < n) {
+        do_initialize(F, i);
+        const Inst& s = inst[inst_of[i]];
+        do_stage1(F, s, i, s.dt);
+        do_wrap(F, s, i);
+        const double* z = F.f[SPHMHD_F_Z];
+        red_values(v, F.f[SPHMHD_F_H][i], F.f[SPHMHD_F_X][i], F.f[SPHMHD_F_Y][i], z ? z[i] : 0.0);
+        k = inst_of[i];
+    }
+    stage_reduce_begin<NR>(k, v, inst, ninst, ctl, arrive, hblk, kblk, false);
 }
 
 __global__ void k_clock(Inst* inst, int ninst) {
@@ -440,6 +480,15 @@ void launch_stage2_init_stage1(Dev& d) {
             hipLaunchKernelGGL(k_stage2_init_stage1<kRed>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields,
                                d.inst, d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
     }
+}
+void launch_init_stage1(Dev& d) {
+    if (d.n == 0) return;
+    if (d.periodic)
+        hipLaunchKernelGGL(k_init_stage1<2>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.inst_of,
+                           d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
+    else
+        hipLaunchKernelGGL(k_init_stage1<kRed>, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst,
+                           d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
 }
 void launch_clock(Dev& d) {
     hipLaunchKernelGGL(k_clock, dim3(nblk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, d.ninst);
