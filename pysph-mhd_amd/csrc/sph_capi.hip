@@ -890,8 +890,7 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
             if (rc > worst) worst = rc;
             if (tuning) tune_end_step(c);
         }
-        launch_stage2(c->d);
-        launch_clock(c->d);
+        launch_stage2_clock(c->d);  // stage2 of the last step and the clock commit
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->d.stream));
     } catch (...) {

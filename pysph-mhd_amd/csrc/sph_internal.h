@@ -1072,6 +1072,7 @@ void launch_stage1(Dev& d);
 void launch_stage2(Dev& d);
 void launch_stage2_init_stage1(Dev& d);
 void launch_init_stage1(Dev& d);
+void launch_stage2_clock(Dev& d);
 void launch_clock(Dev& d);
 void launch_sync_initial(Dev& d);
 void launch_copy(Dev& d, double* dst, const double* src, int64_t n);

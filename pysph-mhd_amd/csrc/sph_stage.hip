@@ -337,6 +337,20 @@ __global__ void __launch_bounds__(kBlock) k_stage2_init_stage1(Fields F, Inst* i
     stage_reduce_begin<NR>(k, v, inst, ninst, ctl, arrive, hblk, kblk, true);
 }
 
+// The last step of a call: stage2 (k_stage2's body), then the clock commit
+// (k_clock) by the last workgroup, once every particle has read the step
+__global__ void __launch_bounds__(kBlock) k_stage2_clock(Fields F, Inst* inst, const int32_t* __restrict__ inst_of,
+                                                         int64_t n, int ninst, int32_t* arrive) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Inst& s = inst[inst_of[i]];
+        do_stage2(F, s, i, s.dt);
+        do_wrap(F, s, i);
+    }
+    if (last_to_arrive(arrive))
+        for (int q = threadIdx.x; q < ninst; q += blockDim.x) advance_clock(inst[q]);
+}
+
 // The first step of a call: initialize and stage1 (k_initialize and
 // k_stage1's bodies, in that order), the h and box reductions of the
 // evaluation that follows, and its start (as k_eval_begin, then k_hmax's
@@ -481,6 +495,15 @@ void launch_stage2_init_stage1(Dev& d) {
                                d.inst, d.inst_of, d.n, d.ninst, d.ctl, d.arrive, d.hblk, d.kblk);
     }
 }
+void launch_stage2_clock(Dev& d) {
+    if (d.n == 0) {
+        launch_clock(d);
+        return;
+    }
+    hipLaunchKernelGGL(k_stage2_clock, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, d.inst_of, d.n,
+                       d.ninst, d.arrive);
+}
+
 void launch_init_stage1(Dev& d) {
     if (d.n == 0) return;
     if (d.periodic)
