@@ -7,12 +7,13 @@ timed region; the timed region is exactly --steps steps between two stream
 synchronisations, after the warm start (application.py:110-147) and
 --warmup untimed steps.
 
-Headline workload: the 2-D MHD rotor on a 512^2 lattice, FP64 (the
-single-GPU configuration the metric is quoted on).  The JSON line also
-carries the other single-GPU workloads (the 1M-particle Orszag-Tang vortex,
-the 128^3 blast and the 64-instance uneven ensemble), the roofline of the
-dominant kernel (the fused force pass) and the CPU oracle timed on a bounded
-sample of the headline workload on this host.
+Headline workload: the 3-D MHD blast on a 128^3 lattice (2,097,152
+particles, quintic kernel, FP64), the largest single-GPU configuration of
+the benchmark set.  The JSON line also carries the other single-GPU
+workloads (the 512^2 rotor, the 1M-particle Orszag-Tang vortex and the
+64-instance uneven ensemble), the roofline of the dominant kernel (the fused
+force pass) with the density passes' beside it, and the CPU oracle timed on
+a bounded sample of the headline workload on this host.
 
 Prints ONE JSON line.
 """
@@ -53,26 +54,50 @@ TUNE_STEPS = 6
 TRAFFIC_FILE = ROOT / "profiles" / "k_force_hbm_traffic.md"
 
 
-def _flops_per_pair(dim: int) -> float:
-    """Algorithmic FP64 flops of one neighbour entry of the fused force pass,
-    counting add/sub/mul/div/sqrt as 1 and fma as 2, from the operation
-    counts of the reference bodies as fused (kernel gradients for h_i and
-    h_j, MHDAccelerations, MagneticStressAccelerations, MagneticStateRates,
-    Cullen-Dehnen sums)."""
-    return 330.0 if dim == 2 else 390.0
+# FP64 operations the force kernel executes per neighbour entry, counted by
+# rocprofv3 (SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 per
+# dispatch: 64 lanes x (add + mul + trans + 2 fma) / pairs), kept in the
+# profile summaries as "fp64_flops_per_pair[<workload>]: <flops>".
+FLOPS_FILE = ROOT / "profiles" / "k_force_fp64_ops.md"
+
+# Used only for a workload whose counters are not in FLOPS_FILE: the
+# operation count of the fused reference bodies written out by hand
+# (add/sub/mul/div/sqrt 1, fma 2).
+FLOPS_PER_PAIR_ESTIMATE = {2: 330.0, 3: 390.0}
+
+# Density pass: compulsory bytes per particle of one full pass (reads x, y,
+# (z), h, m, h0, converged; writes rho, h, omega, converged) and FP64
+# operations per neighbour entry (distance, kernel value and h-derivative,
+# two sums), add/sub/mul/div/sqrt 1, fma 2.
+DENSITY_BYTES_PER_PARTICLE = {2: 80.0, 3: 88.0}
+DENSITY_FLOPS_PER_PAIR = 30.0
+
+
+def _profile_value(path: Path, key: str):
+    """The number after "<key>:" on a line of a committed profile summary,
+    or None."""
+    try:
+        for line in path.read_text().splitlines():
+            if line.strip().startswith(key + ":"):
+                return float(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(workload: str):
     """HBM bytes per force launch for this workload from the committed
     profile summary, or None when it has not been profiled."""
-    try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
-            key = f"traffic_bytes_per_launch[{workload}]:"
-            if line.strip().startswith(key):
-                return float(line.split(":", 1)[1].strip())
-    except OSError:
-        pass
-    return None
+    return _profile_value(TRAFFIC_FILE, f"traffic_bytes_per_launch[{workload}]")
+
+
+def flops_per_pair(workload: str, dim: int):
+    """(FP64 flops per neighbour entry of the force pass, how it was
+    obtained): the counted value of the profile summary, else the estimate."""
+    v = _profile_value(FLOPS_FILE, f"fp64_flops_per_pair[{workload}]")
+    if v is not None:
+        return v, "counted (rocprofv3 FP64 VALU instruction counters)"
+    return FLOPS_PER_PAIR_ESTIMATE[dim], "estimate (hand count of the reference bodies)"
 
 
 def parse():
@@ -80,11 +105,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rotor512",
+    ap.add_argument("--workload", default="blast128",
                     choices=["rotor512", "ot1m", "blast128", "ensemble64", "ot128", "ot128x64"])
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 512 MiB FP64 buffer 5 times (scales HBM byte counters)")
     return ap.parse_args()
@@ -227,10 +252,11 @@ def run_tangent(steps: int, warmup: int = 2):
 
 def roofline(res):
     """Roofline of the dominant kernel (the fused force pass).  Its binding
-    roof is the FP64 rate: per launch it does flops_per_pair x pairs of FP64
-    work on bytes_alg of compulsory HBM traffic, about 24 flop/B against a
-    ridge of 9.8 flop/B.  The HBM fraction of the same launch is reported
-    beside it."""
+    roof is the FP64 vector rate (no step is a dense contraction, so no MFMA
+    runs): per launch it does flops_per_pair x pairs of FP64 work on
+    bytes_alg of compulsory HBM traffic, about 20-30 flop/B against a ridge
+    of 9.8 flop/B.  The HBM fraction of the same launch is reported beside
+    it, and the density passes' figures after it."""
     tm = res["timing"]
     n = res["particles"]
     dim = res["dim"]
@@ -238,14 +264,32 @@ def roofline(res):
     t_force = tm["force_ms"] / launches * 1e-3
     bytes_alg = FORCE_BYTES_PER_PARTICLE[dim] * n
     pairs = res["pairs_timing"]
-    flops = _flops_per_pair(dim) * pairs
+    fpp, fpp_how = flops_per_pair(res["workload"], dim)
+    flops = fpp * pairs
     gbs = bytes_alg / t_force / 1e9
     tflops = flops / t_force / 1e12
     t_attain = max(bytes_alg / (HBM_PEAK_GBS * 1e9), flops / (FP64_PEAK_TFLOPS * 1e12))
+    # density passes: per step, one full pass's compulsory bytes and flops
+    # (later passes only revisit unconverged particles)
+    steps = max(1, res["timing_steps"])
+    t_dens = tm["density_ms"] / steps * 1e-3
+    dens = None
+    if t_dens > 0:
+        dbytes = DENSITY_BYTES_PER_PARTICLE[dim] * n
+        dflops = DENSITY_FLOPS_PER_PAIR * pairs
+        dens = {
+            "kernel": "k_density (G0 summation density + Newton step for h), all passes of a step",
+            "ms_per_step": round(t_dens * 1e3, 4),
+            "bytes_alg_per_step": dbytes,
+            "flops_alg_per_step": dflops,
+            "hbm_GBps": round(dbytes / t_dens / 1e9, 2),
+            "hbm_frac": round(dbytes / t_dens / 1e9 / HBM_PEAK_GBS, 5),
+            "fp64_TFLOPs": round(dflops / t_dens / 1e12, 3),
+            "fp64_frac": round(dflops / t_dens / 1e12 / FP64_PEAK_TFLOPS, 4),
+        }
     return {
-        "bound": "mfma",
-        "bound_note": "FP64 compute: the pass runs on the vector ALUs (no contraction, no MFMA); "
-                      "MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
+        "bound": "fp64",
+        "bound_note": "FP64 vector ALU rate (78.6 TFLOP/s on MI355X); no MFMA: no step is a dense contraction",
         "kernel": "k_force (fused G3+G5+G7+G8 pair loop, G3/G5/G6/G8/G9 epilogue)",
         "achieved": round(tflops, 3),
         "peak": FP64_PEAK_TFLOPS,
@@ -254,7 +298,8 @@ def roofline(res):
         "traffic": measured_traffic(res["workload"]),
         "avg_launch_ms": round(tm["force_ms"] / launches, 5),
         "launches": tm["force_launches"],
-        "flops_per_pair": _flops_per_pair(dim),
+        "flops_per_pair": fpp,
+        "flops_per_pair_source": fpp_how,
         "pairs_per_launch": pairs,
         "bytes_alg_per_launch": bytes_alg,
         "hbm": {
@@ -264,6 +309,7 @@ def roofline(res):
             "frac": round(gbs / HBM_PEAK_GBS, 5),
         },
         "attainable_frac": round(t_attain / t_force, 4),
+        "density": dens,
     }
 
 
@@ -352,7 +398,8 @@ def main():
     head = run_gpu(args.workload, args.steps, args.warmup)
     extras = {}
     if not args.no_extras:
-        for name, steps in (("ot1m", 20), ("blast128", 5), ("ensemble64", 20), ("ot128", 50), ("ot128x64", 20)):
+        for name, steps in (("rotor512", 20), ("ot1m", 20), ("blast128", 10), ("ensemble64", 20), ("ot128", 50),
+                            ("ot128x64", 20)):
             if name == args.workload:
                 continue
             r = run_gpu(name, steps, 2)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 5
+#define SPHMHD_ABI_VERSION 6
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -134,6 +134,7 @@ typedef struct sphmhd_stats {
     int32_t block_lists;         /* workgroup size of the prep + neighbour-list launches */
     int32_t block_force;         /* workgroup size of the force pass */
     int32_t launch_tuned;        /* 1 once the launch shapes were chosen by timing */
+    int64_t steps;               /* PEC steps completed by sphmhd_step / sphmhd_tangent_step */
 } sphmhd_stats;
 
 /* Device time of the hot kernels, measured with HIP events recorded on the
@@ -188,8 +189,11 @@ int sphmhd_pec_stage2(sphmhd_ctx* ctx);
  * clipped to tf - t).  Replaces solver.py:57-61. */
 int sphmhd_advance_clock(sphmhd_ctx* ctx);
 
-/* nsteps full PEC steps with the clock advanced after each: the body of
- * MHDSolver.solve (solver.py:42-73) without dumps.  Kernels are fused
+/* Up to nsteps full PEC steps with the clock advanced after each: the body of
+ * MHDSolver.solve (solver.py:42-73) without dumps.  The call stops early,
+ * after the step that brings every instance's clock to its tf (t + dt >= tf;
+ * dt is clipped to tf - t), so a run bounded by tf never takes the zero-length
+ * steps that would follow; stats.steps counts the steps taken.  Kernels are fused
  * (stage2 of step n with initialize+stage1 of step n+1); the fields at
  * return are exactly those after the last stage2.  Steps after the first
  * replay a captured hipGraph and are enqueued one ahead of the host's wait

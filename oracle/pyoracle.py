@@ -42,6 +42,8 @@ def lib():
                      "orc_sync_initial_state", "orc_warm_start"):
             getattr(L, name).argtypes = [vp]
         L.orc_step.argtypes = [vp, i64]
+        L.orc_steps_taken.restype = i64
+        L.orc_steps_taken.argtypes = [vp]
         L.orc_get_clock.argtypes = [vp, dp, dp]
         L.orc_set_clock.argtypes = [vp, dp, dp]
         L.orc_get_pending_dt.argtypes = [vp, dp]
@@ -147,6 +149,9 @@ class OracleSystem:
 
     def step(self, nsteps=1):
         return self.L.orc_step(self.h, int(nsteps))
+
+    def steps_taken(self) -> int:
+        return int(self.L.orc_steps_taken(self.h))
 
     def sync_initial_state(self):
         self.L.orc_sync_initial_state(self.h)

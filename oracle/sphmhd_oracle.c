@@ -102,6 +102,7 @@ struct orc {
     int32_t* itype;
     double *t, *dt, *dt_pending;
     int32_t* last_iters;
+    int64_t steps;    /* PEC steps taken by orc_step */
     grid_t* grid;
     uint32_t* key;    /* per sorted position (global key) */
     int32_t* perm;    /* sorted position -> storage index */
@@ -1248,6 +1249,9 @@ int orc_advance_clock(orc_t* o) {
     return 0;
 }
 
+/* Up to nsteps PEC steps; the run ends after the step that brings every
+ * instance's clock to its tf (t + dt >= tf), so a tf-bounded run takes no
+ * zero-length steps (same contract as sphmhd_step). */
 int orc_step(orc_t* o, int64_t nsteps) {
     int status = 0;
     for (int64_t s = 0; s < nsteps; ++s) {
@@ -1255,11 +1259,18 @@ int orc_step(orc_t* o, int64_t nsteps) {
         orc_stage1(o);
         int st = orc_eval(o);
         if (st > status) status = st;
+        int at_tf = 1;
+        for (int k = 0; k < o->ninst; ++k)
+            if (!(o->t[k] + o->dt[k] >= o->prm[k].tf)) at_tf = 0;
         orc_stage2(o);
         orc_advance_clock(o);
+        o->steps += 1;
+        if (at_tf) break;
     }
     return status;
 }
+
+int64_t orc_steps_taken(orc_t* o) { return o->steps; }
 
 /* application.py:100-108 */
 int orc_sync_initial_state(orc_t* o) {

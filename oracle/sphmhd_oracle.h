@@ -66,6 +66,7 @@ int orc_stage1(orc_t* o);
 int orc_stage2(orc_t* o);
 int orc_advance_clock(orc_t* o);
 int orc_step(orc_t* o, int64_t nsteps);
+int64_t orc_steps_taken(orc_t* o);
 int orc_sync_initial_state(orc_t* o);
 int orc_warm_start(orc_t* o);
 int orc_get_clock(orc_t* o, double* t, double* dt);

@@ -449,6 +449,8 @@ static int eval_process(sphmhd_ctx* c, int spec) {
         redo = false;
         c->stats.force_redos += 1;
     }
+    // a redone pass of the step that reaches tf raised the flag again
+    if (c->hcur == &c->hctl[2] && c->hcur->abort) launch_clear_abort(d);
     c->stats.max_neighbours = c->hcur->max_nb;
     c->stats.pairs_force = c->hcur->pairs;
     c->stats.cells = c->hcur->total_cells;
@@ -889,6 +891,10 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
             if (rc < 0) return rc;
             if (rc > worst) worst = rc;
             if (tuning) tune_end_step(c);
+            c->stats.steps += 1;
+            // this step brings every clock to tf: the run ends with it (the
+            // step enqueued after it, if any, exited at entry)
+            if (c->hcur->at_tf) break;
         }
         launch_stage2_clock(c->d);  // stage2 of the last step and the clock commit
         HIPCHK(hipGetLastError());
@@ -1160,6 +1166,7 @@ static int t_eval(sphmhd_ctx* c) {
         redo = false;
         c->stats.force_redos += 1;
     }
+    if (c->hcur->abort) launch_clear_abort(d);  // a redone pass of the step that reaches tf
     c->stats.max_neighbours = c->hcur->max_nb;
     c->stats.pairs_force = c->hcur->pairs;
     c->stats.cells = c->hcur->total_cells;
@@ -1260,6 +1267,8 @@ int sphmhd_tangent_step(sphmhd_ctx* c, int64_t nsteps) {
         launch_t_stage2(c->d, c->tg);
         launch_t_clock(c->d, c->tg);
         HIPCHK(hipGetLastError());
+        c->stats.steps += 1;
+        if (c->hcur->at_tf) break;  // the clocks reached tf
     }
     HIPCHK(hipStreamSynchronize(c->d.stream));
     return worst;

@@ -120,6 +120,7 @@ struct Ctl {
     // the state as that evaluation left it for the host to finish.
     int32_t abort;
     int32_t seq;  // evaluations ended so far (the host mirror's slot is seq % 2)
+    int32_t at_tf;  // set by an evaluation's end: its step brings every clock to tf
     // list build
     alignas(128) int64_t pairs;
     int32_t max_nb;
@@ -137,7 +138,8 @@ __host__ __device__ inline int32_t* act_count(Ctl* c, int k) { return k ? &c->ac
 // the host's copy of the control block (pinned, written by the device)
 struct CtlView {
     int64_t total_cells, pairs, count;
-    int32_t all_done, status, max_nb, iters_max, overflow, nbig, any_split, abort, seq, pad;
+    int32_t all_done, status, max_nb, iters_max, overflow, nbig, any_split, abort, seq;
+    int32_t at_tf;  // the step of this evaluation brings every instance's clock to its tf
 };
 
 __device__ inline CtlView ctl_view(const Ctl& c) {
@@ -154,7 +156,7 @@ __device__ inline CtlView ctl_view(const Ctl& c) {
     v.any_split = c.any_split;
     v.abort = c.abort;
     v.seq = c.seq;
-    v.pad = 0;
+    v.at_tf = c.at_tf;
     return v;
 }
 
@@ -201,12 +203,16 @@ __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlon
 // tail (group control, grid set-up, a top-level scan) inside the launch
 // instead of as a kernel of its own.
 //
-// Visibility: each workgroup's memory operations have completed before it
-// arrives (a workgroup-scope release waits for them), but plain stores may
-// still sit in its XCD's L2.  A device-scope fence would write back and
-// invalidate the L2 in every workgroup (measured: it doubled the step), so
-// what a tail reads of its own launch's results is written with device
-// atomics and read with ld_dev / st_dev, which are coherent across the XCDs.
+// Visibility: what a tail reads of its own launch's results is written with
+// device-scope atomics or st_dev and read with ld_dev, which are coherent
+// across the XCDs; plain stores may still sit in an XCD's L2 (a device-scope
+// fence would write back and invalidate the L2 in every workgroup: measured,
+// it doubled the step).  Ordering: on gfx950 a workgroup-scope release does
+// not wait for outstanding vector memory operations, so every wave first
+// waits for all of its own (stores and atomics without return count in
+// vmcnt) with s_waitcnt 0; only then does the barrier let thread 0 issue the
+// arrival atomic, so the last workgroup never reads a slot another workgroup
+// has not finished writing.
 //
 // Same-address atomics serialise (about 10 ns each), so a thousand
 // workgroups do not share one counter: they arrive in kArriveGroups groups
@@ -218,6 +224,7 @@ constexpr int kArriveInts = (kArriveGroups + 1) * kArriveStride;
 
 __device__ __forceinline__ bool last_to_arrive(int32_t* arrive) {
     __shared__ int s_last;
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt, expcnt, lgkmcnt all zero
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -264,6 +271,7 @@ __device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int rese
         ctl->act0 = 0;
         ctl->act1 = 0;
         ctl->overflow = 0;
+        ctl->at_tf = 0;
     }
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];

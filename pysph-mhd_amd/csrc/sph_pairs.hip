@@ -747,22 +747,38 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
 }
 
 // end of an evaluation (one workgroup): the instances' warning bits (1:
-// density hit max_iterations, 2: NaN) into ctl->status, then ctl into its
-// host mirror, which the host reads after the stream's wait
+// density hit max_iterations, 2: NaN) into ctl->status, whether the step
+// being evaluated (clock t, dt) ends at or past every instance's tf, then ctl
+// into its host mirror, which the host reads after the stream's wait
 __device__ void eval_end_body(const Inst* inst, int ninst, Ctl* ctl, CtlView* hout) {
-    __shared__ int wst[kBlock / kWave];
-    int st = 0;
-    for (int k = threadIdx.x; k < ninst; k += blockDim.x) st |= ld_dev(&inst[k].status);  // force pass's atomics
-    for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
-    if ((threadIdx.x & (kWave - 1)) == 0) wst[threadIdx.x / kWave] = st;
+    __shared__ int wst[kBlock / kWave], wfin[kBlock / kWave];
+    int st = 0, fin = 1;
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+        st |= ld_dev(&inst[k].status);  // force pass's atomics
+        if (!(inst[k].t + inst[k].dt >= inst[k].tf)) fin = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        st |= __shfl_xor(st, off);
+        fin &= __shfl_xor(fin, off);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        wst[threadIdx.x / kWave] = st;
+        wfin[threadIdx.x / kWave] = fin;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)blockDim.x / kWave; ++w) st |= wst[w];
+        for (int w = 1; w < (int)blockDim.x / kWave; ++w) {
+            st |= wst[w];
+            fin &= wfin[w];
+        }
         CtlView v = ctl_view(*ctl);
         v.status |= st;
+        v.at_tf = fin;
+        ctl->at_tf = fin;
         ctl->status = v.status;
-        // more passes or longer lists needed: the step enqueued next exits
-        v.abort = (!v.all_done || v.overflow) ? 1 : 0;
+        // more passes or longer lists needed, or the run ends with this step:
+        // the step enqueued next exits
+        v.abort = (!v.all_done || v.overflow || fin) ? 1 : 0;
         ctl->abort = v.abort;
         ctl->seq = v.seq + 1;
         hout[v.seq & 1] = v;

@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -112,6 +112,7 @@ class Stats(ctypes.Structure):
         ("block_lists", ctypes.c_int32),
         ("block_force", ctypes.c_int32),
         ("launch_tuned", ctypes.c_int32),
+        ("steps", ctypes.c_int64),
     ]
 
 

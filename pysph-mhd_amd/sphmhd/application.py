@@ -1,11 +1,18 @@
-"""MHDApplication (reference: src/application.py:1-147).
+"""MHDApplication over the HIP backend (reference interface:
+src/application.py:26-147).
 
-Same hooks and the same start-up sequence: periodic DomainManager from the
-case bounds, the three-evaluation warm start with the magnetic field restored
-in between (application.py:110-132) and the initial step from dt_cfl and the
-force norm (application.py:134-147).  Host arrays are numpy; the device copy
-is kept in step through ``particle.gpu.push/pull`` exactly where the
-reference pushes and pulls."""
+Cases subclass it and override the reference's hooks -- ``create_mhd_particles``,
+``bounds`` and ``refresh_initial_thermodynamics`` -- and read the settings
+``initialize`` copies from the configuration.  What differs is where the
+start-up runs: the particles live on the GPU, so the warm start (three force
+evaluations with the magnetic field put back after the first) is one call of
+the library (``sphmhd_warm_start``).  Only a case that refreshes its
+thermodynamics after the first density sum (the vortex) has the host drive
+the three evaluations itself, calling the hook in between on a host copy of
+the summed density.  The order of operations is the reference's in both
+paths (application.py:110-147); the initial time step is the smaller of the
+Courant and force limits over all particles (application.py:134-147).
+"""
 from __future__ import annotations
 
 import numpy as np
@@ -13,8 +20,42 @@ import numpy as np
 from .base import Application, DomainManager
 from .config import SimulationConfig
 from .equations import COURANT_FACTOR, FORCE_FACTOR
-from .particles import INITIAL_PREDICTOR_PAIRS, INITIAL_STATE_GPU_PROPERTIES
 from .scheme import MHDScheme
+
+# settings copied from the configuration by initialize(): attribute name ->
+# (section, key) of SimulationConfig
+_SETTINGS = {
+    "gamma": ("physics", "gamma"),
+    "mu0": ("physics", "mu0"),
+    "kernel": ("numerics", "kernel"),
+    "hfact": ("numerics", "hfact"),
+    "density": ("numerics", "density"),
+    "periodic_mode": ("numerics", "periodic_mode"),
+    "tf": ("solver", "tf"),
+    "pfreq": ("solver", "pfreq"),
+    "adaptive_timestep": ("solver", "adaptive_timestep"),
+    "timestep_factor": ("solver", "timestep_factor"),
+    "nx": ("case", "nx"),
+}
+
+
+def initial_time_step(h, dt_cfl, au, av, aw, timestep_factor):
+    """First step of a run (application.py:134-147): the smallest of the
+    per-particle Courant limits tsf * 0.3 * h / dt_cfl and, over particles
+    with a non-zero acceleration, the force limits tsf * 0.25 * sqrt(h/|a|).
+    Raises when a signal speed is not positive or the step is not."""
+    h = np.asarray(h, dtype=np.float64)
+    dt_cfl = np.asarray(dt_cfl, dtype=np.float64)
+    if not np.all(dt_cfl > 0.0):
+        raise RuntimeError("initial dt: every dt_cfl must be positive (application.py:139)")
+    limit = float(np.min(timestep_factor * COURANT_FACTOR * h / dt_cfl))
+    anorm = np.sqrt(au * au + av * av + aw * aw)
+    moving = anorm > 0.0
+    if moving.any():
+        limit = min(limit, float(np.min(timestep_factor * FORCE_FACTOR * np.sqrt(h[moving] / anorm[moving]))))
+    if not limit > 0.0:
+        raise RuntimeError("initial dt must be positive (application.py:147)")
+    return limit
 
 
 class MHDApplication(Application):
@@ -24,42 +65,35 @@ class MHDApplication(Application):
         self.config = config
         super().__init__(fname=config.case.name)
 
+    # -- hooks of the reference -------------------------------------------------
     def initialize(self):
-        self.gamma = self.config.physics.gamma
-        self.mu0 = self.config.physics.mu0
-        self.kernel = self.config.numerics.kernel
-        self.hfact = self.config.numerics.hfact
-        self.density = self.config.numerics.density
-        self.periodic_mode = self.config.numerics.periodic_mode
-        self.tf = self.config.solver.tf
-        self.pfreq = self.config.solver.pfreq
-        self.adaptive_timestep = self.config.solver.adaptive_timestep
-        self.timestep_factor = self.config.solver.timestep_factor
-        self.nx = self.config.case.nx
+        for attr, (section, key) in _SETTINGS.items():
+            setattr(self, attr, getattr(getattr(self.config, section), key))
 
     def create_mhd_particles(self):  # pragma: no cover - interface
         raise NotImplementedError
 
     def refresh_initial_thermodynamics(self, particle):
-        pass
+        """Called once, after the first density sum of the warm start; the
+        base does nothing (application.py:43-44)."""
 
     @property
     def bounds(self):  # pragma: no cover - interface
         raise NotImplementedError
 
     def create_particles(self):
-        particles = self.create_mhd_particles()
-        assert len(particles) == 1
-        assert particles[0].name == "fluid"
-        self.scheme.setup_properties(particles[0])
-        return particles
+        arrays = self.create_mhd_particles()
+        if len(arrays) != 1 or arrays[0].name != "fluid":
+            raise ValueError("the MHD scheme runs exactly one particle array, named 'fluid'")
+        self.scheme.setup_properties(arrays[0])
+        return arrays
 
     def create_domain(self):
-        if self.periodic_mode is None:
+        if self.periodic_mode is None:  # open domain: no wrap, no minimum image
             return DomainManager()
-        xmin, xmax, ymin, ymax, zmin, zmax = self.bounds
-        return DomainManager(xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, zmin=zmin, zmax=zmax, periodic_in_x=True,
-                             periodic_in_y=True, periodic_in_z=True, periodic_mode=self.periodic_mode)
+        box = dict(zip(("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"), self.bounds))
+        return DomainManager(periodic_in_x=True, periodic_in_y=True, periodic_in_z=True,
+                             periodic_mode=self.periodic_mode, **box)
 
     def create_scheme(self):
         return MHDScheme(self.config)
@@ -68,55 +102,48 @@ class MHDApplication(Application):
         self.scheme.configure_solver(dt=1.0, tf=self.tf, adaptive_timestep=self.adaptive_timestep, pfreq=self.pfreq)
 
     def solve(self):
-        self._warm_initial_derivatives(self.solver)
-        self._compute_initial_dt(self.solver)
+        self.warm_start()
         super().solve()
 
-    @staticmethod
-    def _sync_initial_mhd_state(particle):
-        particle.Bevolx[:] = particle.Bx / particle.rho
-        particle.Bevoly[:] = particle.By / particle.rho
-        particle.Bevolz[:] = particle.Bz / particle.rho
-        particle.psi[:] = 0.0
-        for predicted, current in INITIAL_PREDICTOR_PAIRS:
-            getattr(particle, predicted)[:] = getattr(particle, current)
-        if particle.gpu is not None:
-            particle.gpu.push(*INITIAL_STATE_GPU_PROPERTIES)
+    # -- start-up on the device -------------------------------------------------
+    def _refreshes_thermodynamics(self) -> bool:
+        return type(self).refresh_initial_thermodynamics is not MHDApplication.refresh_initial_thermodynamics
 
-    def _warm_initial_derivatives(self, solver):
+    def warm_start(self):
+        """Accelerations of the initial state and the first time step, then
+        tell the solver not to evaluate them again (solver.py:35-38)."""
+        system = self.evaluator.system
+        if self._refreshes_thermodynamics():
+            self._warm_start_with_hook(system)
+        else:
+            system.warm_start()
         particle = self.particles[0]
-        initial_magnetic_field = (particle.Bx.copy(), particle.By.copy(), particle.Bz.copy())
-        particle.h0[:] = particle.h
-        self._sync_initial_mhd_state(particle)
-        particle.gpu.push("h0")
-        solver.integrator.initial_acceleration(solver.t, solver.dt)
-        particle.gpu.pull("rho", "h", "converged")
-        if not np.all(particle.converged == 1):
-            raise RuntimeError("warm start: density iteration did not converge (application.py:120)")
-        particle.Bx[:], particle.By[:], particle.Bz[:] = initial_magnetic_field
+        particle.gpu.pull("rho", "h", "Bx", "By", "Bz", "converged", "dt_cfl", "au", "av", "aw")
+        self.solver.dt = initial_time_step(particle.h, particle.dt_cfl, particle.au, particle.av, particle.aw,
+                                           self.timestep_factor)
+        self.solver.initial_acceleration_is_current = True
+
+    def _warm_start_with_hook(self, system):
+        """The three evaluations run from the host so that the case's hook can
+        see the first summed density: evaluation 1 from h0 = h and the synced
+        predicted state, the field restored, the hook, then two evaluations in
+        which h and omega stay as evaluation 1 left them (converged stays 1)."""
+        field = system.field
+        saved_b = [field(c).clone() for c in ("Bx", "By", "Bz")]
+
+        def evaluate(times):
+            field("h0").copy_(field("h"))
+            system.sync_initial_state()
+            for _ in range(times):
+                system.eval()
+            if not bool((field("converged") == 1.0).all()):
+                raise RuntimeError("warm start: the density iteration did not converge (application.py:120,131)")
+
+        evaluate(1)
+        for dst, src in zip(("Bx", "By", "Bz"), saved_b):
+            field(dst).copy_(src)
+        particle = self.particles[0]
+        particle.gpu.pull("rho", "h", "converged", "e")
         self.refresh_initial_thermodynamics(particle)
-        particle.h0[:] = particle.h
-        particle.gpu.push("e", "h0")
-        self._sync_initial_mhd_state(particle)
-        for _ in range(2):
-            solver.integrator.initial_acceleration(solver.t, solver.dt)
-        particle.gpu.pull("rho", "h", "Bx", "By", "Bz", "converged")
-        if not np.all(particle.converged == 1):
-            raise RuntimeError("warm start: density iteration did not converge (application.py:131)")
-        solver.initial_acceleration_is_current = True
-
-    def _compute_initial_dt(self, solver):
-        particle = self.particles[0]
-        particle.gpu.pull("h", "dt_cfl", "au", "av", "aw")
-        force_norm = np.sqrt(particle.au * particle.au + particle.av * particle.av + particle.aw * particle.aw)
-        if not np.all(particle.dt_cfl > 0.0):
-            raise RuntimeError("initial dt: dt_cfl must be positive (application.py:139)")
-        dt_courant = self.timestep_factor * COURANT_FACTOR * particle.h / particle.dt_cfl
-        dt_candidates = [float(np.min(dt_courant))]
-        force_mask = force_norm > 0.0
-        if np.any(force_mask):
-            dt_force = self.timestep_factor * FORCE_FACTOR * np.sqrt(particle.h[force_mask] / force_norm[force_mask])
-            dt_candidates.append(float(np.min(dt_force)))
-        solver.dt = min(dt_candidates)
-        if not solver.dt > 0.0:
-            raise RuntimeError("initial dt must be positive (application.py:147)")
+        particle.gpu.push("e")
+        evaluate(2)

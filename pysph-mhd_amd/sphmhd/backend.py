@@ -153,12 +153,15 @@ class HIPAccelerationEval:
         return self.system.step(nsteps)
 
     def run(self, nsteps: int, t: float, dt: float):
-        """nsteps PEC steps starting from the host clock (t, dt); returns the
-        new (t, dt) from the device clock."""
+        """Up to nsteps PEC steps starting from the host clock (t, dt); the
+        device stops after the step that reaches tf.  Returns the new (t, dt)
+        from the device clock and the number of steps taken."""
         self.system.set_clock(t, dt)
+        before = self.system.steps_taken()
         self.system.step(nsteps)
+        taken = self.system.steps_taken() - before
         tt, dtt = self.system.clock()
-        return float(tt[0]), float(dtt[0])
+        return float(tt[0]), float(dtt[0]), int(taken)
 
     def pending_dt(self):
         return self.system.pending_dt()

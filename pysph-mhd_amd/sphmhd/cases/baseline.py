@@ -221,8 +221,12 @@ def alfven_2d(nx: int = 97, ny: int = 168, *, amplitude: float = 0.1, kernel: st
                     artificial_magnetic_dissipation=0.0, cleaning_speed_factor=1.0,
                     cleaning_damping_factor=cleaning_damping_factor, timestep_factor=0.5, alpha_min=0.0,
                     alpha_max=alpha_max, tf=tf)
+    # the fields are linear in the amplitude: their exact derivative, the
+    # initial tangent of a rollout differentiated with respect to it
+    s_ph, c_ph = np.sin(ph), np.cos(ph)
+    d_amp = {"u": -s_ph * sa, "v": s_ph * ca, "w": c_ph.copy(), "Bx": -s_ph * sa, "By": s_ph * ca, "Bz": c_ph.copy()}
     return Setup(f"alfven_2d_{nx}x{ny}", [p], np.array([0, n], dtype=np.int64), f,
-                 info={"nx": nx, "ny": ny, "amplitude": amplitude})
+                 info={"nx": nx, "ny": ny, "amplitude": amplitude, "tangents": {"amplitude": d_amp}})
 
 
 def jittered(setup: Setup, frac: float = 0.1, seed: int = 1) -> Setup:

@@ -110,7 +110,13 @@ class DeviceSystem:
         check(self.lib.sphmhd_advance_clock(self._ctx))
 
     def step(self, nsteps: int = 1) -> int:
+        """Up to nsteps PEC steps (fewer when every instance's clock reaches
+        its tf first); returns the library's status code."""
         return check(self.lib.sphmhd_step(self._ctx, int(nsteps)))
+
+    def steps_taken(self) -> int:
+        """PEC steps completed by this context so far."""
+        return int(self.stats()["steps"])
 
     def sync_initial_state(self) -> None:
         check(self.lib.sphmhd_sync_initial_state(self._ctx))

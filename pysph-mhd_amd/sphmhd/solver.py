@@ -1,10 +1,15 @@
-"""MHDSolver (reference: src/solver.py:1-77).
+"""MHDSolver (reference interface: src/solver.py:15-77).
 
-The loop keeps the reference's structure -- dump, optional startup RHS,
-dt from the last evaluation, then step / advance t / new dt / dump until tf
-or max_steps -- but hands the device runs of steps between dumps
-(``evaluator.run``), because the time step, the clock and every field stay
-on the GPU; the host clock is re-read after each run."""
+The time loop of the reference -- dump the start, make sure the
+accelerations of the initial state exist, take the first step size, then
+step, advance the clock, pick the next step size and dump every ``pfreq``
+steps until ``tf`` or ``max_steps`` -- with the steps handed to the device in
+runs that end at the next dump (``evaluator.run``): step sizes, the clock and
+every field stay on the GPU, and the host re-reads the clock after each run.
+A run stops early after the step that lands on ``tf`` and reports how many
+steps it took, so ``count`` is exact and no zero-length step is taken once
+the clock has reached ``tf``.
+"""
 from __future__ import annotations
 
 import logging
@@ -19,30 +24,40 @@ class MHDSolver(Solver):
         super().__init__(*args, **kwargs)
         self.initial_acceleration_is_current = False
 
+    def _time_left(self) -> bool:
+        """Steps remain while the clock is short of tf (by more than the
+        end-time tolerance), the next step has a length and the step budget
+        is not spent."""
+        return (self.tf - self.t) > self._epsilon and self.dt > 0.0 and self.count < self.max_steps
+
+    def _run_length(self) -> int:
+        """Steps handed to the device at once: one when callbacks watch each
+        step, else up to the next dump (and never past max_steps)."""
+        if self.pre_step_callbacks or self.post_step_callbacks:
+            return 1
+        to_dump = self.pfreq - (self.count % self.pfreq) if self.pfreq > 0 else self.max_steps
+        return max(1, min(to_dump, self.max_steps - self.count))
+
     def solve(self, show_progress=False):
         self._epsilon = EPSILON * self.tf
         self.dump_output()
         self.barrier()
 
-        # Skip the startup RHS when the application has already warmed it.
-        if self.initial_acceleration_is_current:
-            self.initial_acceleration_is_current = False
-        else:
+        # A warm start by the application leaves the initial accelerations
+        # in place; only without one are they evaluated here.
+        if not self.initial_acceleration_is_current:
             self.integrator.initial_acceleration(self.t, self.dt)
+        self.initial_acceleration_is_current = False
 
         self.dt = self._get_timestep()
-        ev = self.integrator.evaluator
-        while (self.tf - self.t) > self._epsilon and self.count < self.max_steps:
-            if self.pre_step_callbacks or self.post_step_callbacks:
-                batch = 1
-            else:
-                to_dump = self.pfreq - (self.count % self.pfreq) if self.pfreq > 0 else self.max_steps
-                batch = max(1, min(to_dump, self.max_steps - self.count))
+        evaluator = self.integrator.evaluator
+        while self._time_left():
+            run = self._run_length()
             for callback in self.pre_step_callbacks:
                 callback(self)
             logger.debug("Iteration=%d, time=%f, timestep=%f", self.count, self.t, self.dt)
-            self.t, self.dt = ev.run(batch, self.t, self.dt)
-            self.count += batch
+            self.t, self.dt, taken = evaluator.run(run, self.t, self.dt)
+            self.count += taken
             self._epsilon = EPSILON * self.tf * self.count
             for callback in self.post_step_callbacks:
                 callback(self)

@@ -80,9 +80,13 @@ class TangentSystem(DeviceSystem):
 
 
 def _ic_tangent(setup_fn: Callable, kw: dict, name: str) -> dict[str, np.ndarray]:
-    """d(initial fields)/d(kw[name]) by a central difference of the initial
-    condition builder (exact for the builders here, whose fields are linear
-    or quadratic in their amplitudes)."""
+    """d(initial fields)/d(kw[name]): the builder's own exact derivative when
+    it records one (``info["tangents"][name]``, e.g. alfven_2d, whose fields
+    are linear in the amplitude), else a central difference of the builder
+    (exact up to rounding for fields at most quadratic in the input)."""
+    exact = setup_fn(**kw).info.get("tangents", {}).get(name)
+    if exact is not None:
+        return {k: np.asarray(v, dtype=np.float64) for k, v in exact.items()}
     x = kw[name]
     delta = 1.0e-3 * max(abs(x), 1.0)
     fp = setup_fn(**{**kw, name: x + delta}).fields

@@ -20,14 +20,16 @@ def init_fields(setup):
     return f
 
 
-def natural_scales(orc) -> dict[str, float]:
+def natural_scales(orc, mask=None) -> dict[str, float]:
     """Norms of the magnitude each divergence-cleaning field would have if B
     varied on the scale h.  Fields that are analytically zero for a given
     state (div B of a uniform field, psi before any divergence has grown) hold
     only round-off, so their relative error is measured against 1e-3 of this
-    scale instead of their own norm."""
-    bx, by, bz = orc.get("Bx"), orc.get("By"), orc.get("Bz")
-    rho, h, cs = orc.get("rho"), orc.get("h"), orc.get("cs")
+    scale instead of their own norm.  ``mask`` selects the particles (one
+    instance of an ensemble), as in compare()."""
+    sel = (lambda v: v) if mask is None else (lambda v: v[mask])
+    bx, by, bz = sel(orc.get("Bx")), sel(orc.get("By")), sel(orc.get("Bz"))
+    rho, h, cs = sel(orc.get("rho")), sel(orc.get("h")), sel(orc.get("cs"))
     b = np.sqrt(bx * bx + by * by + bz * bz)
     vw = np.sqrt(cs * cs + b * b / rho)
     sc = {
@@ -46,7 +48,7 @@ def compare(dev, orc, names, tol, label="", scales=None, mask=None):
     1e-3 of its natural scale; ``tol`` is a number or a {name: tolerance}
     map, ``mask`` selects the particles compared."""
     worst = {}
-    scales = natural_scales(orc) if scales is None else scales
+    scales = natural_scales(orc, mask) if scales is None else scales
     for nm in names:
         a = dev.get(nm)
         b = orc.get(nm)

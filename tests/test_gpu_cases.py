@@ -147,7 +147,7 @@ def test_vortex_refreshes_energy_after_first_density(gpu, tmp_path):
 
     app = case_app("mhdvortex", CASE_NX["mhdvortex"], directory=str(tmp_path / "out"))
     app.setup(["-d", str(tmp_path / "out"), "--max-steps", "0"])
-    app._warm_initial_derivatives(app.solver)
+    app.warm_start()
     pa = app.particles[0]
     pa.gpu.pull("e", "rho", "x", "y")
     # the two evaluations after the refresh re-sum rho from the same h, so

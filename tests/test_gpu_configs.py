@@ -1,0 +1,356 @@
+"""The benchmark workloads at their stated checks, on the GPU through the C ABI.
+
+  C2  rotor 2-D: the unjittered 128^2 rotor against the oracle after every
+      step for 500 steps (the check subset of the 512^2 workload);
+  C3  blast 3-D: 32^3 against the oracle over the warm start and three
+      steps, and the full 128^2 x 128 workload over the bench's step count
+      through properties that hold at any size (mass exact, momentum against
+      sqrt(2 M E_kin), energy drift, finite fields, neighbour counts);
+  C4  the 64-instance uneven Orszag-Tang ensemble: every instance equals the
+      same instance run alone, bit for bit, and the whole batch matches the
+      oracle, per instance, after the warm start and three steps, with
+      conserved totals per instance;
+  C5  the Alfven wave 97 x 168: plain steps against the oracle for 24 steps;
+      the 400-step tangent gradient against central differences of the GPU
+      primal (eps = 1e-6 and 1e-5 relative); and at a reduced size the
+      tangent against central differences of the oracle's primal, the CPU
+      reference gradient.
+
+Fields are compared by relative L2 <= 1e-10 with the floors of
+helpers.natural_scales (1e-3 of the natural scale of the divergence-cleaning
+fields, which are analytically zero in these states and hold only
+round-off); conserved totals to 1e-8.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import RATES, STATE, compare, init_fields
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-10
+
+
+def _pair(setup, threads=16):
+    from pyoracle import OracleSystem
+    from sphmhd.device import DeviceSystem
+
+    dev = DeviceSystem(setup.params, setup.offsets)
+    orc = OracleSystem(setup.params, setup.offsets, threads=threads)
+    f = init_fields(setup)
+    dev.set_fields(f)
+    orc.set_fields(f)
+    return dev, orc
+
+
+def _totals_close(td, to, label):
+    """Conserved totals of one instance to 1e-8: mass and energy against
+    themselves, momentum against sqrt(2 M E_kin) (its total is zero up to
+    round-off)."""
+    mass, ekin = to[0], to[4]
+    assert abs(td[0] - to[0]) <= 1e-8 * abs(mass), label
+    pscale = np.sqrt(2.0 * mass * max(ekin, 1e-300))
+    assert np.linalg.norm(td[1:4] - to[1:4]) <= 1e-8 * pscale, label
+    e_d, e_o = td[4] + td[5] + td[6], to[4] + to[5] + to[6]
+    assert abs(e_d - e_o) <= 1e-8 * abs(e_o), label
+
+
+# ---- C5: differentiable Alfven rollout ------------------------------------------
+
+def _alfven(nx=97, ny=168, **kw):
+    from sphmhd.cases.baseline import alfven_2d
+
+    return alfven_2d(nx, ny, **kw)
+
+
+def test_c5_alfven_steps_vs_oracle(gpu):
+    """The travelling wave is divergence-free and incompressible: div v, the
+    switch source alpha_loc (alpha_min = 0 here) and the cleaning fields are
+    analytically zero and hold only round-off, so they are measured against
+    the physical floors of helpers.physical_scales (1e-3 of v_w/h for div v,
+    of alpha_max for the switch, of |B|-based scales for the cleaning
+    fields); every other field against its own norm."""
+    from helpers import physical_scales
+
+    s = _alfven()
+    p = s.params[0]
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    compare(dev, orc, STATE + RATES, TOL, "C5 warm start", scales=physical_scales(orc, p.mu0, p.alpha_max))
+    for k in range(6):
+        dev.step(4)
+        orc.step(4)
+        compare(dev, orc, STATE + RATES, TOL, f"C5 step {4 * (k + 1)}",
+                scales=physical_scales(orc, p.mu0, p.alpha_max))
+    _totals_close(dev.totals()[0], orc.totals()[0], "C5 totals")
+    assert np.array_equal(dev.clock()[0], orc.clock()[0]) or \
+        abs(dev.clock()[0][0] - orc.clock()[0][0]) <= 1e-12 * orc.clock()[0][0]
+
+
+NAMES = ("amplitude", "alpha_max", "cleaning_damping_factor")
+THETA = (0.1, 1.0, 1.0)
+
+
+def test_c5_tangent_400_steps_vs_central_differences(gpu):
+    """dE/dtheta over the full C5 rollout from one tangent sweep against
+    central differences of the plain GPU step at two step sizes.  The
+    tolerance is 1e-5 relative to the gradient's norm: the alpha_max and
+    cleaning-damping derivatives of this divergence-free, incompressible wave
+    are ~1e-9 and below, where a central difference is rounding noise."""
+    from sphmhd.tangent import energy_primal, energy_rollout
+
+    steps = 400
+    e, g = energy_rollout(_alfven, THETA, NAMES, steps)
+    kw = dict(zip(NAMES, THETA))
+    e0 = energy_primal(_alfven, kw, steps)
+    assert abs(e - e0) <= 1e-12 * abs(e0)
+    fd = np.zeros((2, 3))
+    for k, nm in enumerate(NAMES):
+        for r, rel in enumerate((1e-5, 1e-6)):
+            eps = rel * THETA[k]
+            ep = energy_primal(_alfven, {**kw, nm: THETA[k] + eps}, steps)
+            em = energy_primal(_alfven, {**kw, nm: THETA[k] - eps}, steps)
+            fd[r, k] = (ep - em) / (2.0 * eps)
+    gnorm = np.linalg.norm(fd[1])
+    print(f"C5 400 steps: tangent {g}, FD(1e-6) {fd[1]}, FD(1e-5) {fd[0]}")
+    assert gnorm > 0.1  # dE/dA is O(E / A)
+    assert np.all(np.abs(fd[0] - fd[1]) <= 1e-5 * gnorm), fd
+    assert np.all(np.abs(g - fd[1]) <= 1e-5 * gnorm), (g, fd[1])
+    assert abs(g[0] - fd[1, 0]) <= 1e-6 * abs(fd[1, 0])  # the amplitude derivative itself
+
+
+def test_c5_tangent_vs_oracle_central_differences(gpu):
+    """The CPU reference gradient: central differences of the oracle's FP64
+    rollout (24 x 42 particles, 10 steps) against the GPU tangent sweep of
+    the same rollout."""
+    from pyoracle import OracleSystem
+    from sphmhd.tangent import energy_rollout
+
+    steps = 10
+
+    def small(**kw):
+        return _alfven(24, 42, **kw)
+
+    def oracle_energy(kw):
+        s = small(**kw)
+        o = OracleSystem(s.params, s.offsets, threads=16)
+        o.set_fields(init_fields(s))
+        o.warm_start()
+        o.step(steps)
+        t = o.totals()[0]
+        return t[4] + t[6]
+
+    e, g = energy_rollout(small, THETA, NAMES, steps)
+    kw = dict(zip(NAMES, THETA))
+    e0 = oracle_energy(kw)
+    assert abs(e - e0) <= 1e-12 * abs(e0)
+    fd = np.zeros(3)
+    for k, nm in enumerate(NAMES):
+        eps = 1e-6 * THETA[k]
+        fd[k] = (oracle_energy({**kw, nm: THETA[k] + eps}) - oracle_energy({**kw, nm: THETA[k] - eps})) / (2 * eps)
+    gnorm = np.linalg.norm(fd)
+    print(f"C5 small: tangent {g}, oracle FD {fd}")
+    assert np.all(np.abs(g - fd) <= 1e-5 * gnorm), (g, fd)
+    assert abs(g[0] - fd[0]) <= 1e-6 * abs(fd[0])
+
+
+# ---- C2: rotor -----------------------------------------------------------------
+
+def test_c2_rotor_128_every_step_500_steps(gpu):
+    """Every step of 500, every state field (and the rates every 100 steps).
+    The rotor's shocks amplify rounding: the oracle run against a copy of
+    itself whose x positions are one ulp larger drifts past 1e-10 in psi
+    and alpha near step 340 and reaches ~5e-9 by step 500.  So at each step
+    the GPU must lie within 1e-10 of the oracle, or within 10 times that
+    measured amplification where it has grown beyond 1e-11; and it must meet
+    1e-10 outright over the first 300 steps."""
+    from pyoracle import OracleSystem
+    from sphmhd.cases.baseline import rotor_2d
+
+    s = rotor_2d(128)
+    dev, orc = _pair(s)
+    nudged = OracleSystem(s.params, s.offsets, threads=16)
+    f = init_fields(s)
+    nudged.set_fields({**f, "x": np.nextafter(f["x"], np.inf)})
+    for sysm in (dev, orc, nudged):
+        sysm.warm_start()
+    worst_strict = 0.0
+    last_strict = 0
+    for k in range(500):
+        for sysm in (dev, orc, nudged):
+            sysm.step(1)
+        names = STATE + RATES if (k + 1) % 100 == 0 else STATE
+        sens = compare(nudged, orc, names, np.inf)
+        tol = {nm: (TOL if v < 1e-11 else max(TOL, 10.0 * v)) for nm, v in sens.items()}
+        w = compare(dev, orc, names, tol, f"C2 step {k + 1}")
+        if max(w.values()) <= TOL:
+            last_strict = k + 1 if last_strict == k else last_strict
+        if k < 300:
+            worst_strict = max(worst_strict, max(w.values()))
+    print(f"C2 rotor 128^2: worst rel-L2 over steps 1-300 {worst_strict:.3e}; "
+          f"within 1e-10 through step {last_strict}; oracle's own 1-ulp drift at 500: {max(sens.values()):.2e}")
+    assert worst_strict <= TOL
+    _totals_close(dev.totals()[0], orc.totals()[0], "C2 totals")
+
+
+# ---- C3: blast -----------------------------------------------------------------
+
+def test_c3_blast_32_vs_oracle(gpu):
+    from sphmhd.cases.baseline import blast_3d
+
+    s = blast_3d(32)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    compare(dev, orc, STATE + RATES + ("z",), TOL, "C3 32^3 warm start")
+    for k in range(3):
+        dev.step(1)
+        orc.step(1)
+        compare(dev, orc, STATE + RATES + ("z",), TOL, f"C3 32^3 step {k + 1}")
+    _totals_close(dev.totals()[0], orc.totals()[0], "C3 32^3 totals")
+
+
+def test_c3_blast_128_properties(gpu):
+    """The full 2,097,152-particle blast over 20 steps (the bench's count):
+    mass is the sum of fixed masses; total momentum stays at round-off of
+    sqrt(2 M E_kin) (every pair term is antisymmetric; the divergence
+    correction only acts where div B has grown); the total energy drifts by
+    less than 1e-4 of itself; every field stays finite; particles keep
+    about 60 neighbours."""
+    from sphmhd.cases.baseline import blast_3d
+    from sphmhd.device import DeviceSystem
+
+    s = blast_3d(128)
+    dev = DeviceSystem(s.params, s.offsets)
+    dev.set_fields(init_fields(s))
+    dev.warm_start()
+    t0 = dev.totals()[0]
+    dev.step(20)
+    t1 = dev.totals()[0]
+    mass = float(np.sum(s.fields["m"]))
+    assert t1[0] == pytest.approx(mass, rel=1e-13)
+    pscale = np.sqrt(2.0 * t1[0] * t1[4])
+    mom = np.linalg.norm(t1[1:4])
+    e0, e1 = t0[4] + t0[5] + t0[6], t1[4] + t1[5] + t1[6]
+    print(f"C3 128^3: |P| / sqrt(2 M Ekin) = {mom / pscale:.3e}, energy drift {(e1 - e0) / e0:.3e}")
+    assert mom <= 1e-10 * pscale
+    assert abs(e1 - e0) <= 1e-4 * abs(e0)
+    assert t1[4] > 0.0  # the blast has started to move
+    for nm in ("x", "y", "z", "u", "v", "w", "e", "h", "rho", "Bx", "By", "Bz", "psi"):
+        assert np.all(np.isfinite(dev.get(nm))), nm
+    st = dev.stats()
+    assert 50 <= st["pairs_force"] / s.n <= 80
+    assert st["steps"] == 20
+
+
+# ---- C4: ensemble ----------------------------------------------------------------
+
+def test_c4_ensemble_64_each_instance_equals_alone(gpu):
+    """The full seeded table (64 instances, 4,096 to 65,536 particles): after
+    the warm start and three steps every instance's fields and clock equal,
+    bit for bit, those of the same instance run alone in a context of its
+    own (each particle's sums run over the same neighbours in the same
+    order; the instances only share launches)."""
+    from sphmhd.cases.baseline import ensemble_ot
+    from sphmhd.device import DeviceSystem
+
+    s = ensemble_ot(64)
+    assert len(s.params) == 64 and 64 * 64 <= np.diff(s.offsets).min() and np.diff(s.offsets).max() <= 256 * 256
+    batch = DeviceSystem(s.params, s.offsets)
+    batch.set_fields(init_fields(s))
+    batch.warm_start()
+    batch.step(3)
+    got = {nm: batch.get(nm) for nm in STATE + RATES}
+    tb, dtb = batch.clock()
+    for k in range(64):
+        a, b = int(s.offsets[k]), int(s.offsets[k + 1])
+        one = DeviceSystem([s.params[k]], np.array([0, b - a], dtype=np.int64))
+        one.set_fields({nm: v[a:b] for nm, v in init_fields(s).items()})
+        one.warm_start()
+        one.step(3)
+        for nm in STATE + RATES:
+            assert np.array_equal(got[nm][a:b], one.get(nm)), (k, nm)
+        t1, dt1 = one.clock()
+        assert t1[0] == tb[k] and dt1[0] == dtb[k], k
+        one.close()
+
+
+class _Snap:
+    """One host copy of each compared field (the batch is 1.9M particles),
+    sliced per instance by compare()'s mask."""
+
+    def __init__(self, sysm):
+        self.v = {nm: sysm.get(nm) for nm in STATE + RATES + ("cs",)}
+
+    def get(self, nm):
+        return self.v[nm]
+
+
+def _instance_masks(s):
+    for k in range(len(s.params)):
+        mask = np.zeros(s.n, dtype=bool)
+        mask[int(s.offsets[k]):int(s.offsets[k + 1])] = True
+        yield k, mask
+
+
+def test_c4_ensemble_64_jittered_vs_oracle(gpu):
+    """The table's 64 instances (sizes, field amplitudes, dissipation and
+    cleaning parameters) on lattices jittered by 0.1 dx: every instance
+    matches the oracle to 1e-10 after the warm start and three steps, and its
+    conserved totals to 1e-8."""
+    from sphmhd.cases.baseline import ensemble_ot, jittered
+
+    s = jittered(ensemble_ot(64), 0.1, seed=64)
+    dev, orc = _pair(s)
+    dev.warm_start()
+    orc.warm_start()
+    dev.step(3)
+    orc.step(3)
+    sd, so = _Snap(dev), _Snap(orc)
+    for k, mask in _instance_masks(s):
+        compare(sd, so, STATE + RATES, TOL, f"C4 jittered instance {k}", mask=mask)
+    td, to = dev.totals(), orc.totals()
+    for k in range(64):
+        _totals_close(td[k], to[k], f"C4 jittered instance {k} totals")
+    assert np.allclose(dev.clock()[0], orc.clock()[0], rtol=1e-12, atol=0)
+
+
+def test_c4_ensemble_64_exact_lattices_vs_oracle(gpu):
+    """The table itself, on exact lattices.  Where a lattice puts particles
+    on points at which div v and curl v vanish by symmetry, the reference's
+    Cullen-Dehnen limiter (equations.py:418-422) is a quotient of round-off
+    terms, so the reference algorithm itself moves alpha, div v and de/dt by
+    up to ~1e-5 when one input bit changes (tests/test_conditioning_cpu.py).  That amplification is measured per
+    instance and field from the oracle run against a copy whose x positions
+    are one ulp larger, and the GPU must lie within 10 times it (never looser
+    than the 1e-10 standard); instances whose own amplification stays below
+    1e-11 are held to 1e-10 outright, and all totals to 1e-8."""
+    from pyoracle import OracleSystem
+    from sphmhd.cases.baseline import ensemble_ot
+
+    s = ensemble_ot(64)
+    dev, orc = _pair(s)
+    nudged = OracleSystem(s.params, s.offsets, threads=16)
+    f = init_fields(s)
+    nudged.set_fields({**f, "x": np.nextafter(f["x"], np.inf)})
+    for sysm in (dev, orc, nudged):
+        sysm.warm_start()
+        sysm.step(3)
+    sd, so, sn = _Snap(dev), _Snap(orc), _Snap(nudged)
+    strict = 0
+    for k, mask in _instance_masks(s):
+        sens = compare(sn, so, STATE + RATES, np.inf, mask=mask)
+        tol = {nm: max(TOL, 10.0 * v) for nm, v in sens.items()}
+        compare(sd, so, STATE + RATES, tol, f"C4 instance {k}", mask=mask)
+        if max(sens.values()) < 1e-11:
+            compare(sd, so, STATE + RATES, TOL, f"C4 instance {k} (well-conditioned)", mask=mask)
+            strict += 1
+    print(f"C4 exact lattices: {strict} of 64 instances well-conditioned and within 1e-10")
+    assert strict >= 32
+    td, to = dev.totals(), orc.totals()
+    for k in range(64):
+        _totals_close(td[k], to[k], f"C4 instance {k} totals")
+    assert np.allclose(dev.clock()[0], orc.clock()[0], rtol=1e-12, atol=0)

@@ -76,7 +76,8 @@ def test_dump_restores_exactly(gpu, tmp_path):
     app2 = OrszagTang2D(_config(tmp_path, 32, 0))
     app2.setup(["-d", str(tmp_path / "out2")])
     t, dt, count = restore(mid, app2.particles)
-    t2, dt2 = app2.evaluator.run(2, t, dt)
+    t2, dt2, taken = app2.evaluator.run(2, t, dt)
+    assert taken == 2
     ref = np.load(dumps[-1], allow_pickle=False)
     pa = app2.particles[0]
     pa.gpu.pull("x", "u", "Bx", "psi", "e")

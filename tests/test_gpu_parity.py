@@ -322,6 +322,7 @@ def test_orszag_tang_128_100_steps(gpu):
         dev.step(10)
         orc.step(10)
         compare(dev, orc, STATE, TOL, f"C1 step {10 * (k + 1)}")
+    compare(dev, orc, RATES, TOL, "C1 rates at step 100")
     td = dev.totals()[0]
     to = orc.totals()[0]
     for q, name in enumerate(("mass", "momx", "momy", "momz", "ekin", "etherm", "emag", "divb", "volume")):
@@ -344,3 +345,43 @@ def test_neighbour_list_growth(gpu):
     orc.step(2)
     compare(dev, orc, STATE + RATES, TOL, "grown lists")
     assert np.allclose(dev.clock()[1], orc.clock()[1], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("graphs", ["0", "1"])
+def test_run_stops_at_tf(gpu, monkeypatch, graphs):
+    """A run bounded by tf (inside the fourth step): sphmhd_step(100) takes
+    four steps, the last cut to tf - t, and stops with the clock on tf; the
+    step enqueued ahead of the host's wait (graphs on) exits at entry.  The
+    state equals the oracle's, which stops after the same step."""
+    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.device import DeviceSystem
+    from pyoracle import OracleSystem
+
+    free = OracleSystem(_ot(40).params, _ot(40).offsets)
+    free.set_fields(init_fields(_ot(40)))
+    free.warm_start()
+    ts = []
+    for _ in range(4):
+        free.step(1)
+        ts.append(free.clock()[0][0])
+    tf = ts[2] + 0.5 * (ts[3] - ts[2])
+    s = jittered(orszag_tang_2d(40, tf=tf), 0.1, seed=40)
+    monkeypatch.setenv("SPHMHD_GRAPHS", graphs)
+    dev = DeviceSystem(s.params, s.offsets)
+    orc = OracleSystem(s.params, s.offsets)
+    for sysm in (dev, orc):
+        sysm.set_fields(init_fields(s))
+    dev.set_launch((256, 256, 256))
+    dev.warm_start()
+    orc.warm_start()
+    dev.step(100)
+    orc.step(100)
+    assert dev.steps_taken() == 4 and orc.steps_taken() == 4
+    td, dtd = dev.clock()
+    assert td[0] == tf and dtd[0] == 0.0
+    compare(dev, orc, STATE + RATES, TOL, "stopped at tf")
+    # a later call takes no further state forward than a zero-length step
+    x = dev.get("x")
+    dev.step(3)
+    assert dev.steps_taken() == 5
+    assert np.array_equal(dev.get("x"), x)
