@@ -165,8 +165,8 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
     itself whose x positions are one ulp larger drifts past 1e-10 in psi
     and alpha near step 340 and reaches ~5e-9 by step 500.  So at each step
     the GPU must lie within 1e-10 of the oracle, or within 10 times that
-    measured amplification where it has grown beyond 1e-11; and it must meet
-    1e-10 outright over the first 300 steps."""
+    measured amplification (its largest field) where it has grown beyond
+    1e-11; and it must meet 1e-10 outright over the first 300 steps."""
     from pyoracle import OracleSystem
     from sphmhd.cases.baseline import rotor_2d
 
@@ -184,7 +184,8 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
             sysm.step(1)
         names = STATE + RATES if (k + 1) % 100 == 0 else STATE
         sens = compare(nudged, orc, names, np.inf)
-        tol = {nm: (TOL if v < 1e-11 else max(TOL, 10.0 * v)) for nm, v in sens.items()}
+        grown = max(sens.values())
+        tol = TOL if grown < 1e-11 else max(TOL, 10.0 * grown)
         w = compare(dev, orc, names, tol, f"C2 step {k + 1}")
         if max(w.values()) <= TOL:
             last_strict = k + 1 if last_strict == k else last_strict
@@ -325,9 +326,10 @@ def test_c4_ensemble_64_exact_lattices_vs_oracle(gpu):
     terms, so the reference algorithm itself moves alpha, div v and de/dt by
     up to ~1e-5 when one input bit changes (tests/test_conditioning_cpu.py).  That amplification is measured per
     instance and field from the oracle run against a copy whose x positions
-    are one ulp larger, and the GPU must lie within 10 times it (never looser
-    than the 1e-10 standard); instances whose own amplification stays below
-    1e-11 are held to 1e-10 outright, and all totals to 1e-8."""
+    are one ulp larger, and the GPU must lie within 10 times the largest
+    field-wise amplification of its instance (never looser than the 1e-10
+    standard); instances whose own amplification stays below 1e-11 are held
+    to 1e-10 outright, and all totals to 1e-8."""
     from pyoracle import OracleSystem
     from sphmhd.cases.baseline import ensemble_ot
 
@@ -343,7 +345,9 @@ def test_c4_ensemble_64_exact_lattices_vs_oracle(gpu):
     strict = 0
     for k, mask in _instance_masks(s):
         sens = compare(sn, so, STATE + RATES, np.inf, mask=mask)
-        tol = {nm: max(TOL, 10.0 * v) for nm, v in sens.items()}
+        # one nudge is one sample of how rounding moves the instance: the
+        # fields it moved least still move by as much under other roundings
+        tol = max(TOL, 10.0 * max(sens.values()))
         compare(sd, so, STATE + RATES, tol, f"C4 instance {k}", mask=mask)
         if max(sens.values()) < 1e-11:
             compare(sd, so, STATE + RATES, TOL, f"C4 instance {k} (well-conditioned)", mask=mask)
