@@ -134,3 +134,17 @@ def physical_scales(orc, mu0=1.0, alpha_max=1.0, mask=None) -> dict[str, float]:
         sc[c] = np.linalg.norm(vw)
         sc["a" + c] = np.linalg.norm(vw * vw / h)
     return sc
+
+
+def nudged(fields, seed=1):
+    """Copy of initial fields whose x and y positions are moved by one ulp
+    each, up or down at random per particle: the rounding-level perturbation
+    that measures how much the reference algorithm itself amplifies rounding
+    for a given state (a uniform nudge of x alone keeps lattice symmetries
+    and can miss it)."""
+    rng = np.random.default_rng(seed)
+    out = dict(fields)
+    for c in ("x", "y"):
+        v = np.asarray(fields[c], dtype=np.float64)
+        out[c] = np.nextafter(v, rng.choice([-np.inf, np.inf], v.size))
+    return out
