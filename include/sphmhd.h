@@ -221,6 +221,17 @@ int sphmhd_get_pending_dt(sphmhd_ctx* ctx, double* dt);
 /* Conserved totals, host array of n_instances * SPHMHD_NTOTALS doubles. */
 int sphmhd_totals(sphmhd_ctx* ctx, double* out);
 
+/* sphmhd_totals without any allocation.  No counterpart in the reference; it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Enqueues the reduction
+ * on the context's stream and writes n_instances * SPHMHD_NTOTALS doubles to
+ * the DEVICE array out_dev (no host copy, no synchronisation).  The partial
+ * sums use a static device array of the library (2 MiB per GPU, held until
+ * the process ends), so at most 16384 instances (SPHMHD_ECAP above that),
+ * and two calls that overlap on one GPU race.  Values equal sphmhd_totals'
+ * up to the order of the additions. */
+int sphmhd_totals_noalloc(sphmhd_ctx* ctx, double* out_dev);
+
 /* Neighbour-search exports for parity checks, into HOST buffers of n
  * entries: sorted cell key (instance-local key plus the instance's cell
  * base) and the storage index at each sorted position. */

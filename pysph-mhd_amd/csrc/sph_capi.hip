@@ -572,6 +572,16 @@ static int check_ready(sphmhd_ctx* c) {
     return 0;
 }
 
+int sph::ctx_view(sphmhd_ctx* c, Dev** d, int64_t* longest) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    *d = &c->d;
+    int64_t m = 0;
+    for (size_t k = 0; k + 1 < c->offsets.size(); ++k) m = std::max(m, c->offsets[k + 1] - c->offsets[k]);
+    *longest = m;
+    return 0;
+}
+
 static int count_unconverged(sphmhd_ctx* c, int64_t* out) {
     launch_count_unconverged(c->d);
     int rc = read_ctl(c);
@@ -712,6 +722,8 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.scratch3, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.aprev, 3 * n))) return cleanup(rc);
         if ((rc = dalloc(&d.totals, (int64_t)n_instances * SPHMHD_NTOTALS))) return cleanup(rc);
+        // this device's copy of the allocation-free totals' static scratch
+        if ((rc = scratch_setup(&d.tot_scratch))) return cleanup(rc);
         // the control block's host mirror: pinned, mapped and coherent, so the
         // kernel that ends an evaluation writes it directly (no copy)
         hipError_t e = hipHostMalloc((void**)&c->hctl, 3 * sizeof(CtlView), hipHostMallocMapped | hipHostMallocCoherent);

@@ -1013,8 +1013,15 @@ struct Dev {
     double* aprev;          // 3 x n: the accelerations the force pass reads as "previous",
                             // kept by the first prep of an evaluation for a redone pass
     double* totals;         // ninst x SPHMHD_NTOTALS
+    double* tot_scratch;    // this device's copy of the static totals scratch (sph_totals_static.hip)
     Fields fields;
 };
+
+// sph_totals_static.hip: resolves the current device's copy of its static
+// scratch array (once per device); sph_capi.hip: a context's device state
+// and its longest instance, for entry points defined in other files
+int scratch_setup(double** out);
+int ctx_view(sphmhd_ctx* c, Dev** d, int64_t* longest);
 
 // tangent-linear step (sph_tangent.hip)
 constexpr int kTanDirs = SPHMHD_NTANGENTS;

@@ -228,6 +228,7 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_set_clock": ([vp, dp, dp], ctypes.c_int),
         "sphmhd_get_pending_dt": ([vp, dp], ctypes.c_int),
         "sphmhd_totals": ([vp, dp], ctypes.c_int),
+        "sphmhd_totals_noalloc": ([vp, vp], ctypes.c_int),
         "sphmhd_cells": ([vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_neighbours": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "sphmhd_get_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),

@@ -152,6 +152,17 @@ class DeviceSystem:
         check(self.lib.sphmhd_totals(self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def totals_noalloc(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """The conserved totals through sphmhd_totals_noalloc: enqueued on the
+        context's stream into a device tensor [ninst, NTOTALS], with no
+        allocation in the library and no synchronisation."""
+        if out is None:
+            out = torch.empty((self.ninst, NTOTALS), dtype=torch.float64, device=self.device)
+        if out.dtype != torch.float64 or not out.is_contiguous() or out.numel() < self.ninst * NTOTALS:
+            raise ValueError("out must be a contiguous float64 device tensor of ninst x NTOTALS")
+        check(self.lib.sphmhd_totals_noalloc(self._ctx, ctypes.c_void_p(out.data_ptr())))
+        return out
+
     def totals_dict(self, k: int = 0) -> dict[str, float]:
         return dict(zip(TOTALS, self.totals()[k].tolist()))
 
