@@ -157,9 +157,10 @@ def run_gpu(name: str, steps: int, warmup: int, timing_steps: int = 10):
     """Timed region: `steps` steps in one sphmhd_step call (steps after the
     first replay the captured hipGraph), between two stream synchronisations,
     without per-phase events (each event record drains the queue; on the
-    512^2 rotor they cost ~10% of a step).  Then `timing_steps` single steps
+    512^2 rotor they cost ~10% of a step).  Then `timing_steps` steps
     with HIP events on the library's stream give the per-kernel durations the
-    roofline uses."""
+    roofline uses (one sphmhd_step call: all but its last step skip the
+    diagnostic rows, as in the timed region)."""
     import torch
 
     from sphmhd.device import DeviceSystem
@@ -181,8 +182,7 @@ def run_gpu(name: str, steps: int, warmup: int, timing_steps: int = 10):
     wall = time.perf_counter() - t0
     st = dev.stats()
     dev.set_timing(True)
-    for _ in range(timing_steps):
-        dev.step(1)
+    dev.step(timing_steps)  # one call, as in the timed region (per-phase events, no enqueueing ahead)
     dev.synchronize()
     tm = dev.timing()
     pairs = dev.stats()["pairs_force"]
@@ -437,7 +437,7 @@ def main():
                    "step": "PEC step incl. all density iterations", "parallelism": "single GPU"},
         "pair_interactions_per_s": round(head["stats"]["pairs_force"] * args.steps / head["wall_s"], 1),
         "timing_ms": {k: round(v, 4) if isinstance(v, float) else v for k, v in head["timing"].items()},
-        "timing_note": f"HIP-event sums over {head['timing_steps']} single steps run after the timed region",
+        "timing_note": f"HIP-event sums over one call of {head['timing_steps']} steps run after the timed region",
         "graph_replays": head["stats"]["graph_replays"],
         "launch_blocks": {"density": head["launch"][0], "lists": head["launch"][1], "force": head["launch"][2],
                           "note": "workgroup sizes chosen by timing the first steps"},

@@ -23,7 +23,8 @@
  *             not move during an evaluation): c = R h_max / k with
  *             k = min(floor(h_max/h_min), 4) when h_max >= 2 h_min, else 1
  *             (cells split where h varies, so the small-h particles do not
- *             scan cells sized for the largest h); n_a = max(1, floor(L_a/c)),
+ *             scan cells sized for the largest h), and in 3-D k doubled (at
+ *             most 4: half-width cells); n_a = max(1, floor(L_a/c)),
  *             while prod n_a > max(1, 8 N) the split is undone first and c
  *             then doubled; c_eff_a = L_a / n_a over the box: the domain on
  *             periodic axes, else the particles' box at this evaluation
@@ -903,7 +904,8 @@ static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
         }
     }
     double R = orc_kernel_radius(P->kernel);
-    double split = cell_split(hmax, hmin);
+    /* in 3-D at least 2 cells per R h_max (twice the h split, at most 4) */
+    double split = P->dim == 3 ? fmin(2.0 * cell_split(hmax, hmin), 4.0) : cell_split(hmax, hmin);
     double c = R * hmax / split;
     double lmax = 0.0;
     for (int a = 0; a < P->dim; ++a) lmax = dmax(lmax, L[a]);

@@ -80,6 +80,7 @@ struct sphmhd_ctx {
     bool bound = false;
     int max_iter_all = 1;
     int min_iter_all = 1;
+    bool lean_now = false;  // the device's Ctl::lean (see sphmhd_step)
     int spec_passes = 2;  // density passes launched before the first wait
     int spec_fixed = 0;   // SPHMHD_SPEC: a fixed count instead (results do not depend on it)
     int spec_margin = 0;       // one extra pass for a while after a redo
@@ -842,6 +843,36 @@ struct StreamSwap {
     }
 };
 
+// the force pass of the evaluation just finished again, from the kept
+// previous accelerations (same results, and the rows of the current lean
+// setting written)
+static int force_redo(sphmhd_ctx* c) {
+    Dev& d = c->d;
+    launch_clear_abort(d);
+    launch_reset_nb_stats(d);
+    launch_reset_force_outputs(d);
+    int rc = force_pass(c, false);
+    if (rc) return rc;
+    if ((rc = read_ctl(c))) return rc;
+    if (c->hcur->abort) launch_clear_abort(d);
+    c->stats.force_redos += 1;
+    return 0;
+}
+
+// the host's copy of the device's lean flag; set() launches only on change
+// and returns the value set; clears the flag when the step call ends
+struct LeanGuard {
+    sphmhd_ctx* c;
+    bool set(bool v) {
+        if (v != c->lean_now) {
+            launch_set_lean(c->d, v ? 1 : 0);
+            c->lean_now = v;
+        }
+        return v;
+    }
+    ~LeanGuard() { set(false); }
+};
+
 int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
     int rc = check_ready(c);
     if (rc) return rc;
@@ -865,15 +896,23 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
         // host's turn.  If step s needs more density passes or longer lists,
         // its end sets the abort flag and every kernel of step s + 1 exits at
         // entry: the host then finishes step s and enqueues step s + 1 again.
+        // Every step but the last of the call runs lean: its force pass skips
+        // the diagnostic rows no later kernel reads, so the fields at return
+        // are those of plain single steps.  The flag is set before a step's
+        // launches are enqueued and cleared on every way out.
+        LeanGuard lean{c};
         bool ahead = false;  // step s was enqueued during step s - 1
         int ahead_spec = 0;
+        bool step_lean = false, ahead_lean = false;  // the flag step s and step s + 1 were enqueued with
         for (int64_t s = 0; s < nsteps; ++s) {
             // tuning steps are launched one by one with events around the phases
             const bool tuning = c->tune.state <= 1;
             if (tuning && (rc = tune_begin_step(c))) return rc;
             int spec = ahead_spec;
+            step_lean = ahead_lean;
             if (!ahead) {
                 spec = next_spec(c);
+                step_lean = lean.set(s + 1 < nsteps);
                 rc = (!tuning && s > 0 && swap.on && c->use_graphs) ? step_front_graph(c, spec)
                                                                      : step_front(c, s == 0, spec);
                 if (rc < 0) return rc;
@@ -884,6 +923,7 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
                 HIPCHK(hipEventRecord(c->sev, c->d.stream));  // end of step s
                 ahead_spec = next_spec(c);
                 const int64_t replays = c->stats.graph_replays;
+                ahead_lean = lean.set(s + 2 < nsteps);
                 rc = step_front_graph(c, ahead_spec);  // replayed, or launched plainly if a capture failed
                 if (rc < 0) return rc;
                 ahead = true;
@@ -905,8 +945,15 @@ int sphmhd_step(sphmhd_ctx* c, int64_t nsteps) {
             if (tuning) tune_end_step(c);
             c->stats.steps += 1;
             // this step brings every clock to tf: the run ends with it (the
-            // step enqueued after it, if any, exited at entry)
-            if (c->hcur->at_tf) break;
+            // step enqueued after it, if any, exited at entry); a step that
+            // ran lean repeats its force pass to leave the diagnostic rows
+            if (c->hcur->at_tf) {
+                if (step_lean) {
+                    lean.set(false);
+                    if ((rc = force_redo(c)) < 0) return rc;
+                }
+                break;
+            }
         }
         launch_stage2_clock(c->d);  // stage2 of the last step and the clock commit
         HIPCHK(hipGetLastError());

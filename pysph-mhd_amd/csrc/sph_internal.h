@@ -121,6 +121,9 @@ struct Ctl {
     int32_t abort;
     int32_t seq;  // evaluations ended so far (the host mirror's slot is seq % 2)
     int32_t at_tf;  // set by an evaluation's end: its step brings every clock to tf
+    // 1 while a multi-step run is before its last step: the force pass then
+    // skips the diagnostic rows no later kernel reads (set by the host)
+    int32_t lean;
     // list build
     alignas(128) int64_t pairs;
     int32_t max_nb;
@@ -352,6 +355,14 @@ __host__ __device__ inline double cell_split(double hmax, double hmin) {
     return 1.0;
 }
 
+// cells per R h_max along an axis as the grid is built: in 3-D at least 2
+// (twice the h split, at most 4): a 5^3 stencil of half-width cells holds
+// 125/8 = 15.6 cell volumes of candidates against 27 for 3^3 full cells
+__host__ __device__ inline double cell_split_dim(double hmax, double hmin, int dim) {
+    const double k = cell_split(hmax, hmin);
+    return dim == 3 ? ::fmin(2.0 * k, 4.0) : k;
+}
+
 // Grid set-up (one workgroup): one thread per instance, then a serial prefix
 // of cell bases.
 __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
@@ -381,7 +392,7 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         // where h varies by 2x or more the cells are split below R h_max (at
         // most 4 per axis), so the small-h particles do not scan cells sized
         // for the largest h
-        double split = cell_split(hm, hn);
+        double split = cell_split_dim(hm, hn, s.dim);
         double c = R * hm / split;
         double lmax = 0.0;
         for (int a = 0; a < s.dim; ++a) lmax = ::fmax(lmax, s.L[a]);
@@ -413,7 +424,8 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         }
         for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
         s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
-        s.split = (int)split;
+        // above 1 only where h varies (the lists then prune by cell h)
+        s.split = cell_split(hm, hn) > 1.0 ? (int)split : 1;
         stencil_from_hmax(s);
     }
     __syncthreads();
@@ -1081,6 +1093,7 @@ int report_error(int code, const std::string& msg);
 void launch_eval_end(Dev& d);
 void launch_ctl_export(Dev& d);
 void launch_clear_abort(Dev& d);
+void launch_set_lean(Dev& d, int lean);
 
 void launch_initialize(Dev& d);
 void launch_stage1(Dev& d);

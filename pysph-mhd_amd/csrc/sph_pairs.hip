@@ -387,7 +387,8 @@ template <int K, int DIM>
 __device__ __forceinline__ void force_block(Fields F, const double* __restrict__ fpack,
                                             const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                             const int32_t* __restrict__ nbcnt, Inst* inst,
-                                            const int32_t* __restrict__ inst_of, int64_t n, int stride) {
+                                            const int32_t* __restrict__ inst_of, int64_t n, int stride,
+                                            int lean) {
     using R = FRec<DIM>;
     constexpr int U = R::UNITS;
     const double2* __restrict__ pk = (const double2*)fpack;
@@ -681,24 +682,29 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
         if (f2 > 0.0) dt_adapt = fmin(dt_adapt, s.c_force * sqrt(hi / sqrt(f2)));
         bad = !isfinite(f2) || !isfinite(ae);
 
+        // the rates the stages and the next evaluation read
         f[SPHMHD_F_AU][i] = ax;
         f[SPHMHD_F_AV][i] = ay;
         f[SPHMHD_F_AW][i] = az;
         f[SPHMHD_F_AE][i] = ae;
         f[SPHMHD_F_AH][i] = ah;
-        f[SPHMHD_F_DIV][i] = div;
-        f[SPHMHD_F_DIV_FOR_PSI][i] = div;
-        f[SPHMHD_F_DT_CFL][i] = dtcfl;
-        f[SPHMHD_F_DT_ADAPT][i] = dt_adapt;
         f[SPHMHD_F_ABEVOLX][i] = A.bx;
         f[SPHMHD_F_ABEVOLY][i] = A.by;
         f[SPHMHD_F_ABEVOLZ][i] = A.bz;
         f[SPHMHD_F_APSI][i] = apsi;
-        f[SPHMHD_F_DIVBSYMM][i] = rhoi * divb;
-        f[SPHMHD_F_DIVBDIFF][i] = A.divbd;
         f[SPHMHD_F_ALPHALOC][i] = alphaloc;
-        f[SPHMHD_F_XILIMITER][i] = xil;
-        f[SPHMHD_F_DIVVDT][i] = divvdt;
+        // the reference's diagnostics, which no later kernel reads: skipped
+        // in the evaluations of a multi-step run that are not its last
+        if (!lean) {
+            f[SPHMHD_F_DIV][i] = div;
+            f[SPHMHD_F_DIV_FOR_PSI][i] = div;
+            f[SPHMHD_F_DT_CFL][i] = dtcfl;
+            f[SPHMHD_F_DT_ADAPT][i] = dt_adapt;
+            f[SPHMHD_F_DIVBSYMM][i] = rhoi * divb;
+            f[SPHMHD_F_DIVBDIFF][i] = A.divbd;
+            f[SPHMHD_F_XILIMITER][i] = xil;
+            f[SPHMHD_F_DIVVDT][i] = divvdt;
+        }
     }
 
     // min dt_adapt and the NaN flag per instance: wave, then block, then one
@@ -795,6 +801,8 @@ __global__ void k_ctl_export(const Ctl* ctl, CtlView* hout) { hout[2] = ctl_view
 
 __global__ void k_clear_abort(Ctl* ctl) { ctl->abort = 0; }
 
+__global__ void k_set_lean(Ctl* ctl, int v) { ctl->lean = v; }
+
 // the fused force pass; its last workgroup ends the evaluation
 template <int K, int DIM>
 __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
@@ -803,7 +811,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride,
                                                   int ninst, Ctl* ctl, CtlView* hout, int32_t* arrive) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
-    force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride);
+    force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride, ctl->lean);
     if (last_to_arrive(arrive)) eval_end_body(inst, ninst, ctl, hout);
 }
 
@@ -865,6 +873,8 @@ void launch_ctl_export(Dev& d) {
 }
 
 void launch_clear_abort(Dev& d) { hipLaunchKernelGGL(k_clear_abort, dim3(1), dim3(1), 0, d.stream, d.ctl); }
+
+void launch_set_lean(Dev& d, int lean) { hipLaunchKernelGGL(k_set_lean, dim3(1), dim3(1), 0, d.stream, d.ctl, lean); }
 
 void launch_force(Dev& d) {
     if (d.n == 0) return;

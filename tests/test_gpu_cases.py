@@ -185,16 +185,16 @@ def test_reference_case_100_steps(gpu, name, nx):
 
 @pytest.mark.parametrize("name,nx", [("jadvect", 32), ("mhdvortex", 24), ("mhdrotor", 24)])
 def test_reference_case_cells_and_neighbours_bit_exact(gpu, name, nx):
-    """Crowded cells (jadvect and mhdvortex put 60-200 particles in a cell:
-    rank-ordered and bitonic-sorted buckets) and split cells (the rotor's
-    disk): keys, permutation and neighbour lists bit for bit."""
+    """Large-h 3-D slabs (jadvect hfact 1.9, mhdvortex 3.6: 12-33 particles
+    per half-width cell) and split cells (the rotor's disk): keys,
+    permutation and neighbour lists bit for bit.  Buckets past the ranked
+    size (bitonic sorts in LDS and in global memory) are exercised by
+    test_gpu_parity.test_cells_crowded_bit_exact."""
     s = case_setup(name, nx)
     dev, orc = _pair(s)
     kd, pd = dev.cells()
     ko, po = orc.cells()
     assert np.array_equal(kd, ko) and np.array_equal(pd, po)
-    counts = np.bincount(ko.astype(np.int64))
-    assert counts.max() > 64 or name == "mhdrotor"
     od, idd = dev.neighbours()
     oo, io = orc.neighbours()
     assert np.array_equal(od, oo) and np.array_equal(idd, io)

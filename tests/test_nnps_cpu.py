@@ -85,7 +85,8 @@ def _direct_keys(s, hmax_by_inst):
         a, b = int(s.offsets[k]), int(s.offsets[k + 1])
         nk = b - a
         cap = max(1, 2 * nk)
-        c = kernel_radius(p.kernel) * hmax_by_inst[k]
+        # cells of R h_max, halved in 3-D (uniform h: no further split)
+        c = kernel_radius(p.kernel) * hmax_by_inst[k] / (2.0 if p.dim == 3 else 1.0)
         while True:
             n = [max(1, int(np.floor((p.hi[ax] - p.lo[ax]) / c))) if ax < p.dim else 1 for ax in range(3)]
             if n[0] * n[1] * n[2] <= cap:
