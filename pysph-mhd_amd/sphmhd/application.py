@@ -113,10 +113,12 @@ class MHDApplication(Application):
         """Accelerations of the initial state and the first time step, then
         tell the solver not to evaluate them again (solver.py:35-38)."""
         system = self.evaluator.system
+        self.evaluator.check_periodic_images()
         if self._refreshes_thermodynamics():
             self._warm_start_with_hook(system)
         else:
             system.warm_start()
+        self.evaluator.check_periodic_images()  # with the summed h
         particle = self.particles[0]
         particle.gpu.pull("rho", "h", "Bx", "By", "Bz", "converged", "dt_cfl", "au", "av", "aw")
         self.solver.dt = initial_time_step(particle.h, particle.dt_cfl, particle.au, particle.av, particle.aw,

@@ -131,6 +131,10 @@ def plan_params(pa, groups, kernel, domain, scheme=None, solver=None):
     return p, plan
 
 
+class GhostImagesError(RuntimeError):
+    pass
+
+
 class HIPAccelerationEval:
     def __init__(self, pa, groups, kernel, domain, scheme=None, solver=None):
         from .device import DeviceSystem
@@ -138,12 +142,36 @@ class HIPAccelerationEval:
         self.pa = pa
         self.groups = groups
         self.params, self.plan = plan_params(pa, groups, kernel, domain, scheme, solver)
+        # periodic_mode "ghost" (config.py:99; application.py:63-78 passes it
+        # to the DomainManager) runs as the minimum image: the two give the
+        # same neighbours, with XIJ differing in rounding only, while every
+        # periodic length exceeds twice the support radius.  Below that a
+        # ghost layer holds several images of one particle, which this build
+        # does not enumerate, so such a state is refused.
+        self.ghost = bool(domain is not None and getattr(domain, "periodic_mode", None) == "ghost")
+        self.radius = kernel.radius_scale if kernel is not None else 2.0
         self.density_eq = groups[0].equations[0]
         self.system = DeviceSystem(self.params, n=len(pa))
         pa.gpu.attach(self.system)
         pa.gpu.push(*[n for n in FIELDS if n in pa.properties], "itype")
 
+    def check_periodic_images(self) -> None:
+        """Raise GhostImagesError when ghost periodicity would need more than
+        one image of a particle (a periodic length <= 2 R h_max)."""
+        if not self.ghost:
+            return
+        hmax = float(self.system.field("h").max())
+        p = self.params
+        for a in range(p.dim):
+            length = p.hi[a] - p.lo[a]
+            if not length > 2.0 * self.radius * hmax:
+                raise GhostImagesError(
+                    f"periodic_mode 'ghost': axis {a} is {length:g} long, not more than twice the support "
+                    f"radius {self.radius * hmax:g}, so a ghost layer would hold several images of a particle; "
+                    "this build runs ghost periodicity as the minimum image, which equals it only below that")
+
     def initial_acceleration(self) -> int:
+        self.check_periodic_images()
         rc = self.system.eval()
         conv = self.system.get("converged")
         self.density_eq.equation_has_converged = 1 if np.all(conv == 1.0) else -1
@@ -156,6 +184,7 @@ class HIPAccelerationEval:
         """Up to nsteps PEC steps starting from the host clock (t, dt); the
         device stops after the step that reaches tf.  Returns the new (t, dt)
         from the device clock and the number of steps taken."""
+        self.check_periodic_images()
         self.system.set_clock(t, dt)
         before = self.system.steps_taken()
         self.system.step(nsteps)
