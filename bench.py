@@ -43,10 +43,11 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, vector and matrix alike
 # apsi, alphaloc, dt_adapt (11 fields, 88 B).
 FORCE_BYTES_PER_PARTICLE = {2: 248.0, 3: 256.0}
 
-# The first steps of a context time each workgroup size of the density, list
-# and force launches and keep the fastest (results do not depend on it); the
-# bench runs them before the warm-up steps, outside the timed region.
-TUNE_STEPS = 6
+# The first steps of a context time each workgroup size and launch mode
+# (plain or persistent) of the density, list and force launches and keep the
+# fastest (results do not depend on it); the bench runs them before the
+# warm-up steps, outside the timed region.
+TUNE_STEPS = 12
 
 # Measured HBM traffic of the force kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
 # per dispatch, scaled by a plain copy of known size), kept in the profile
@@ -199,6 +200,7 @@ def run_gpu(name: str, steps: int, warmup: int, timing_steps: int = 10):
         "timing_steps": timing_steps,
         "stats": st,
         "launch": dev.launch()[0],
+        "persistent": [st["persist_density"], st["persist_lists"], st["persist_force"]],
         "pairs_timing": pairs,
     }
     dev.close()
@@ -412,7 +414,14 @@ def main():
                 "ms_per_step": round(r["ms_per_step"], 4),
                 "pair_interactions_per_s": round(r["stats"]["pairs_force"] * steps / r["wall_s"], 1),
                 "launch_blocks": list(r["launch"]),
+                "persistent": r["persistent"],
                 "roofline": roofline(r),
+            }
+        if "ensemble64" in extras and "ot1m" in extras:
+            extras["ensemble_vs_plain_equal_n"] = {
+                "ratio": round(extras["ensemble64"]["particle_steps_per_s"] / extras["ot1m"]["particle_steps_per_s"], 3),
+                "what": "particle-steps/s of the 64-instance uneven ensemble (1.89M particles) over one plain "
+                        "Orszag-Tang instance of 1M particles",
             }
         if "ot128" in extras and "ot128x64" in extras:
             extras["batched_vs_plain"] = {
@@ -440,6 +449,7 @@ def main():
         "timing_note": f"HIP-event sums over one call of {head['timing_steps']} steps run after the timed region",
         "graph_replays": head["stats"]["graph_replays"],
         "launch_blocks": {"density": head["launch"][0], "lists": head["launch"][1], "force": head["launch"][2],
+                          "persistent": head["persistent"],
                           "note": "workgroup sizes chosen by timing the first steps"},
         "roofline": roofline(head),
         "extra_workloads": extras,

@@ -83,8 +83,8 @@ typedef struct sphmhd_params {
 
 /* Bound particle properties.  The names are the reference's property names
  * (particles.py:1-162).  Properties the reference writes but never reads
- * again (grho*, arho, rho_sum, dwdh, dt_force, cd_*, dvdx_*) are not part of
- * the device path. */
+ * again (grho*, arho, rho_sum, dwdh, dt_force, cd_*) are not part of
+ * the device path; dvdx_* come on request (sphmhd_bind_strain). */
 typedef enum sphmhd_field {
     SPHMHD_F_X = 0, SPHMHD_F_Y, SPHMHD_F_Z,
     SPHMHD_F_U, SPHMHD_F_V, SPHMHD_F_W,
@@ -135,6 +135,10 @@ typedef struct sphmhd_stats {
     int32_t block_force;         /* workgroup size of the force pass */
     int32_t launch_tuned;        /* 1 once the launch shapes were chosen by timing */
     int64_t steps;               /* PEC steps completed by sphmhd_step / sphmhd_tangent_step */
+    int32_t persist_density;     /* 1: the density passes run persistent (work queues), 0: plain */
+    int32_t persist_lists;       /* the same for the neighbour-list build */
+    int32_t persist_force;       /* the same for the force pass */
+    int32_t reserved;
 } sphmhd_stats;
 
 /* Device time of the hot kernels, measured with HIP events recorded on the
@@ -169,6 +173,14 @@ int sphmhd_destroy(sphmhd_ctx* ctx);
  * when dim == 2) and itype (int32, particles.py:7).  No copies are made.
  * Replaces ParticleArray / pa.gpu property binding. */
 int sphmhd_bind(sphmhd_ctx* ctx, double* const* fields, int32_t* itype);
+
+/* StrainDiagnostics (equations.py:432-471, group G4 of scheme.py:75): the
+ * velocity gradient dvdx_ab = d v_a / d x_b of the Cullen-Dehnen sums.  No
+ * later group reads it, so it is produced only on request: bind 9 device
+ * FP64 rows (dvdx_xx, dvdx_xy, dvdx_xz, dvdx_yx, ..., dvdx_zz, storage order,
+ * borrowed) and every evaluation that writes the diagnostic rows (all but the
+ * non-final steps of a multi-step sphmhd_step call) fills them; NULL unbinds. */
+int sphmhd_bind_strain(sphmhd_ctx* ctx, double* const* dvdx);
 
 /* One force evaluation: neighbour-search update, then groups G0..G9 of
  * scheme.py:70-81 (density iteration to convergence, EOS, Cullen-Dehnen
@@ -270,6 +282,15 @@ int sphmhd_hcp_lattice(const double* bounds, double spacing, double* x, double* 
  * again.  The environment variable SPHMHD_TUNE=0 disables tuning (all 256). */
 int sphmhd_set_launch(sphmhd_ctx* ctx, const int32_t* blocks);
 int sphmhd_get_launch(sphmhd_ctx* ctx, int32_t* blocks, int32_t* tuned);
+
+/* Launch modes of the same three launch classes.  No counterpart in the
+ * reference.  Plain: one workgroup per tile of sorted particles.  Persistent:
+ * as many workgroups as the GPU holds at once, each pulling tiles (of any
+ * instance of the ensemble) from per-XCD work queues until all are empty.
+ * modes[3] = 0 or 1 fixes them; NULL lets the tuner choose them with the
+ * workgroup sizes (the default; SPHMHD_PERSIST=0/1 fixes every class).  The
+ * results never depend on the mode; sphmhd_stats reports the modes in use. */
+int sphmhd_set_persistent(sphmhd_ctx* ctx, const int32_t* modes);
 
 /* ---- Tangent-linear step -------------------------------------------------
  * No counterpart in the reference, which has no derivatives; it serves the

@@ -81,6 +81,7 @@ struct sphmhd_ctx {
     int max_iter_all = 1;
     int min_iter_all = 1;
     bool lean_now = false;  // the device's Ctl::lean (see sphmhd_step)
+    bool persist_fixed = false;  // launch modes set by the caller or SPHMHD_PERSIST, not tuned
     int spec_passes = 2;  // density passes launched before the first wait
     int spec_fixed = 0;   // SPHMHD_SPEC: a fixed count instead (results do not depend on it)
     int spec_margin = 0;       // one extra pass for a while after a redo
@@ -120,12 +121,15 @@ struct sphmhd_ctx {
     TanDev tg;
     bool tbound = false;
     // launch-shape tuner: the first kTuneSteps steps of sphmhd_step rotate
-    // the workgroup size of the tuned launches through kShapes (the results
-    // do not depend on it) and time each launch class with HIP events
+    // the tuned launches through the configurations (workgroup size, which is
+    // also the tile of particles a work item covers, and plain or persistent
+    // launch; the results do not depend on either) and time each launch class
+    // with HIP events
     struct Tuner {
         int state = 0;  // 0: not started, 1: running, 2: tuned, 3: fixed (caller or SPHMHD_TUNE=0)
         int step = 0;
-        double best_ms[kLaunchClasses][3];
+        double best_ms[kLaunchClasses][6];   // fastest sample per class and configuration
+        double worst_ms[kLaunchClasses][6];  // slowest sample
         bool saved_timing = false;
         double saved_ms[5];
         int64_t saved_count[5];
@@ -133,10 +137,16 @@ struct sphmhd_ctx {
 };
 
 static const int kShapes[3] = {64, 128, 256};
+// configuration c: workgroup size kShapes[c % 3], persistent when c >= 3;
+// only the first three (plain) when the launch mode is fixed
+constexpr int kTuneConfigs = 6;
 constexpr int kTuneRounds = 2;
-constexpr int kTuneSteps = 3 * kTuneRounds;
+constexpr int kTuneSteps = kTuneConfigs * kTuneRounds;
+static int tune_configs(const sphmhd_ctx* c);
 
 enum { T_FORCE = 0, T_DENSITY, T_NNPS, T_LISTS, T_STAGE };
+
+static int tune_configs(const sphmhd_ctx* c) { return c->persist_fixed ? 3 : kTuneConfigs; }
 
 // Inside a stream capture the records become event nodes of the graph
 // (hipEventRecordExternal), re-recorded at every replay.
@@ -190,14 +200,19 @@ static int tune_begin_step(sphmhd_ctx* c) {
         T.step = 0;
         for (auto& row : T.best_ms)
             for (double& v : row) v = 1.0e300;
+        for (auto& row : T.worst_ms)
+            for (double& v : row) v = 0.0;
         T.saved_timing = c->timing;
         for (int k = 0; k < 5; ++k) {
             T.saved_ms[k] = c->tm[k].ms;
             T.saved_count[k] = c->tm[k].count;
         }
     }
-    const int b = kShapes[T.step % 3];
-    for (int q = 0; q < kLaunchClasses; ++q) c->d.blk[q] = b;
+    const int cfg = T.step % tune_configs(c);
+    for (int q = 0; q < kLaunchClasses; ++q) {
+        c->d.blk[q] = kShapes[cfg % 3];
+        if (!c->persist_fixed) c->d.persist[q] = cfg >= 3 ? 1 : 0;
+    }
     c->timing = true;
     for (PhaseTimer& t : c->tm) {
         t.ms = 0.0;
@@ -213,18 +228,26 @@ static int tune_begin_step(sphmhd_ctx* c) {
 // the caller back its timing mode and sums
 static void tune_end_step(sphmhd_ctx* c) {
     auto& T = c->tune;
-    const int s = T.step % 3;
+    const int ncfg = tune_configs(c);
+    const int s = T.step % ncfg;
     const int timer[kLaunchClasses] = {T_DENSITY, T_LISTS, T_FORCE};
     for (int q = 0; q < kLaunchClasses; ++q) {
         const PhaseTimer& t = c->tm[timer[q]];
-        if (t.count > 0) T.best_ms[q][s] = std::min(T.best_ms[q][s], t.ms);
+        if (t.count > 0) {
+            T.best_ms[q][s] = std::min(T.best_ms[q][s], t.ms);
+            T.worst_ms[q][s] = std::max(T.worst_ms[q][s], t.ms);
+        }
     }
-    if (++T.step < kTuneSteps) return;
+    if (++T.step < ncfg * kTuneRounds) return;
+    // the default (256, plain) unless another configuration's slowest sample
+    // beats the default's fastest by 5% (the early steps the samples come
+    // from differ from each other by a few percent)
     for (int q = 0; q < kLaunchClasses; ++q) {
         int best = 2;
-        for (int k = 1; k >= 0; --k)
-            if (T.best_ms[q][k] < 0.97 * T.best_ms[q][best]) best = k;
-        c->d.blk[q] = kShapes[best];
+        for (int k = 0; k < ncfg; ++k)
+            if (k != 2 && T.worst_ms[q][k] < 0.95 * (best == 2 ? T.best_ms[q][2] : T.worst_ms[q][best])) best = k;
+        c->d.blk[q] = kShapes[best % 3];
+        if (!c->persist_fixed) c->d.persist[q] = best >= 3 ? 1 : 0;
     }
     T.state = 2;
     c->timing = T.saved_timing;
@@ -638,6 +661,11 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         d.prune_lists = c->prune_mode == 1;
         if (const char* g = std::getenv("SPHMHD_TUNE"))
             if (std::strcmp(g, "0") == 0) c->tune.state = 3;
+        // SPHMHD_PERSIST=1 / 0: every tuned launch persistent / plain (not tuned)
+        if (const char* g = std::getenv("SPHMHD_PERSIST")) {
+            c->persist_fixed = true;
+            for (int q = 0; q < kLaunchClasses; ++q) d.persist[q] = std::strcmp(g, "0") != 0;
+        }
         int64_t cap = 0;
         for (int k = 0; k < n_instances; ++k) {
             int64_t nk = offsets[k + 1] - offsets[k];
@@ -699,6 +727,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.inst, n_instances))) return cleanup(rc);
         if ((rc = dalloc(&d.ctl, 1))) return cleanup(rc);
         if ((rc = dalloc(&d.arrive, kArriveInts))) return cleanup(rc);
+        if ((rc = dalloc(&d.queue, kQueueInts))) return cleanup(rc);
         if ((rc = dalloc(&d.hblk, kRed * ((n + kBlock - 1) / kBlock)))) return cleanup(rc);
         if ((rc = dalloc(&d.kblk, (n + kBlock - 1) / kBlock))) return cleanup(rc);
         if ((rc = dalloc(&d.inst_of, n))) return cleanup(rc);
@@ -761,7 +790,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
     if (c->sev) (void)hipEventDestroy(c->sev);
-    void* ptrs[] = {d.inst, d.ctl, d.arrive, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
+    void* ptrs[] = {d.inst, d.ctl, d.arrive, d.queue, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.hcell, d.cls, d.cle, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
                     d.act[1]};
     for (void* p : ptrs)
@@ -785,6 +814,19 @@ int sphmhd_bind(sphmhd_ctx* c, double* const* fields, int32_t* itype) {
     c->d.fields.itype = itype;
     c->bound = true;
     c->graph_gen += 1;
+    return 0;
+}
+
+int sphmhd_bind_strain(sphmhd_ctx* c, double* const* dvdx) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    for (int q = 0; q < 9; ++q) c->d.strain.d[q] = dvdx ? dvdx[q] : nullptr;
+    if (dvdx)
+        for (int q = 0; q < 9; ++q)
+            if (!dvdx[q]) {
+                for (int r = 0; r < 9; ++r) c->d.strain.d[r] = nullptr;
+                return fail(SPHMHD_EINVAL, "strain row " + std::to_string(q) + " is null");
+            }
+    c->graph_gen += 1;  // captured steps hold the force pass's arguments
     return 0;
 }
 
@@ -1108,6 +1150,25 @@ int sphmhd_get_stats(sphmhd_ctx* c, sphmhd_stats* out) {
     out->block_lists = c->d.blk[LC_LISTS];
     out->block_force = c->d.blk[LC_FORCE];
     out->launch_tuned = c->tune.state == 2 ? 1 : 0;
+    out->persist_density = c->d.persist[LC_DENSITY];
+    out->persist_lists = c->d.persist[LC_LISTS];
+    out->persist_force = c->d.persist[LC_FORCE];
+    return 0;
+}
+
+int sphmhd_set_persistent(sphmhd_ctx* c, const int32_t* modes) {
+    if (!c) return fail(SPHMHD_EINVAL, "null context");
+    if (c->tune.state == 1) return fail(SPHMHD_ESTATE, "launch shapes are being tuned");
+    if (modes) {
+        for (int q = 0; q < kLaunchClasses; ++q)
+            if (modes[q] != 0 && modes[q] != 1) return fail(SPHMHD_EINVAL, "launch modes must be 0 or 1");
+        for (int q = 0; q < kLaunchClasses; ++q) c->d.persist[q] = modes[q];
+        c->persist_fixed = true;
+    } else {
+        c->persist_fixed = false;
+        c->tune.state = 0;  // tuned again with the workgroup sizes
+    }
+    c->graph_gen += 1;
     return 0;
 }
 

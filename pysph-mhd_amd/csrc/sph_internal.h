@@ -168,6 +168,12 @@ struct Fields {
     int32_t* itype;
 };
 
+// the rows StrainDiagnostics fills (dvdx_xx, _xy, _xz, _yx, ..., _zz), all
+// null unless bound by sphmhd_bind_strain
+struct Strain {
+    double* d[9];
+};
+
 // ---------------------------------------------------------------------------
 // exact helpers (must round exactly like the CPU oracle)
 // ---------------------------------------------------------------------------
@@ -543,6 +549,57 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t tiles) {
     const int64_t per = (tiles + kXcds - 1) / kXcds;
     const int64_t b = blockIdx.x;
     return (b % kXcds) * per + b / kXcds;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent launches of the pair kernels
+// ---------------------------------------------------------------------------
+// A pair kernel covers a sequence of tiles, tile q being the B consecutive
+// sorted particles (or active-list entries) a block of B threads owns; a
+// tile may hold the end of one instance and the start of the next, and every
+// particle is evaluated with its own instance's parameters, so the uneven
+// instances of an ensemble share the launches.  A plain launch has one block
+// per tile (xcd_tile).  A persistent launch has as many blocks as the GPU
+// holds resident at once, and each pulls tiles from work queues until all are
+// empty: first the queue of its XCD (the contiguous eighth of the tiles
+// xcd_tile would give that XCD, so neighbouring tiles still share an L2),
+// then the others in turn.  The queue heads (kXcds counters on lines of their
+// own) are zeroed by the launch's last workgroup, so a persistent launch
+// needs no clearing launch before it and replays from a captured graph.
+constexpr int kQueueInts = kXcds * kArriveStride;
+
+// the block's next tile, the same in every thread; -1 once every queue is
+// empty.  polls (thread 0's, start 0): queues found empty so far.
+__device__ __forceinline__ int64_t pull_tile(int32_t* queue, int64_t tiles, int& polls) {
+    __shared__ int64_t s_tile;
+    if (threadIdx.x == 0) {
+        const int64_t per = (tiles + kXcds - 1) / kXcds;
+        const int home = (int)(blockIdx.x % kXcds);
+        int64_t got = -1;
+        while (polls < kXcds) {
+            const int x = (home + polls) % kXcds;
+            const int64_t lo = (int64_t)x * per, hi = tiles < lo + per ? tiles : lo + per;
+            if (lo < hi) {
+                const int64_t k = atomicAdd(&queue[x * kArriveStride], 1);
+                if (lo + k < hi) {
+                    got = lo + k;
+                    break;
+                }
+            }
+            ++polls;  // this queue is empty
+        }
+        s_tile = got;
+    }
+    __syncthreads();
+    const int64_t t = s_tile;
+    __syncthreads();  // s_tile is written again by the next call
+    return t;
+}
+
+// by the last workgroup of a persistent launch, every block having pulled
+// its last tile
+__device__ __forceinline__ void reset_queues(int32_t* queue) {
+    if (threadIdx.x < kXcds) atomicExch(&queue[threadIdx.x * kArriveStride], 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,6 +1063,8 @@ struct Dev {
     unsigned long long* hblk;  // per block of the fused stage kernel: its kRed maxima (instance in kblk)
     int32_t* kblk;          // that block's instance, or -1 when it straddles instances
     int32_t blk[3];         // workgroup size of the density, list (prep + build) and force launches
+    int32_t persist[3];     // 1: that launch class runs persistent (work queues), 0: one block per tile
+    int32_t* queue;         // kQueueInts: the work-queue heads of persistent launches (zero between launches)
     int32_t* inst_of;       // storage index -> instance (== sorted position -> instance)
     uint32_t *keys, *keys_alt;
     int32_t *perm, *perm_alt;
@@ -1027,6 +1086,7 @@ struct Dev {
     double* totals;         // ninst x SPHMHD_NTOTALS
     double* tot_scratch;    // this device's copy of the static totals scratch (sph_totals_static.hip)
     Fields fields;
+    Strain strain;          // optional velocity-gradient rows (sphmhd_bind_strain)
 };
 
 // sph_totals_static.hip: resolves the current device's copy of its static
@@ -1089,6 +1149,11 @@ void launch_nb_build(Dev& d, bool cell_h = false);
 void launch_force(Dev& d);
 // sets the message sphmhd_last_error() returns; returns code
 int report_error(int code, const std::string& msg);
+
+// grid of a persistent launch: the blocks of `fn` (B threads, lds bytes of
+// dynamic LDS) the GPU holds at once, a multiple of kXcds, at most the
+// plain grid of `tiles` tiles
+unsigned persistent_grid(const void* fn, int B, size_t lds, int64_t tiles);
 
 void launch_eval_end(Dev& d);
 void launch_ctl_export(Dev& d);

@@ -389,21 +389,20 @@ __global__ void k_reset_nb_stats(Ctl* ctl) {
 // (measured faster than per-thread rows, whose lines do not stay in L1)
 // ---------------------------------------------------------------------------
 
+// one tile of B sorted particles starting at t0
 template <int DIM, bool PRUNE>
-__global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
-                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
-                                                     const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
-                                                     int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
-                                                     int stride, Ctl* ctl, int cap,
-                                                     const unsigned long long* __restrict__ hcell) {
-    if (ctl->abort) return;
+__device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
+                                         const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                                         const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
+                                         int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
+                                         int stride, Ctl* ctl, int cap, const unsigned long long* __restrict__ hcell,
+                                         int64_t t0) {
     // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
     // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
     // rarely fit and the LDS would cost occupancy (cap 0)
     __shared__ StageHdr H;
     extern __shared__ double4 stage[];
     const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
-    const int64_t t0 = xcd_tile((n + B - 1) / B) * B;
     if (t0 >= n) return;  // whole block past the end (the block barriers below need every wave)
     if constexpr (DIM == 2) {
         if (threadIdx.x < kWave) stage_plan<DIM, kWave>(H, skey, cstart, cend, inst, inst_of, t0, B, n, cap);
@@ -482,6 +481,28 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
     }
 }
 
+template <int DIM, bool PRUNE>
+__global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
+                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
+                                                     const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
+                                                     int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
+                                                     int stride, Ctl* ctl, int cap,
+                                                     const unsigned long long* __restrict__ hcell, int32_t* queue,
+                                                     int32_t* arrive) {
+    if (ctl->abort) return;
+    const int B = blockDim.x;
+    if (queue) {  // persistent: tiles from the work queues, then the last block resets them
+        int polls = 0;
+        for (int64_t tile; (tile = pull_tile(queue, (n + B - 1) / B, polls)) >= 0;)
+            nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
+                                 tile * B);
+        if (last_to_arrive(arrive)) reset_queues(queue);
+    } else {
+        nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
+                             xcd_tile((n + B - 1) / B) * B);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -551,12 +572,15 @@ void launch_pack_density(Dev& d) {
 template <int DIM, bool PRUNE>
 static void nb_build(Dev& d, const unsigned long long* hc) {
     const int B = d.blk[LC_LISTS];
-    const unsigned grid = xcd_grid((d.n + B - 1) / B);
     // 2-D blocks stage candidate runs in LDS, cap records sized with the block
     const int cap = DIM == 2 ? kStageCap * B / kBlock : 0;
+    int32_t* q = d.persist[LC_LISTS] ? d.queue : nullptr;
+    const unsigned grid = q ? persistent_grid((const void*)k_nb_build<DIM, PRUNE>, B, sizeof(double4) * cap,
+                                              (d.n + B - 1) / B)
+                            : xcd_grid((d.n + B - 1) / B);
     hipLaunchKernelGGL((k_nb_build<DIM, PRUNE>), dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
                        (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
-                       d.nb_stride, d.ctl, cap, hc);
+                       d.nb_stride, d.ctl, cap, hc, q, d.arrive);
 }
 
 void launch_nb_build(Dev& d, bool cell_h) {

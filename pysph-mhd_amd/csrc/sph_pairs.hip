@@ -13,6 +13,10 @@
 // a) is removed by packing the previous a into the force pack before the loop.
 // One thread per destination in cell order; FP64 throughout; no atomics on
 // particle data (gather form), so results are deterministic.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "sph_internal.h"
@@ -39,9 +43,9 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
                                               const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                               Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                               const int32_t* __restrict__ act_in, const int32_t* cnt_in,
-                                              int32_t* __restrict__ act_out, int32_t* cnt_out) {
+                                              int32_t* __restrict__ act_out, int32_t* cnt_out, int64_t tile) {
     const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
-    const int64_t idx0 = xcd_tile((n + B - 1) / B) * B;
+    const int64_t idx0 = tile * B;
     const int64_t items = act_in ? (int64_t)*cnt_in : n;
     if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
     const int64_t idx = idx0 + threadIdx.x;
@@ -248,10 +252,23 @@ __global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __r
                                                     Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                                     const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                     int32_t* __restrict__ act_out, int32_t* cnt_out, Ctl* ctl,
-                                                    int ninst, int next_list, int32_t* arrive) {
+                                                    int ninst, int next_list, int32_t* arrive, int32_t* queue) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
-    density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out, cnt_out);
-    if (last_to_arrive(arrive)) g0_control_body(inst, ninst, ctl, next_list);
+    const int B = blockDim.x;
+    if (queue) {  // persistent: tiles of this pass's items from the work queues
+        const int64_t items = act_in ? (int64_t)*cnt_in : n;
+        int polls = 0;
+        for (int64_t tile; (tile = pull_tile(queue, (items + B - 1) / B, polls)) >= 0;)
+            density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out,
+                                  cnt_out, tile);
+    } else {
+        density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out, cnt_out,
+                              xcd_tile((n + B - 1) / B));
+    }
+    if (last_to_arrive(arrive)) {
+        g0_control_body(inst, ninst, ctl, next_list);
+        if (queue) reset_queues(queue);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -388,13 +405,13 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
                                             const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                             const int32_t* __restrict__ nbcnt, Inst* inst,
                                             const int32_t* __restrict__ inst_of, int64_t n, int stride,
-                                            int lean) {
+                                            int lean, Strain strain, int64_t tile) {
     using R = FRec<DIM>;
     constexpr int U = R::UNITS;
     const double2* __restrict__ pk = (const double2*)fpack;
 
     const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
-    const int64_t t0 = xcd_tile((n + B - 1) / B) * B;
+    const int64_t t0 = tile * B;
     if (t0 >= n) return;  // whole block past the end (the block barrier below needs every wave)
     const int64_t t = t0 + threadIdx.x;
     const bool valid = t < n;
@@ -643,6 +660,14 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
         }
         const double divvdt = div_a - (dvxdx * dvxdx + dvydy * dvydy + dvzdz * dvzdz +
                                        2.0 * (dvxdy * dvydx + dvxdz * dvzdx + dvydz * dvzdy));
+        // StrainDiagnostics (equations.py:432-471): the same velocity
+        // gradient from the same sums, on request (rows bound) and with the
+        // other diagnostics
+        if (!lean && strain.d[0]) {
+            strain.d[0][i] = dvxdx; strain.d[1][i] = dvxdy; strain.d[2][i] = dvxdz;
+            strain.d[3][i] = dvydx; strain.d[4][i] = dvydy; strain.d[5][i] = dvydz;
+            strain.d[6][i] = dvzdx; strain.d[7][i] = dvzdy; strain.d[8][i] = dvzdz;
+        }
         const double divv = dvxdx + dvydy + dvzdz;
         const double cvx = dvzdy - dvydz, cvy = dvxdz - dvzdx, cvz = dvydx - dvxdy;
         const double comp = fmax(-divv, 0.0);
@@ -809,10 +834,23 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride,
-                                                  int ninst, Ctl* ctl, CtlView* hout, int32_t* arrive) {
+                                                  int ninst, Ctl* ctl, CtlView* hout, int32_t* arrive,
+                                                  Strain strain, int32_t* queue) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
-    force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride, ctl->lean);
-    if (last_to_arrive(arrive)) eval_end_body(inst, ninst, ctl, hout);
+    const int B = blockDim.x;
+    const int lean = ctl->lean;
+    if (queue) {  // persistent: tiles from the work queues
+        int polls = 0;
+        for (int64_t tile; (tile = pull_tile(queue, (n + B - 1) / B, polls)) >= 0;)
+            force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride, lean, strain, tile);
+    } else {
+        force_block<K, DIM>(F, fpack, perm, nb, nbcnt, inst, inst_of, n, stride, lean, strain,
+                            xcd_tile((n + B - 1) / B));
+    }
+    if (last_to_arrive(arrive)) {
+        eval_end_body(inst, ninst, ctl, hout);
+        if (queue) reset_queues(queue);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +858,34 @@ __global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __rest
 // ---------------------------------------------------------------------------
 
 static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+unsigned persistent_grid(const void* fn, int B, size_t lds, int64_t tiles) {
+    // resident blocks per CU of each (kernel, shape), queried once
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> per_cu;
+    static int cus = 0;
+    int blocks = 1;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (cus == 0) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+                cus = 256;
+        }
+        const auto key = std::make_tuple(fn, B, lds);
+        auto it = per_cu.find(key);
+        if (it == per_cu.end()) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, B, lds) != hipSuccess || nb < 1) nb = 1;
+            it = per_cu.emplace(key, nb).first;
+        }
+        blocks = it->second;
+    }
+    int64_t g = (int64_t)blocks * cus;
+    g = std::max<int64_t>(kXcds, g / kXcds * kXcds);
+    return (unsigned)std::min<int64_t>(g, xcd_grid(tiles));
+}
 
 #define SPH_DISPATCH(KID, DIMV, CALL)                                                    \
     switch (KID) {                                                                       \
@@ -843,10 +909,13 @@ void launch_density(Dev& d) {
     int32_t* aout = d.act[p & 1];
     int32_t* cout = act_count(d.ctl, p & 1);
     const int B = d.blk[LC_DENSITY];
+    int32_t* q = d.persist[LC_DENSITY] ? d.queue : nullptr;
 #define CALL_DENS(KK, DD)                                                                                   \
-    { hipLaunchKernelGGL((k_density<KK, DD>), dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields,   \
+    { const unsigned g = q ? persistent_grid((const void*)k_density<KK, DD>, B, 0, nblk(d.n, B))            \
+                           : xcd_grid(nblk(d.n, B));                                                        \
+      hipLaunchKernelGGL((k_density<KK, DD>), dim3(g), dim3(B), 0, d.stream, d.fields,                      \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
-                         ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive); }
+                         ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive, q); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS
 }
@@ -879,10 +948,13 @@ void launch_set_lean(Dev& d, int lean) { hipLaunchKernelGGL(k_set_lean, dim3(1),
 void launch_force(Dev& d) {
     if (d.n == 0) return;
     const int B = d.blk[LC_FORCE];
+    int32_t* q = d.persist[LC_FORCE] ? d.queue : nullptr;
 #define CALL_FORCE(KK, DD)                                                                                  \
-    { hipLaunchKernelGGL((k_force<KK, DD>), dim3(xcd_grid(nblk(d.n, B))), dim3(B), 0, d.stream, d.fields, d.fpack, \
+    { const unsigned g = q ? persistent_grid((const void*)k_force<KK, DD>, B, 0, nblk(d.n, B))              \
+                           : xcd_grid(nblk(d.n, B));                                                        \
+      hipLaunchKernelGGL((k_force<KK, DD>), dim3(g), dim3(B), 0, d.stream, d.fields, d.fpack,               \
                          d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride, d.ninst, d.ctl,          \
-                         d.hctl_dev, d.arrive); }
+                         d.hctl_dev, d.arrive, d.strain, q); }
     SPH_DISPATCH(d.kernel, d.dim, CALL_FORCE)
 #undef CALL_FORCE
 }

@@ -113,6 +113,10 @@ class Stats(ctypes.Structure):
         ("block_force", ctypes.c_int32),
         ("launch_tuned", ctypes.c_int32),
         ("steps", ctypes.c_int64),
+        ("persist_density", ctypes.c_int32),
+        ("persist_lists", ctypes.c_int32),
+        ("persist_force", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -229,6 +233,8 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_get_pending_dt": ([vp, dp], ctypes.c_int),
         "sphmhd_totals": ([vp, dp], ctypes.c_int),
         "sphmhd_totals_noalloc": ([vp, vp], ctypes.c_int),
+        "sphmhd_bind_strain": ([vp, vp], ctypes.c_int),
+        "sphmhd_set_persistent": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_cells": ([vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_neighbours": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "sphmhd_get_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),

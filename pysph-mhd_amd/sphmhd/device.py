@@ -152,6 +152,24 @@ class DeviceSystem:
         check(self.lib.sphmhd_totals(self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    STRAIN = ("dvdx_xx", "dvdx_xy", "dvdx_xz", "dvdx_yx", "dvdx_yy", "dvdx_yz", "dvdx_zx", "dvdx_zy", "dvdx_zz")
+
+    def bind_strain(self, on: bool = True) -> None:
+        """StrainDiagnostics on request: with rows bound, every evaluation that
+        writes the diagnostics also writes the velocity gradient dvdx_*."""
+        if on:
+            self.strain_rows = torch.zeros((9, self.n), dtype=torch.float64, device=self.device)
+            ptrs = (ctypes.c_void_p * 9)(*[self.strain_rows[k].data_ptr() for k in range(9)])
+            check(self.lib.sphmhd_bind_strain(self._ctx, ptrs))
+        else:
+            check(self.lib.sphmhd_bind_strain(self._ctx, None))
+            self.strain_rows = None
+
+    def strain(self) -> dict[str, np.ndarray]:
+        torch.cuda.synchronize(self.device)
+        host = self.strain_rows.cpu().numpy()
+        return {nm: host[k].copy() for k, nm in enumerate(self.STRAIN)}
+
     def totals_noalloc(self, out: torch.Tensor | None = None) -> torch.Tensor:
         """The conserved totals through sphmhd_totals_noalloc: enqueued on the
         context's stream into a device tensor [ninst, NTOTALS], with no
@@ -204,6 +222,16 @@ class DeviceSystem:
         else:
             arr = (ctypes.c_int32 * 3)(*[int(b) for b in blocks])
             check(self.lib.sphmhd_set_launch(self._ctx, arr))
+
+    def set_persistent(self, modes=None) -> None:
+        """Launch modes (density, lists, force), each 0 (one workgroup per
+        tile) or 1 (persistent workgroups pulling tiles from work queues);
+        None lets the tuner choose them (results never depend on it)."""
+        if modes is None:
+            check(self.lib.sphmhd_set_persistent(self._ctx, None))
+        else:
+            arr = (ctypes.c_int32 * 3)(*[int(m) for m in modes])
+            check(self.lib.sphmhd_set_persistent(self._ctx, arr))
 
     def launch(self) -> tuple[tuple[int, int, int], bool]:
         arr = (ctypes.c_int32 * 3)()

@@ -26,11 +26,12 @@ def _case(which):
     return jittered(ensemble_ot(4, sizes=[24, 40, 32, 28]), 0.1, seed=9)
 
 
-def _run(setup, blocks, steps):
+def _run(setup, blocks, steps, modes=(0, 0, 0)):
     from sphmhd.device import DeviceSystem
 
     d = DeviceSystem(setup.params, setup.offsets)
     d.set_launch(blocks)
+    d.set_persistent(modes)
     d.set_fields(init_fields(setup))
     d.warm_start()
     d.step(steps)
@@ -53,20 +54,37 @@ def test_every_shape_gives_the_same_steps(gpu, which):
         assert np.array_equal(clock[0], ref_clock[0]) and np.array_equal(clock[1], ref_clock[1])
 
 
+@pytest.mark.parametrize("which", ["ot2d", "blast3d", "ensemble"])
+def test_persistent_launches_give_the_same_steps(gpu, which):
+    """Persistent launches (workgroups pulling tiles of any instance from
+    per-XCD work queues, the queues reset by each launch's last workgroup)
+    against plain launches, bit for bit, in several mixes of modes and
+    shapes; the modes reported are those set."""
+    s = _case(which)
+    ref, ref_clock, _ = _run(s, (256, 256, 256), 4)
+    for blocks, modes in (((256, 256, 256), (1, 1, 1)), ((64, 128, 256), (1, 0, 1)), ((128, 64, 64), (0, 1, 1))):
+        got, clock, st = _run(s, blocks, 4, modes)
+        assert (st["persist_density"], st["persist_lists"], st["persist_force"]) == modes
+        for nm in STATE + RATES:
+            assert np.array_equal(got[nm], ref[nm]), (blocks, modes, nm)
+        assert np.array_equal(clock[0], ref_clock[0]) and np.array_equal(clock[1], ref_clock[1])
+
+
 def test_autotuned_run_equals_fixed_run(gpu):
-    """The first steps of a context rotate the shapes (timed with events)
-    and keep the fastest; the trajectory is that of any fixed shape."""
+    """The first steps of a context rotate the configurations (shape and
+    plain or persistent launch, timed with events) and keep the fastest; the
+    trajectory is that of any fixed configuration."""
     from sphmhd.device import DeviceSystem
 
     s = _case("ot2d")
-    ref, ref_clock, _ = _run(s, (256, 256, 256), 9)
+    ref, ref_clock, _ = _run(s, (256, 256, 256), 15)
     d = DeviceSystem(s.params, s.offsets)
     d.set_launch(None)
     d.set_fields(init_fields(s))
     d.warm_start()
     d.set_timing(True)
-    d.step(4)  # tuning steps span the two calls
-    d.step(5)
+    d.step(5)  # the 12 tuning steps span the two calls
+    d.step(10)
     blocks, tuned = d.launch()
     assert tuned and all(b in (64, 128, 256) for b in blocks)
     st = d.stats()

@@ -385,3 +385,32 @@ def test_run_stops_at_tf(gpu, monkeypatch, graphs):
     dev.step(3)
     assert dev.steps_taken() == 5
     assert np.array_equal(dev.get("x"), x)
+
+
+@pytest.mark.parametrize("which", ["ot2d", "blast3d"])
+def test_strain_diagnostics_on_request(gpu, which):
+    """StrainDiagnostics (equations.py:432-471) from the bound strain rows
+    against the oracle's dvdx_* after the warm start and two single steps:
+    2-D takes the reference's fallback branch (its 3x3 determinant is zero),
+    3-D the inverted one.  Velocity gradients are measured against 1e-3 of
+    v_w / h where a component is analytically zero (w and z gradients in
+    2-D)."""
+    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.device import DeviceSystem
+
+    s = _ot(40) if which == "ot2d" else jittered(blast_3d(14), 0.1, seed=3)
+    dev, orc = _pair(s)
+    dev.bind_strain(True)
+    dev.warm_start()
+    orc.warm_start()
+    for _ in range(2):
+        dev.step(1)
+        orc.step(1)
+    got = dev.strain()
+    cs, h = orc.get("cs"), orc.get("h")
+    floor = 1e-3 * np.linalg.norm(cs / h)
+    for nm in DeviceSystem.STRAIN:
+        a, b = got[nm], orc.get(nm)
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), floor)
+        assert err <= TOL, (nm, err)
+    assert np.linalg.norm(got["dvdx_xx"]) > 0.0
