@@ -134,22 +134,25 @@ def calibrate_copy():
                       "copy_GBps": round(2 * n * 8 / dt / 1e9, 1)}))
 
 
-def make_setup(name: str):
+def make_setup(name: str, host_fields: bool = True):
+    """The workload's setup; host_fields=False leaves the initial fields to
+    the GPU generator (cases.device.load_setup)."""
     from sphmhd.cases import baseline as B
 
+    hf = host_fields
     if name == "rotor512":
-        return B.rotor_2d(512), "rotor_2d 512x512 (262,144 particles), cubic kernel, h=1.2dx"
+        return B.rotor_2d(512, host_fields=hf), "rotor_2d 512x512 (262,144 particles), cubic kernel, h=1.2dx"
     if name == "ot1m":
-        return B.orszag_tang_2d(1024), "orszag_tang_2d 1024x1024 (1,048,576 particles), quintic, h=1.2dx"
+        return B.orszag_tang_2d(1024, host_fields=hf), "orszag_tang_2d 1024x1024 (1,048,576 particles), quintic, h=1.2dx"
     if name == "ot128":
-        return B.orszag_tang_2d(128), "orszag_tang_2d 128x128 (16,384 particles), quintic, h=1.2dx, one instance"
+        return B.orszag_tang_2d(128, host_fields=hf), "orszag_tang_2d 128x128 (16,384 particles), quintic, h=1.2dx, one instance"
     if name == "ot128x64":
-        s = B.ensemble_of([B.orszag_tang_2d(128) for _ in range(64)])
+        s = B.ensemble_of([B.orszag_tang_2d(128, host_fields=hf) for _ in range(64)])
         return s, "64 identical orszag_tang_2d 128x128 instances in one batch (1,048,576 particles)"
     if name == "blast128":
-        return B.blast_3d(128), "blast_3d 128^3 (2,097,152 particles), quintic, h=0.81dx"
+        return B.blast_3d(128, host_fields=hf), "blast_3d 128^3 (2,097,152 particles), quintic, h=0.81dx"
     if name == "ensemble64":
-        s = B.ensemble_ot(64)
+        s = B.ensemble_ot(64, host_fields=hf)
         return s, f"ensemble_ot 64 instances, N_k = n_k^2 with n_k~U{{64..256}} seed 0 ({s.n:,} particles)"
     raise ValueError(name)
 
@@ -164,12 +167,12 @@ def run_gpu(name: str, steps: int, warmup: int, timing_steps: int = 10):
     diagnostic rows, as in the timed region)."""
     import torch
 
+    from sphmhd.cases.device import load_setup
     from sphmhd.device import DeviceSystem
 
-    setup, desc = make_setup(name)
+    setup, desc = make_setup(name, host_fields=False)
     dev = DeviceSystem(setup.params, setup.offsets, device="cuda:0")
-    init = {k: v for k, v in setup.fields.items()}
-    dev.set_fields(init)
+    load_setup(dev, setup)  # initial fields generated on the GPU
     dev.warm_start()
     dev.step(TUNE_STEPS)
     if warmup:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 6
+#define SPHMHD_ABI_VERSION 7
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -271,6 +271,43 @@ int sphmhd_synchronize(sphmhd_ctx* ctx);
  * Enqueued on hip_stream (0: the null stream); no context needed. */
 int sphmhd_hcp_lattice(const double* bounds, double spacing, double* x, double* y, double* z,
                        int64_t cap, int64_t* count, void* hip_stream);
+
+/* Initial fields of a case, computed on the GPU, one thread per particle.
+ * Replaces the fields half of the cases' create_mhd_particles
+ * (src/cases/orstang.py:18-44, mhdrotor.py:18-53, mhdblast.py:17-43,
+ * alfven.py:21-49, jadvect.py:18-49, mhdshock.py:21-62, mhdsine.py:18-42,
+ * mhdvortex.py:28-54, mhdwave.py:18-43, jet.py:68-193).  The benchmark
+ * lattices (the first four ids) and the jets also write x, y, z; the other
+ * reference cases read the positions already in fields[X..Z] (the
+ * close-packed lattice of sphmhd_hcp_lattice, masked and joined as the case
+ * does).  params: the case's particle-independent constants (masses,
+ * smoothing lengths, the lattice step ...), at most 48, in the order the
+ * host wrapper documents; fields: the SPHMHD_NFIELDS row pointers of a bound
+ * layout (x..Bz, alpha1 required, the others untouched); itype: written by
+ * SPHMHD_CASE_MHDSHOCK (walls 3, fluid 1) when non-null.  For the jets,
+ * aux_dev holds the single jet's (axial, y, z) rows of n_aux particles each
+ * and n is n_aux times the jet count.  Arithmetic in the reference's order
+ * with contraction off: equal to the host builders' bit for bit except
+ * through sin, cos, exp, tanh, cosh and pow (a few ulp).  Enqueued on
+ * hip_stream; no context needed. */
+enum {
+    SPHMHD_CASE_ORSZAG_TANG_2D = 1, /* square lattice, cell centres */
+    SPHMHD_CASE_ROTOR_2D = 2,
+    SPHMHD_CASE_BLAST_3D = 3,       /* cubic lattice, x fastest */
+    SPHMHD_CASE_ALFVEN_2D = 4,
+    SPHMHD_CASE_ORSTANG = 5,        /* reference cases, positions given */
+    SPHMHD_CASE_MHDROTOR = 6,
+    SPHMHD_CASE_MHDBLAST = 7,
+    SPHMHD_CASE_ALFVEN = 8,
+    SPHMHD_CASE_JADVECT = 9,
+    SPHMHD_CASE_MHDSHOCK = 10,
+    SPHMHD_CASE_MHDSINE = 11,
+    SPHMHD_CASE_MHDVORTEX = 12,
+    SPHMHD_CASE_MHDWAVE = 13,
+    SPHMHD_CASE_JETS = 14           /* 1jet, 3jet, 12jet */
+};
+int sphmhd_case_fields(int32_t case_id, const double* params, int32_t nparams, double* const* fields,
+                       int32_t* itype, int64_t n, int64_t n_aux, const double* aux_dev, void* hip_stream);
 
 /* Launch shapes.  No counterpart in the reference (PySPH picks its own launch
  * sizes); the results of a step do not depend on them, only its speed does.

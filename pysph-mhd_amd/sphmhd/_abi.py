@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -249,6 +249,7 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_set_launch": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_hcp_lattice": ([ctypes.POINTER(ctypes.c_double), ctypes.c_double, vp, vp, vp, i64,
                                 ctypes.POINTER(ctypes.c_int64), vp], ctypes.c_int),
+        "sphmhd_case_fields": ([i32, dp, i32, ctypes.POINTER(ctypes.c_void_p), vp, i64, i64, vp, vp], ctypes.c_int),
         "sphmhd_get_launch": ([vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_get_timing": ([vp, ctypes.POINTER(Timing)], ctypes.c_int),
     }

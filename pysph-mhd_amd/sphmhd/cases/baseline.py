@@ -4,6 +4,9 @@ Each builder returns a ``Setup``: per-instance parameters, CSR instance
 offsets and the initial properties (the ones an Application's
 create_particles fills: x, y, z, u, v, w, rho, h, m, e, Bx, By, Bz, alpha1).
 Particles sit at lattice cell centres (i + 1/2) dx of periodic boxes.
+Every lattice setup also carries ``device``: per instance, the case id and
+constants with which sphmhd_case_fields writes the same fields on the GPU
+(``cases.device.load_setup``); ``host_fields=False`` skips the host arrays.
 
   orszag_tang_2d   2-D Orszag-Tang vortex (the reference case orstang.py:18-44
                    on a square lattice: quintic kernel, h = 1.2 dx,
@@ -35,6 +38,7 @@ class Setup:
     fields: dict[str, np.ndarray]
     itype: np.ndarray | None = None
     info: dict = field(default_factory=dict)
+    device: list | None = None  # per instance: (case id, constants), or None
 
     @property
     def n(self) -> int:
@@ -59,21 +63,33 @@ def _lattice_3d(n: int, l: float = 1.0):
     return x, y, z, dx
 
 
+# case ids of sphmhd_case_fields (include/sphmhd.h)
+CASE_ORSZAG_TANG_2D, CASE_ROTOR_2D, CASE_BLAST_3D, CASE_ALFVEN_2D = 1, 2, 3, 4
+
+
+def orszag_tang_constants(nx: int, b0_factor: float = 1.0, gamma: float = 5.0 / 3.0, hfact: float = 1.2,
+                          alpha0: float = 0.1) -> list[float]:
+    """nx, dx, rho0, m, h, e, B0, alpha0 of the vortex on an nx^2 lattice."""
+    dx = 1.0 / nx
+    rho0 = 25.0 / (36.0 * np.pi)
+    p0 = 5.0 / (12.0 * np.pi)
+    b0 = b0_factor / np.sqrt(4.0 * np.pi)
+    return [nx, dx, rho0, rho0 * dx * dx, hfact * dx, p0 / ((gamma - 1.0) * rho0), b0, alpha0]
+
+
 def orszag_tang_fields(nx: int, b0_factor: float = 1.0, gamma: float = 5.0 / 3.0, hfact: float = 1.2,
                        alpha0: float = 0.1):
     x, y, dx, _ = _lattice_2d(nx, nx, 1.0, 1.0)
     n = x.size
-    rho0 = 25.0 / (36.0 * np.pi)
-    p0 = 5.0 / (12.0 * np.pi)
-    b0 = b0_factor / np.sqrt(4.0 * np.pi)
+    _, _, rho0, m, h, e, b0, _ = orszag_tang_constants(nx, b0_factor, gamma, hfact, alpha0)
     two_pi = 2.0 * np.pi
     f = {
         "x": x, "y": y, "z": np.zeros(n),
         "u": -np.sin(two_pi * y), "v": np.sin(two_pi * x), "w": np.zeros(n),
         "rho": np.full(n, rho0),
-        "m": np.full(n, rho0 * dx * dx),
-        "h": np.full(n, hfact * dx),
-        "e": np.full(n, p0 / ((gamma - 1.0) * rho0)),
+        "m": np.full(n, m),
+        "h": np.full(n, h),
+        "e": np.full(n, e),
         "Bx": -b0 * np.sin(two_pi * y), "By": b0 * np.sin(2.0 * two_pi * x), "Bz": np.zeros(n),
         "alpha1": np.full(n, alpha0),
     }
@@ -82,20 +98,29 @@ def orszag_tang_fields(nx: int, b0_factor: float = 1.0, gamma: float = 5.0 / 3.0
 
 def orszag_tang_2d(nx: int = 128, *, kernel: str = "quintic", hfact: float = 1.2, b0_factor: float = 1.0,
                    artificial_viscosity: float = 1.0, cleaning_speed_factor: float = 1.0,
-                   cleaning_damping_factor: float = 1.0, tf: float = 1.0e30) -> Setup:
+                   cleaning_damping_factor: float = 1.0, tf: float = 1.0e30, host_fields: bool = True) -> Setup:
     gamma = 5.0 / 3.0
     p = make_params(dim=2, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 0.0), gamma=gamma, hfact=hfact,
                     artificial_viscosity=artificial_viscosity, artificial_thermal_conductivity=1.0,
                     artificial_magnetic_dissipation=1.0, cleaning_speed_factor=cleaning_speed_factor,
                     cleaning_damping_factor=cleaning_damping_factor, timestep_factor=1.0,
                     alpha_min=0.1, alpha_max=1.0, tf=tf)
-    f = orszag_tang_fields(nx, b0_factor, gamma, hfact, alpha0=0.1)
+    f = orszag_tang_fields(nx, b0_factor, gamma, hfact, alpha0=0.1) if host_fields else {}
+    dev = [(CASE_ORSZAG_TANG_2D, orszag_tang_constants(nx, b0_factor, gamma, hfact, alpha0=0.1))]
     return Setup(f"orszag_tang_2d_{nx}", [p], np.array([0, nx * nx], dtype=np.int64), f,
-                 info={"nx": nx, "kernel": kernel})
+                 info={"nx": nx, "kernel": kernel}, device=dev)
 
 
-def rotor_2d(nx: int = 512, *, kernel: str = "cubic", hfact: float = 1.2, tf: float = 1.0e30) -> Setup:
+def rotor_2d(nx: int = 512, *, kernel: str = "cubic", hfact: float = 1.2, tf: float = 1.0e30,
+             host_fields: bool = True) -> Setup:
     gamma = 1.4
+    b = 5.0 / np.sqrt(4.0 * np.pi)
+    p = make_params(dim=2, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 0.0), gamma=gamma, hfact=hfact,
+                    alpha_min=0.0, alpha_max=1.0, tf=tf)
+    dev = [(CASE_ROTOR_2D, [nx, 1.0 / nx, hfact * (1.0 / nx), gamma, b])]
+    info = {"nx": nx, "kernel": kernel}
+    if not host_fields:
+        return Setup(f"rotor_2d_{nx}", [p], np.array([0, nx * nx], dtype=np.int64), {}, info=info, device=dev)
     x, y, dx, _ = _lattice_2d(nx, nx, 1.0, 1.0)
     n = x.size
     r0, r1, omega = 0.1, 0.115, 20.0
@@ -109,7 +134,6 @@ def rotor_2d(nx: int = 512, *, kernel: str = "cubic", hfact: float = 1.2, tf: fl
     vfac = np.where(inside, omega, f_taper * omega * r0 / safe_r)
     u = -vfac * ry
     v = vfac * rx
-    b = 5.0 / np.sqrt(4.0 * np.pi)
     f = {
         "x": x, "y": y, "z": np.zeros(n),
         "u": u, "v": v, "w": np.zeros(n),
@@ -118,18 +142,24 @@ def rotor_2d(nx: int = 512, *, kernel: str = "cubic", hfact: float = 1.2, tf: fl
         "Bx": np.full(n, b), "By": np.zeros(n), "Bz": np.zeros(n),
         "alpha1": np.zeros(n),
     }
-    p = make_params(dim=2, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 0.0), gamma=gamma, hfact=hfact,
-                    alpha_min=0.0, alpha_max=1.0, tf=tf)
-    return Setup(f"rotor_2d_{nx}", [p], np.array([0, n], dtype=np.int64), f, info={"nx": nx, "kernel": kernel})
+    return Setup(f"rotor_2d_{nx}", [p], np.array([0, n], dtype=np.int64), f, info=info, device=dev)
 
 
-def blast_3d(nx: int = 128, *, kernel: str = "quintic", hfact: float = 0.81, tf: float = 1.0e30) -> Setup:
+def blast_3d(nx: int = 128, *, kernel: str = "quintic", hfact: float = 0.81, tf: float = 1.0e30,
+             host_fields: bool = True) -> Setup:
     gamma = 5.0 / 3.0
+    b = 1.0 / np.sqrt(2.0)
+    p = make_params(dim=3, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=gamma, hfact=hfact,
+                    alpha_min=0.0, alpha_max=1.0, tf=tf)
+    dx = 1.0 / nx
+    dev = [(CASE_BLAST_3D, [nx, dx, dx ** 3, hfact * dx, gamma, b, 0.1 ** 2])]
+    info = {"nx": nx, "kernel": kernel}
+    if not host_fields:
+        return Setup(f"blast_3d_{nx}", [p], np.array([0, nx ** 3], dtype=np.int64), {}, info=info, device=dev)
     x, y, z, dx = _lattice_3d(nx)
     n = x.size
     r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2
     pres = np.where(r2 < 0.1 ** 2, 100.0, 1.0)
-    b = 1.0 / np.sqrt(2.0)
     f = {
         "x": x, "y": y, "z": z,
         "u": np.zeros(n), "v": np.zeros(n), "w": np.zeros(n),
@@ -138,9 +168,7 @@ def blast_3d(nx: int = 128, *, kernel: str = "quintic", hfact: float = 0.81, tf:
         "Bx": np.full(n, b), "By": np.full(n, b), "Bz": np.zeros(n),
         "alpha1": np.zeros(n),
     }
-    p = make_params(dim=3, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=gamma, hfact=hfact,
-                    alpha_min=0.0, alpha_max=1.0, tf=tf)
-    return Setup(f"blast_3d_{nx}", [p], np.array([0, n], dtype=np.int64), f, info={"nx": nx, "kernel": kernel})
+    return Setup(f"blast_3d_{nx}", [p], np.array([0, n], dtype=np.int64), f, info=info, device=dev)
 
 
 def ensemble_table(count: int = 64, seed: int = 0):
@@ -155,22 +183,23 @@ def ensemble_table(count: int = 64, seed: int = 0):
 
 
 def ensemble_ot(count: int = 64, seed: int = 0, *, sizes=None, kernel: str = "quintic", hfact: float = 1.2,
-                tf: float = 1.0e30) -> Setup:
+                tf: float = 1.0e30, host_fields: bool = True) -> Setup:
     n_tab, b0f, jit = ensemble_table(count, seed)
     if sizes is not None:
         n_tab = np.asarray(sizes, dtype=np.int64)[:count]
-    params, parts = [], []
+    params, parts, dev = [], [], []
     offsets = [0]
     for k in range(count):
         s = orszag_tang_2d(int(n_tab[k]), kernel=kernel, hfact=hfact, b0_factor=float(b0f[k]),
                            artificial_viscosity=float(jit[k, 0]), cleaning_speed_factor=float(jit[k, 1]),
-                           cleaning_damping_factor=float(jit[k, 2]), tf=tf)
+                           cleaning_damping_factor=float(jit[k, 2]), tf=tf, host_fields=host_fields)
         params.append(s.params[0])
         parts.append(s.fields)
+        dev += s.device
         offsets.append(offsets[-1] + s.n)
     fields = {name: np.concatenate([p[name] for p in parts]) for name in parts[0]}
     return Setup(f"ensemble_ot_{count}", params, np.asarray(offsets, dtype=np.int64), fields,
-                 info={"sizes": n_tab.tolist(), "b0_factor": b0f.tolist(), "jitter": jit.tolist()})
+                 info={"sizes": n_tab.tolist(), "b0_factor": b0f.tolist(), "jitter": jit.tolist()}, device=dev)
 
 
 def ensemble_of(setups) -> Setup:
@@ -182,20 +211,34 @@ def ensemble_of(setups) -> Setup:
         params.append(s.params[0])
         offsets.append(offsets[-1] + s.n)
     fields = {name: np.concatenate([s.fields[name] for s in setups]) for name in setups[0].fields}
+    dev = [d for s in setups for d in s.device] if all(s.device for s in setups) else None
     return Setup(f"ensemble_of_{len(setups)}", params, np.asarray(offsets, dtype=np.int64), fields,
-                 info={"members": [s.name for s in setups]})
+                 info={"members": [s.name for s in setups]}, device=dev)
 
 
 def alfven_2d(nx: int = 97, ny: int = 168, *, amplitude: float = 0.1, kernel: str = "wendland_c6",
               hfact: float = 1.9, alpha_max: float = 1.0, cleaning_damping_factor: float = 1.0,
-              tf: float = 1.0e30) -> Setup:
+              tf: float = 1.0e30, host_fields: bool = True) -> Setup:
     """Travelling circularly polarised Alfven wave along (cos 30, sin 30) in the
     periodic box 1/cos30 x 1/sin30 (one wavelength along each diagonal)."""
     gamma = 5.0 / 3.0
     lx, ly = 1.0 / np.cos(np.pi / 6.0), 1.0 / np.sin(np.pi / 6.0)
+    ca, sa = np.cos(np.pi / 6.0), np.sin(np.pi / 6.0)
+    p = make_params(dim=2, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(lx, ly, 0.0), gamma=gamma, hfact=hfact,
+                    artificial_viscosity=1.0, artificial_thermal_conductivity=0.0,
+                    artificial_magnetic_dissipation=0.0, cleaning_speed_factor=1.0,
+                    cleaning_damping_factor=cleaning_damping_factor, timestep_factor=0.5, alpha_min=0.0,
+                    alpha_max=alpha_max, tf=tf)
+    dx, dy = lx / nx, ly / ny
+    rho, pres = 1.0, 0.1
+    dev = [(CASE_ALFVEN_2D, [nx, dx, dy, amplitude, ca, sa, rho * dx * dy, hfact * np.sqrt(dx * dy),
+                             pres / ((gamma - 1.0) * rho)])]
+    info = {"nx": nx, "ny": ny, "amplitude": amplitude}
+    if not host_fields:
+        return Setup(f"alfven_2d_{nx}x{ny}", [p], np.array([0, nx * ny], dtype=np.int64), {}, info=info,
+                     device=dev)
     x, y, dx, dy = _lattice_2d(nx, ny, lx, ly)
     n = x.size
-    ca, sa = np.cos(np.pi / 6.0), np.sin(np.pi / 6.0)
     xpar = x * ca + y * sa
     ph = 2.0 * np.pi * xpar
     bperp = amplitude * np.sin(ph)
@@ -205,8 +248,6 @@ def alfven_2d(nx: int = 97, ny: int = 168, *, amplitude: float = 0.1, kernel: st
     by = 1.0 * sa + bperp * ca
     vx = -bperp * sa
     vy = bperp * ca
-    rho = 1.0
-    pres = 0.1
     f = {
         "x": x, "y": y, "z": np.zeros(n),
         "u": vx, "v": vy, "w": bz.copy(),
@@ -216,22 +257,20 @@ def alfven_2d(nx: int = 97, ny: int = 168, *, amplitude: float = 0.1, kernel: st
         "Bx": bx, "By": by, "Bz": bz,
         "alpha1": np.zeros(n),
     }
-    p = make_params(dim=2, kernel=kernel, lo=(0.0, 0.0, 0.0), hi=(lx, ly, 0.0), gamma=gamma, hfact=hfact,
-                    artificial_viscosity=1.0, artificial_thermal_conductivity=0.0,
-                    artificial_magnetic_dissipation=0.0, cleaning_speed_factor=1.0,
-                    cleaning_damping_factor=cleaning_damping_factor, timestep_factor=0.5, alpha_min=0.0,
-                    alpha_max=alpha_max, tf=tf)
     # the fields are linear in the amplitude: their exact derivative, the
     # initial tangent of a rollout differentiated with respect to it
     s_ph, c_ph = np.sin(ph), np.cos(ph)
     d_amp = {"u": -s_ph * sa, "v": s_ph * ca, "w": c_ph.copy(), "Bx": -s_ph * sa, "By": s_ph * ca, "Bz": c_ph.copy()}
-    return Setup(f"alfven_2d_{nx}x{ny}", [p], np.array([0, n], dtype=np.int64), f,
-                 info={"nx": nx, "ny": ny, "amplitude": amplitude, "tangents": {"amplitude": d_amp}})
+    info["tangents"] = {"amplitude": d_amp}
+    return Setup(f"alfven_2d_{nx}x{ny}", [p], np.array([0, n], dtype=np.int64), f, info=info, device=dev)
 
 
 def jittered(setup: Setup, frac: float = 0.1, seed: int = 1) -> Setup:
     """Copy of a lattice setup with positions jittered by +-frac*dx (uniform),
-    to avoid exact neighbour-radius ties in tests."""
+    to avoid exact neighbour-radius ties in tests (host fields only: the
+    jittered positions are no lattice the device generator knows)."""
+    if not setup.fields:
+        raise ValueError("jittered needs a setup built with host fields")
     rng = np.random.default_rng(seed)
     f = {k: v.copy() for k, v in setup.fields.items()}
     p0 = setup.params[0]

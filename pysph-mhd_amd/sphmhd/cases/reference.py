@@ -390,3 +390,163 @@ CASES = {
     "mhdwave": MHDWave,
     "orstang": OrszagTang,
 }
+
+
+# ---- the same initial states generated on the GPU ---------------------------
+# Each function returns the case id of sphmhd_case_fields, its constants (the
+# particle-independent values, from the host method's own expressions), the
+# device positions the kernel reads and, for the jets, the single jet's rows.
+# Where the host method raises an array to a power (h of the two densities of
+# the rotor and the shock tube), the constants come from the same numpy array
+# operation, so that they round as the host arrays do.
+
+def _orstang_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    b0 = 1.0 / np.sqrt(4.0 * np.pi)
+    pressure = 0.5 * b0 * b0 * (10.0 / 3.0)
+    rho = app.gamma * pressure
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 5, [xmin, ymin, rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0),
+               pressure / ((app.gamma - 1.0) * rho), b0], pos, None
+
+
+def _mhdrotor_device(app, lattice):
+    import torch
+
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    spacing = (xmax - xmin) / app.nx
+    r0 = 0.1
+    ox, oy, oz = lattice(app.bounds, spacing)
+    keep_out = ox * ox + oy * oy >= r0**2
+    ix, iy, iz = lattice(app.bounds, spacing * 10.0 ** (-1.0 / 3.0))
+    keep_in = ix * ix + iy * iy <= r0**2
+    pos = tuple(torch.cat((o[keep_out], i[keep_in])) for o, i in ((ox, ix), (oy, iy), (oz, iz)))
+    n = len(pos[0])
+    mass = ((xmax - xmin) * (ymax - ymin) + 10.0 * np.pi * r0**2) * (zmax - zmin) / n
+    h_in, h_out = app.hfact * (mass / np.array([10.0, 1.0])) ** (1.0 / 3.0)
+    return 6, [r0, 2.0 / r0, mass, h_in, h_out, app.gamma - 1.0, 5.0 / np.sqrt(4.0 * np.pi)], pos, None
+
+
+def _mhdblast_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    rho = 1.0
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 7, [rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0), app.gamma - 1.0, 10.0 / np.sqrt(2.0),
+               0.125**2], pos, None
+
+
+def _alfven_device(app, lattice):
+    pos = lattice(app.bounds, app.dx)  # rounded through float32 by the kernel
+    mass = float(np.float32(6.75 / len(pos[0])))
+    return 8, [mass, app.hfact * mass ** (1.0 / 3.0)], pos, None
+
+
+def _jadvect_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    rho = 1.0
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 9, [rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0), 0.1 * np.sqrt(5.0)], pos, None
+
+
+def _mhdshock_device(app, lattice):
+    import torch
+
+    assert app.nx % 2 == 0
+    _, _, ymin, ymax, zmin, zmax = app.bounds
+    dxl = 0.5 / app.nx
+    dxr = 2.0 * dxl
+    left = lattice((-0.5, 0.0, ymin, ymax, zmin, zmax), dxl)
+    right = lattice((0.0, 0.5, ymin, ymax, zmin, zmax), dxr)
+    pos = tuple(torch.cat((a, b)) for a, b in zip(left, right))
+    mass = 0.5 * (ymax - ymin) * (zmax - zmin) / len(left[0])
+    h_left, h_right = app.hfact * (mass / np.array([1.0, 0.125])) ** (1.0 / 3.0)
+    return 10, [mass, h_left, h_right, app.gamma - 1.0, -0.5 + 6.0 * dxl, 0.5 - 6.0 * dxr], pos, None
+
+
+def _mhdsine_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    rho = 2.0
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 11, [xmin, rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0)], pos, None
+
+
+def _mhdvortex_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    rho = 1.0
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 12, [rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0), app.gamma - 1.0, 8.0 * np.pi**2], pos, None
+
+
+def _mhdwave_device(app, lattice):
+    xmin, xmax, ymin, ymax, zmin, zmax = app.bounds
+    pos = lattice(app.bounds, (xmax - xmin) / app.nx)
+    n = len(pos[0])
+    rho = 2.0
+    mass = rho * (xmax - xmin) * (ymax - ymin) * (zmax - zmin) / n
+    return 13, [rho, mass, app.hfact * (mass / rho) ** (1.0 / 3.0), np.sqrt(2.0 / 3.0)], pos, None
+
+
+def _jets_device(app, lattice):
+    import torch
+
+    j = app.config.jet
+    a, cy, cz = lattice((j.inner_radius, j.outer_radius, -j.radius, j.radius, -j.radius, j.radius), app.dx)
+    inside = cy * cy + cz * cz < j.radius**2
+    aux = torch.stack((a[inside], cy[inside], cz[inside])).contiguous()
+    per_jet = aux.shape[1]
+    rho = j.number_density * ARGON_MASS_NUMBER * ATOMIC_MASS
+    pressure = rho * (j.temperature_ev * EV_TO_K) * 8.31 / ARGON_MOLAR_MASS
+    volume = pi * j.radius**2 * (j.outer_radius - j.inner_radius) / per_jet
+    consts = [rho, rho * volume, app.hfact * volume ** (1.0 / 3.0), pressure / ((app.gamma - 1.0) * rho), j.radius,
+              j.inner_radius, j.outer_radius, j.end_taper_fraction * (j.outer_radius - j.inner_radius),
+              j.axial_speed, j.azimuthal_speed, j.axial_field, j.toroidal_field, TOROIDAL_PEAK]
+    for deg in JET_ANGLES[app.config.case.name]:
+        ang = deg * pi / 180.0
+        consts += [cos(ang), sin(ang)]
+    return 14, consts, None, aux
+
+
+_DEVICE = {
+    OrszagTang: _orstang_device, MHDRotor: _mhdrotor_device, MHDBlast: _mhdblast_device,
+    AlfvenWave: _alfven_device, CurrentLoopAdvection: _jadvect_device, MHDShock: _mhdshock_device,
+    MHDSine: _mhdsine_device, MHDVortex: _mhdvortex_device, MHDWave: _mhdwave_device, JMXJets: _jets_device,
+}
+
+
+def device_particles(app, device="cuda"):
+    """GPU twin of ``app.create_mhd_particles()``: the lattice from
+    sphmhd_hcp_lattice, masked and joined as the case does with torch, then
+    the fields from sphmhd_case_fields.  Returns (rows, itype): an FP64 device
+    tensor [NFIELDS, n] in the bound layout (rows the case does not set are
+    zero) and an int32 device tensor of n particle types (1, or 3 for the
+    shock tube's walls)."""
+    import torch
+
+    from .device import new_rows, set_positions, write_case
+    from .lattice import close_packed_lattice_device
+
+    def lattice(bounds, spacing):
+        return close_packed_lattice_device(bounds, spacing, device=device)
+
+    maker = _DEVICE.get(type(app))
+    if maker is None:
+        raise TypeError(f"no device generator for {type(app).__name__}")
+    with torch.cuda.device(torch.device(device)):
+        case_id, consts, pos, aux = maker(app, lattice)
+        n = aux.shape[1] * (len(consts) - 13) // 2 if aux is not None else len(pos[0])
+        rows = new_rows(n, device=device)
+        if pos is not None:
+            set_positions(rows, *pos)
+        itype = torch.ones(n, dtype=torch.int32, device=device)
+        write_case(case_id, consts, rows, itype=itype, aux=aux)
+    return rows, itype
