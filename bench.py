@@ -110,7 +110,7 @@ def parse():
                     choices=["rotor512", "ot1m", "blast128", "ensemble64", "ot128", "ot128x64"])
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 512 MiB FP64 buffer 5 times (scales HBM byte counters)")
     return ap.parse_args()

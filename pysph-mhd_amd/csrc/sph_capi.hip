@@ -740,7 +740,7 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.bigc, n / (kSmallBucket + 1) + 1))) return cleanup(rc);
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
-        if ((rc = dalloc(&d.hcell, cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.hcell, 2 * cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cls, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cle, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);

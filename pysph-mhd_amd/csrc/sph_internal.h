@@ -642,8 +642,9 @@ __device__ __forceinline__ void x_shifts(int lo, int hi, int n0, double L, doubl
 // particle than R max(h_i, h_c), h_c the cell's largest h where per-cell
 // values are given (hcell) and otherwise a bound hb on every h.  Such a cell
 // holds no pair of the particle, so every sum and list comes out exactly as
-// from the full scan.  Whole stencil rows are dropped and row ends trimmed,
-// so a row stays one contiguous range.  The first and last cells of a
+// from the full scan.  Whole stencil rows are dropped (by the row's largest h
+// where per-cell values are given, before any cell of it is read) and row
+// ends trimmed, so a row stays one contiguous range.  The first and last cells of a
 // periodic axis can hold particles binned there by clamping (outside the box
 // before the first wrap), so their gap along that axis counts as zero; gaps
 // are shrunk by a margin far above the rounding of the binning and of the
@@ -655,6 +656,7 @@ struct Prune {
     int per;
     double R, hi, hb;
     const unsigned long long* hcell;
+    const unsigned long long* hrow;  // with hcell: the largest h of each row, at the row's first cell
 
     // gap along axis a between the particle and stencil cell q (unwrapped)
     __device__ __forceinline__ double gap(int a, int q) const {
@@ -673,7 +675,7 @@ struct Prune {
     // trims [xl, xh] of the stencil row starting at global cell `row`;
     // false when nothing is left
     __device__ __forceinline__ bool trim(double g2yz, int64_t row, int& xl, int& xh) const {
-        if (far(g2yz, hb)) return false;
+        if (far(g2yz, hrow ? bitsd(hrow[row]) : hb)) return false;
         auto skip = [&](int q) {
             const double g = gap(0, q);
             if (!hcell) return far(g2yz + g * g, hb);
@@ -689,7 +691,8 @@ struct Prune {
 // hb: the bound on every h for cells without values (0 for a reach of R h_i)
 template <int DIM>
 __device__ __forceinline__ void prune_init(Prune& p, const Inst& s, int64_t key, double x, double y, double z,
-                                           double R, double hi, double hb, const unsigned long long* hcell) {
+                                           double R, double hi, double hb, const unsigned long long* hcell,
+                                           const unsigned long long* hrow) {
     const int64_t local = key - s.cell_base;
     p.ci[0] = (int)(local % s.n[0]);
     p.ci[1] = (int)((local / s.n[0]) % s.n[1]);
@@ -708,6 +711,7 @@ __device__ __forceinline__ void prune_init(Prune& p, const Inst& s, int64_t key,
     p.hi = hi;
     p.hb = hb;
     p.hcell = hcell;
+    p.hrow = hcell ? hrow : nullptr;
 }
 
 template <int DIM, class F>
@@ -1073,7 +1077,8 @@ struct Dev {
     int32_t* bigc;          // cells with more than kSmallBucket particles
     int32_t *cstart, *cend;
     int32_t *cls, *cle;     // the scan's block-local cell starts and ends (cell_cap)
-    unsigned long long* hcell;  // per cell (cell_cap): bits of its largest h, for split grids
+    unsigned long long* hcell;  // per cell (cell_cap): bits of its largest h, for split grids; then
+                                // (cell_cap more, at a row's first cell) the largest h of the row
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build
     int32_t* act[2];        // active-particle lists of the density passes

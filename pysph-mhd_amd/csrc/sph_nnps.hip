@@ -261,6 +261,7 @@ __global__ void k_cell_finish_rank(const int32_t* __restrict__ cls, const int32_
         cstart[c] = s;
         cend[c] = e;
         hcell[c] = 0ull;  // filled from the final h before pruned lists are built
+        hcell[cap + c] = 0ull;  // the row maxima, at each row's first cell
         if (e - s > kSmallBucket) bigc[atomicAdd(&ctl->nbig, 1)] = (int32_t)c;
     }
     if (q >= n) return;
@@ -359,11 +360,22 @@ __global__ void k_pack_xh(Fields F, const int32_t* __restrict__ perm, double4* _
 // sort's k_cell_finish_rank)
 __global__ void k_cell_h(const double4* __restrict__ xh, const uint32_t* __restrict__ skey,
                          const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
-                         unsigned long long* __restrict__ hcell, int64_t n, const Ctl* ctl) {
+                         unsigned long long* __restrict__ hcell, int64_t cap, int64_t n, const Ctl* ctl) {
     if (ctl->abort) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool use = t < n && inst[inst_of[t]].split > 1;
-    cell_hmax(hcell, use ? skey[t] : 0xFFFFFFFFu, use ? xh[t].w : 0.0, use);
+    const Inst* s = t < n ? &inst[inst_of[t]] : nullptr;
+    const bool use = s && s->split > 1;
+    const uint32_t key = use ? skey[t] : 0xFFFFFFFFu;
+    const double h = use ? xh[t].w : 0.0;
+    cell_hmax(hcell, key, h, use);
+    // the row's first cell: sorted keys give sorted row keys, so equal rows
+    // stay runs of the wave
+    uint32_t rkey = key;
+    if (use) {
+        const int64_t local = (int64_t)key - s->cell_base;
+        rkey = (uint32_t)(s->cell_base + local - local % s->n[0]);
+    }
+    cell_hmax(hcell + cap, rkey, h, use);
 }
 
 // before re-running a force pass: forget the partial pass's dt minimum and
@@ -396,7 +408,7 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
                                          const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                          int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                          int stride, Ctl* ctl, int cap, const unsigned long long* __restrict__ hcell,
-                                         int64_t t0) {
+                                         const unsigned long long* __restrict__ hrow, int64_t t0) {
     // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
     // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
     // rarely fit and the LDS would cost occupancy (cap 0)
@@ -433,7 +445,7 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
         Prune pr;
         if (PRUNE && s.range_shift)
             prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, bitsd(s.hmax_bits),
-                            hcell && s.split > 1 ? hcell : nullptr);
+                            hcell && s.split > 1 ? hcell : nullptr, hrow);
         auto scan = [&](auto shift_c) {
             constexpr bool kShift = decltype(shift_c)::value;
             for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh,
@@ -487,7 +499,8 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl, int cap,
-                                                     const unsigned long long* __restrict__ hcell, int32_t* queue,
+                                                     const unsigned long long* __restrict__ hcell,
+                                                     const unsigned long long* __restrict__ hrow, int32_t* queue,
                                                      int32_t* arrive) {
     if (ctl->abort) return;
     const int B = blockDim.x;
@@ -495,11 +508,11 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         int polls = 0;
         for (int64_t tile; (tile = pull_tile(queue, (n + B - 1) / B, polls)) >= 0;)
             nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
-                                 tile * B);
+                                 hrow, tile * B);
         if (last_to_arrive(arrive)) reset_queues(queue);
     } else {
         nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
-                             xcd_tile((n + B - 1) / B) * B);
+                             hrow, xcd_tile((n + B - 1) / B) * B);
     }
 }
 
@@ -580,7 +593,7 @@ static void nb_build(Dev& d, const unsigned long long* hc) {
                             : xcd_grid((d.n + B - 1) / B);
     hipLaunchKernelGGL((k_nb_build<DIM, PRUNE>), dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
                        (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
-                       d.nb_stride, d.ctl, cap, hc, q, d.arrive);
+                       d.nb_stride, d.ctl, cap, hc, hc ? hc + d.cell_cap : nullptr, q, d.arrive);
 }
 
 void launch_nb_build(Dev& d, bool cell_h) {
@@ -588,7 +601,7 @@ void launch_nb_build(Dev& d, bool cell_h) {
     const bool per_cell = cell_h && d.prune_lists;
     if (per_cell)
         hipLaunchKernelGGL(k_cell_h, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
-                           d.inst, d.inst_of, d.hcell, d.n, d.ctl);
+                           d.inst, d.inst_of, d.hcell, d.cell_cap, d.n, d.ctl);
     const unsigned long long* hc = per_cell ? d.hcell : nullptr;
     if (d.dim == 2) {
         if (d.prune_lists) nb_build<2, true>(d, hc);
