@@ -246,7 +246,7 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
 
 // one density pass; its last workgroup runs the group control
 template <int K, int DIM>
-__global__ void __launch_bounds__(kBlock) k_density(Fields F, const double4* __restrict__ dpack,
+__global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, const double4* __restrict__ dpack,
                                                     const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                                     Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
@@ -830,7 +830,7 @@ __global__ void k_set_lean(Ctl* ctl, int v) { ctl->lean = v; }
 
 // the fused force pass; its last workgroup ends the evaluation
 template <int K, int DIM>
-__global__ void __launch_bounds__(kBlock) k_force(Fields F, const double* __restrict__ fpack,
+__global__ void __launch_bounds__(kBlock, DIM == 3 ? 2 : 1) k_force(Fields F, const double* __restrict__ fpack,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nb,
                                                   const int32_t* __restrict__ nbcnt, Inst* inst,
                                                   const int32_t* __restrict__ inst_of, int64_t n, int stride,
