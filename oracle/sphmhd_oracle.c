@@ -880,6 +880,52 @@ static double box_pad(double mn, double mx) {
     return fmax(0.01 * (mx - mn), 1.0e-9 * fmax(fmax(fabs(mn), fabs(mx)), 1.0));
 }
 
+/* Levels of h for the 3-D cell size of a single instance: the biased
+   exponent and top four mantissa bits of a positive h (16 per octave);
+   level_value(l) is the smallest h of level l. */
+#define HIST_SLOTS 64
+static int h_level(double h) {
+    uint64_t b;
+    memcpy(&b, &h, sizeof b);
+    return (int)(b >> 48);
+}
+static double level_value(int l) { return ldexp(1.0 + (double)(l & 15) / 16.0, (l >> 4) - 1023); }
+
+/* The cell size among c0 (the h_max rule) and half of R times the upper
+   edge (capped at h_max) of each occupied level that minimises the modelled
+   scan cost: per particle of level edge u, s = max(1, ceil(R u / c)) cells
+   per axis, (2s+1)^3 cells of (c/u)^3 h-volumes holding hfact^3 particles
+   each, plus 6 candidates' worth per stencil row ((2s+1)^2); ties keep the
+   larger cell.  cnt[j]: particles of level l0 + j, j < nl. */
+static double cell_model_3d(double R, double hfact, double hm, int l0, int nl, const int64_t* cnt, double c0) {
+    const double k3 = hfact * hfact * hfact;
+    double best = c0, bcost = 0.0;
+    for (int cand = -1; cand < nl; ++cand) {
+        if (cand >= 0 && !cnt[cand]) continue;
+        double c = c0;
+        if (cand >= 0) {
+            double u = level_value(l0 + cand + 1);
+            c = 0.5 * (R * (u < hm ? u : hm));
+        }
+        double cost = 0.0;
+        for (int j = 0; j < nl; ++j) {
+            if (!cnt[j]) continue;
+            double v = level_value(l0 + j + 1);
+            double u = v < hm ? v : hm;
+            double sj = ceil(R * u / c);
+            if (sj < 1.0) sj = 1.0;
+            double w = 2.0 * sj + 1.0;
+            double x = w * c / u;
+            cost += (double)cnt[j] * (k3 * (x * x * x) + 6.0 * (w * w));
+        }
+        if (cand < 0 || cost < bcost || (cost == bcost && c > best)) {
+            best = c;
+            bcost = cost;
+        }
+    }
+    return best;
+}
+
 static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
     const sphmhd_params* P = &o->prm[k];
     grid_t* g = &o->grid[k];
@@ -907,6 +953,21 @@ static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
     /* in 3-D at least 2 cells per R h_max (twice the h split, at most 4) */
     double split = P->dim == 3 ? fmin(2.0 * cell_split(hmax, hmin), 4.0) : cell_split(hmax, hmin);
     double c = R * hmax / split;
+    /* a single 3-D instance while h spans less than 64 levels: the model */
+    int model = o->ninst == 1 && P->dim == 3 && hmin > 0.0 && hmax >= hmin &&
+                h_level(hmax) - h_level(hmin) < HIST_SLOTS;
+    if (model) {
+        int64_t cnt[HIST_SLOTS];
+        int l0 = h_level(hmin), nl = h_level(hmax) - l0 + 1;
+        for (int j = 0; j < HIST_SLOTS; ++j) cnt[j] = 0;
+        for (int64_t i = o->off[k]; i < o->off[k + 1]; ++i) {
+            double h = o->p[P_h][i];
+            if (h > 0.0) cnt[h_level(h) - l0]++;
+        }
+        c = cell_model_3d(R, P->hfact, hmax, l0, nl, cnt, c);
+        c = fmax(c, R * hmax / 8.0);
+        split = R * hmax / c;
+    }
     double lmax = 0.0;
     for (int a = 0; a < P->dim; ++a) lmax = dmax(lmax, L[a]);
     if (!(c > 0.0) || !(c <= lmax)) { /* h_max of 0, NaN or above the box: one cell */
@@ -928,7 +989,9 @@ static void grid_setup(orc_t* o, int k, double hmax, double hmin) {
         }
         if (prod <= cap) break;
         /* too many cells: first undo the split, then coarsen */
-        if (split > 1.0) {
+        if (model) {
+            c = 2.0 * c;
+        } else if (split > 1.0) {
             split -= 1.0;
             c = R * hmax / split;
         } else {

@@ -740,7 +740,13 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if ((rc = dalloc(&d.bigc, n / (kSmallBucket + 1) + 1))) return cleanup(rc);
         if ((rc = dalloc(&d.cstart, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cend, cap))) return cleanup(rc);
-        if ((rc = dalloc(&d.hcell, 2 * cap))) return cleanup(rc);
+        if ((rc = dalloc(&d.hcell, 4 * cap))) return cleanup(rc);
+        // one 3-D instance: the h histogram its cell size is chosen from
+        if (n_instances == 1 && p0.dim == 3) {
+            if ((rc = dalloc(&d.hist, (int64_t)kHistSlots * kHistCopies))) return cleanup(rc);
+            hipError_t eh = hipMemcpy(&d.ctl->hist, &d.hist, sizeof(d.hist), hipMemcpyHostToDevice);
+            if (eh != hipSuccess) return cleanup(fail(SPHMHD_EHIP, hipGetErrorString(eh)));
+        }
         if ((rc = dalloc(&d.cls, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.cle, cap))) return cleanup(rc);
         if ((rc = dalloc(&d.dpack, 4 * n))) return cleanup(rc);
@@ -792,7 +798,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     if (c->sev) (void)hipEventDestroy(c->sev);
     void* ptrs[] = {d.inst, d.ctl, d.arrive, d.queue, d.hblk, d.kblk, d.inst_of, d.keys, d.keys_alt, d.perm, d.perm_alt, d.ccount, d.cblock, d.bigc,
                     d.cstart, d.cend, d.hcell, d.cls, d.cle, d.dpack, d.fpack, d.nb, d.nbcnt, d.scratch3, d.aprev, d.totals, d.act[0],
-                    d.act[1]};
+                    d.act[1], d.hist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);

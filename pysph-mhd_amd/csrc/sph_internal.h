@@ -134,6 +134,10 @@ struct Ctl {
     alignas(128) int32_t act1;
     alignas(128) int32_t nbig;   // cells with more than kSmallBucket particles in this sort
     alignas(128) int64_t count;  // particles counted by k_count
+    // histogram of h for the 3-D cell size (a context of one 3-D instance,
+    // else null): kHistCopies rings of kHistSlots counts, filled by the
+    // launch that reduces h_max and emptied by its grid set-up
+    alignas(128) uint32_t* hist;
 };
 
 __host__ __device__ inline int32_t* act_count(Ctl* c, int k) { return k ? &c->act1 : &c->act0; }
@@ -361,6 +365,71 @@ __host__ __device__ inline double cell_split(double hmax, double hmin) {
     return 1.0;
 }
 
+// Levels of h for the 3-D cell size: the biased exponent and top four
+// mantissa bits of a positive h (16 levels per octave, increasing with h),
+// counted in a ring of kHistSlots slots; level_value(l) is the smallest h of
+// level l, so level l holds [level_value(l), level_value(l + 1)).
+constexpr int kHistSlots = 64;
+constexpr int kHistCopies = 8;  // rings, one per group of workgroups (same-address atomics serialise)
+__host__ __device__ inline int h_level(double h) {
+    unsigned long long b;
+    memcpy(&b, &h, sizeof b);
+    return (int)(b >> 48);
+}
+__host__ __device__ inline double level_value(int l) { return ldexp(1.0 + (double)(l & 15) / 16.0, (l >> 4) - 1023); }
+
+// A workgroup's part of the h histogram: counts per slot in LDS (lh), then
+// one atomic per occupied slot into one of the rings (callers put a barrier
+// between the three)
+__device__ __forceinline__ void hist_clear(uint32_t* lh) {
+    for (int t = threadIdx.x; t < kHistSlots; t += blockDim.x) lh[t] = 0u;
+}
+__device__ __forceinline__ void hist_add(uint32_t* lh, double h) {
+    if (h > 0.0) atomicAdd(&lh[h_level(h) & (kHistSlots - 1)], 1u);
+}
+__device__ __forceinline__ void hist_flush(uint32_t* hist, const uint32_t* lh) {
+    uint32_t* dst = hist + (blockIdx.x % kHistCopies) * kHistSlots;
+    for (int t = threadIdx.x; t < kHistSlots; t += blockDim.x)
+        if (lh[t]) atomicAdd(&dst[t], lh[t]);
+}
+
+// Cell size of a 3-D instance from its h histogram (ring[(l0 + j) % slots],
+// j < nl: the counts of levels l0 .. l0 + nl - 1, l0 the level of h_min):
+// the candidate among c0 (the h_max rule) and half of R times the upper edge
+// of each occupied level that minimises a model of the scan cost, the
+// largest such cell on ties.  Per particle of level upper edge u (capped at
+// h_max): s = max(1, ceil(R u / c)) stencil cells per axis, (2s+1)^3 cells of
+// (c/u)^3 h-volumes holding hfact^3 particles each, plus kRowCost
+// candidates' worth per stencil row ((2s+1)^2).  Contraction off: the
+// oracle makes the same choice from the same counts.
+constexpr double kRowCost = 6.0;
+
+// candidate cand (-1: c0) and its modelled cost
+__host__ __device__ inline double cell_candidate(double R, double hm, int l0, int cand, double c0) {
+#pragma clang fp contract(off)
+    if (cand < 0) return c0;
+    const double u = level_value(l0 + cand + 1);
+    return 0.5 * (R * (u < hm ? u : hm));
+}
+__host__ __device__ inline double cell_cost_3d(double R, double hfact, double hm, int l0, int nl,
+                                               const uint32_t* ring, double c) {
+#pragma clang fp contract(off)
+    const double k3 = hfact * hfact * hfact;
+    double cost = 0.0;
+    for (int j = 0; j < nl; ++j) {
+        const uint32_t nj = ring[(l0 + j) & (kHistSlots - 1)];
+        if (!nj) continue;
+        const double v = level_value(l0 + j + 1);
+        const double u = v < hm ? v : hm;
+        double sj = ceil(R * u / c);
+        if (sj < 1.0) sj = 1.0;
+        const double w = 2.0 * sj + 1.0;
+        const double x = w * c / u;
+        cost += (double)nj * (k3 * (x * x * x) + kRowCost * (w * w));
+    }
+    return cost;
+}
+
 // cells per R h_max along an axis as the grid is built: in 3-D at least 2
 // (twice the h split, at most 4): a 5^3 stencil of half-width cells holds
 // 125/8 = 15.6 cell volumes of candidates against 27 for 3^3 full cells
@@ -372,6 +441,54 @@ __host__ __device__ inline double cell_split_dim(double hmax, double hmin, int d
 // Grid set-up (one workgroup): one thread per instance, then a serial prefix
 // of cell bases.
 __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
+    // a single 3-D instance: its h histogram, summed over the rings (which
+    // are emptied for the next launch) by the whole workgroup
+    __shared__ uint32_t hring[kHistSlots];
+    __shared__ double cand_c[kHistSlots + 1], cand_cost[kHistSlots + 1];
+    __shared__ double model_c;
+    uint32_t* hist = ctl->hist;
+    if (hist) {
+        for (int t = threadIdx.x; t < kHistSlots; t += blockDim.x) hring[t] = 0u;
+        __syncthreads();
+        for (int t = threadIdx.x; t < kHistSlots * kHistCopies; t += blockDim.x) {
+            const uint32_t v = ld_dev(&hist[t]);
+            if (v) {
+                st_dev(&hist[t], 0u);
+                atomicAdd(&hring[t % kHistSlots], v);
+            }
+        }
+        __syncthreads();
+        // the model's candidates costed in parallel (one per thread), then
+        // the cheapest, the largest cell on ties
+        const Inst& s0 = inst[0];
+        const double hm = bitsd(ld_dev(&s0.red[0]));
+        const unsigned long long r1 = ld_dev(&s0.red[1]);
+        const double hn = r1 ? bitsd(~r1) : 0.0;
+        const double R = kernel_radius(s0.kernel);
+        const bool on = s0.dim == 3 && hn > 0.0 && hm >= hn && h_level(hm) - h_level(hn) < kHistSlots;
+        const int l0 = on ? h_level(hn) : 0, nl = on ? h_level(hm) - l0 + 1 : 0;
+        const double c0 = R * hm / cell_split_dim(hm, hn, 3);
+        for (int t = threadIdx.x; t <= nl; t += blockDim.x) {
+            const int cand = t - 1;
+            const bool use = cand < 0 || hring[(l0 + cand) & (kHistSlots - 1)] != 0u;
+            const double c = cell_candidate(R, hm, l0, cand, c0);
+            cand_c[t] = c;
+            cand_cost[t] = use ? cell_cost_3d(R, s0.hfact, hm, l0, nl, hring, c) : -1.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double best = cand_c[0], bcost = cand_cost[0];
+            for (int t = 1; t <= nl; ++t) {
+                const double q = cand_cost[t];
+                if (q >= 0.0 && (q < bcost || (q == bcost && cand_c[t] > best))) {
+                    best = cand_c[t];
+                    bcost = q;
+                }
+            }
+            model_c = on ? best : -1.0;
+        }
+        __syncthreads();
+    }
     int k = threadIdx.x;
     for (; k < ninst; k += blockDim.x) {
         Inst& s = inst[k];
@@ -400,6 +517,13 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         // for the largest h
         double split = cell_split_dim(hm, hn, s.dim);
         double c = R * hm / split;
+        // a single 3-D instance: cells sized by the scan-cost model over its
+        // h levels (while h spans less than the ring), at most 8 per R h_max
+        const bool model = hist && s.dim == 3 && hn > 0.0 && hm >= hn && h_level(hm) - h_level(hn) < kHistSlots;
+        if (model) {
+            c = ::fmax(model_c, R * hm / 8.0);
+            split = R * hm / c;
+        }
         double lmax = 0.0;
         for (int a = 0; a < s.dim; ++a) lmax = ::fmax(lmax, s.L[a]);
         if (!(c > 0.0) || !(c <= lmax)) {  // h_max of 0, NaN or above the box: one cell
@@ -421,7 +545,9 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
             }
             if (prod <= cap) break;
             // too many cells: first undo the split, then coarsen
-            if (split > 1.0) {
+            if (model) {
+                c = 2.0 * c;
+            } else if (split > 1.0) {
                 split -= 1.0;
                 c = R * hm / split;
             } else {
@@ -430,8 +556,10 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
         }
         for (int a = 0; a < 3; ++a) s.ceff[a] = a < s.dim ? s.L[a] / s.n[a] : 1.0;
         s.ncell = (int64_t)s.n[0] * s.n[1] * s.n[2];
-        // above 1 only where h varies (the lists then prune by cell h)
+        // above 1 only where h varies (the lists then prune by cell h): with
+        // the model, where the h_max stencil is wider than half cells' 2
         s.split = cell_split(hm, hn) > 1.0 ? (int)split : 1;
+        if (model) s.split = ceil(R * hm / c) > 2.0 ? 2 : 1;
         stencil_from_hmax(s);
     }
     __syncthreads();
@@ -897,16 +1025,17 @@ template <int DIM, class T, class F>
 __device__ __forceinline__ void for_each_candidate(const Inst& s, int64_t key, const int32_t* __restrict__ cstart,
                                                    const int32_t* __restrict__ cend, const StageHdr& H,
                                                    const T* stage, const T* __restrict__ glob, F&& f,
-                                                   const Prune* pr = nullptr) {
+                                                   const Prune* pr = nullptr, const int* srad = nullptr) {
     const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
     const int64_t local = key - s.cell_base;
     const int cx = (int)(local % n0);
     const int cy = (int)((local / n0) % n1);
     const int cz = DIM == 3 ? (int)(local / ((int64_t)n0 * n1)) : 0;
     int lo0, hi0, lo1, hi1, lo2 = 0, hi2 = 0;
-    axis_span(cx, cx, s.srad[0], n0, s.periodic, lo0, hi0);
-    axis_span(cy, cy, s.srad[1], n1, s.periodic, lo1, hi1);
-    if (DIM == 3) axis_span(cz, cz, s.srad[2], n2, s.periodic, lo2, hi2);
+    const int* sr = srad ? srad : s.srad;  // the instance's stencil, or a particle's narrower one
+    axis_span(cx, cx, sr[0], n0, s.periodic, lo0, hi0);
+    axis_span(cy, cy, sr[1], n1, s.periodic, lo1, hi1);
+    if (DIM == 3) axis_span(cz, cz, sr[2], n2, s.periodic, lo2, hi2);
     int xa0[2], xb0[2];
     double sx0[2];
     const int nseg0 = x_pieces(lo0, hi0, n0, xa0, xb0);
@@ -1078,7 +1207,9 @@ struct Dev {
     int32_t *cstart, *cend;
     int32_t *cls, *cle;     // the scan's block-local cell starts and ends (cell_cap)
     unsigned long long* hcell;  // per cell (cell_cap): bits of its largest h, for split grids; then
-                                // (cell_cap more, at a row's first cell) the largest h of the row
+                                // (cell_cap more, at a row's first cell) the largest h of the row;
+                                // then two cell_cap buffers of the h maxima over each cell's stencil
+    uint32_t* hist;             // the h histogram of a single 3-D instance (Ctl::hist), or null
     double* dpack;          // 4 doubles per sorted particle: {x,y,z,m} for the density
                             // passes, then {x,y,z,h} for the neighbour-list build
     int32_t* act[2];        // active-particle lists of the density passes

@@ -51,6 +51,12 @@ __global__ void k_hmax(Fields F, const int32_t* __restrict__ inst_of, Inst* inst
     const double* __restrict__ y = F.f[SPHMHD_F_Y];
     const double* __restrict__ z = F.f[SPHMHD_F_Z];
     const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = min(c0 + chunk, n);
+    uint32_t* hist = ctl->hist;  // h levels for the 3-D cell size
+    __shared__ uint32_t lh[kHistSlots];
+    if (hist) {
+        hist_clear(lh);
+        __syncthreads();
+    }
     int kcur = -1;
     unsigned long long v[kRed] = {};
     auto flush = [&](int k, const unsigned long long* m) {
@@ -80,11 +86,16 @@ __global__ void k_hmax(Fields F, const int32_t* __restrict__ inst_of, Inst* inst
             }
             unsigned long long w[kRed];
             red_values(w, hh[u], xx[u], yy[u], zz[u]);
+            if (hist) hist_add(lh, hh[u]);
             for (int q = 0; q < kRed; ++q) v[q] = w[q] > v[q] ? w[q] : v[q];
         }
     }
     // threads with no element carry kcur = -1 and v = 0
     block_max_runs<kRed>(kcur, v, [&](int k, const unsigned long long* m, bool) { flush(k, m); });
+    if (hist) {
+        __syncthreads();
+        hist_flush(hist, lh);
+    }
     if (last_to_arrive(arrive)) grid_setup(inst, ninst, ctl);
 }
 
@@ -378,6 +389,53 @@ __global__ void k_cell_h(const double4* __restrict__ xh, const uint32_t* __restr
     cell_hmax(hcell + cap, rkey, h, use);
 }
 
+// One axis of the dilation of the per-cell h maxima over the h_max stencil:
+// dst[c] = the largest src over the cells within srad[axis] of c along the
+// axis (wrapped on periodic axes, clipped on open ones; the whole axis where
+// the stencil spans it).  Instances without per-cell values (not split) get 0.
+__global__ void k_cell_dilate(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst,
+                              const Inst* __restrict__ inst, int ninst, int axis, int64_t cap, const Ctl* ctl) {
+    if (ctl->abort) return;
+    const int64_t ncell = min(cap, ctl->total_cells);
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = ninst - 1;  // the instance whose cells hold c
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (inst[mid].cell_base <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const Inst& s = inst[lo];
+        if (s.split <= 1) {
+            dst[c] = 0ull;
+            continue;
+        }
+        const int64_t local = c - s.cell_base;
+        const int n0 = s.n[0], n1 = s.n[1];
+        const int co = axis == 0 ? (int)(local % n0) : (axis == 1 ? (int)((local / n0) % n1) : (int)(local / ((int64_t)n0 * n1)));
+        const int64_t step = axis == 0 ? 1 : (axis == 1 ? (int64_t)n0 : (int64_t)n0 * n1);
+        const int na = s.n[axis], sr = s.srad[axis];
+        const int64_t first = c - (int64_t)co * step;  // the cell of coordinate 0 on this line
+        unsigned long long m = 0ull;
+        if (2 * sr + 1 >= na) {
+            for (int q = 0; q < na; ++q) {
+                const unsigned long long v = src[first + (int64_t)q * step];
+                m = v > m ? v : m;
+            }
+        } else {
+            for (int q = co - sr; q <= co + sr; ++q) {
+                int w = q;
+                if (w < 0 || w >= na) {
+                    if (!s.periodic) continue;
+                    w = w < 0 ? w + na : w - na;
+                }
+                const unsigned long long v = src[first + (int64_t)w * step];
+                m = v > m ? v : m;
+            }
+        }
+        dst[c] = m;
+    }
+}
+
 // before re-running a force pass: forget the partial pass's dt minimum and
 // warning bits (density warnings are kept)
 __global__ void k_reset_force_outputs(Inst* inst, int ninst, Ctl* ctl) {
@@ -408,7 +466,8 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
                                          const Inst* __restrict__ inst, const int32_t* __restrict__ inst_of,
                                          int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                          int stride, Ctl* ctl, int cap, const unsigned long long* __restrict__ hcell,
-                                         const unsigned long long* __restrict__ hrow, int64_t t0) {
+                                         const unsigned long long* __restrict__ hrow,
+                                         const unsigned long long* __restrict__ hreach, int64_t t0) {
     // 2-D blocks stage their candidates in LDS (measured 7-20 % faster): cap
     // records of dynamic LDS, sized with the workgroup; in 3-D the staged runs
     // rarely fit and the LDS would cost occupancy (cap 0)
@@ -442,10 +501,30 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
         // occupancy): with range shifts the stencil is pruned by cell against
         // h_max, or on split grids against each cell's largest h (hcell, when
         // the caller filled it from the current h)
+        // the particle's reach on split grids (hreach: the largest h within
+        // the h_max stencil of its cell): every j with r < R max(h_i, h_j)
+        // has h_j <= that h, so lies within ceil(R max(h_i, h) / c_eff) cells
+        // (an axis scanned whole keeps its order), and the stencil is pruned
+        // geometrically against that h, without reading any cell's own
+        const bool narrow = PRUNE && hreach && s.split > 1;
+        const double hreach_i = narrow ? bitsd(hreach[skey[t]]) : 0.0;
         Prune pr;
-        if (PRUNE && s.range_shift)
-            prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, bitsd(s.hmax_bits),
-                            hcell && s.split > 1 ? hcell : nullptr, hrow);
+        if (PRUNE && s.range_shift) {
+            if (narrow)
+                prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, hreach_i, nullptr, nullptr);
+            else
+                prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, bitsd(s.hmax_bits),
+                                hcell && s.split > 1 ? hcell : nullptr, hrow);
+        }
+        int sri[3] = {s.srad[0], s.srad[1], s.srad[2]};
+        if (narrow) {
+            const double reach = R * fmax(hi, hreach_i);
+            for (int a = 0; a < DIM; ++a) {
+                if (2 * s.srad[a] + 1 >= s.n[a]) continue;
+                const double q = ceil(reach / s.ceff[a]);
+                sri[a] = q < 1.0 ? 1 : (q < (double)s.srad[a] ? (int)q : s.srad[a]);
+            }
+        }
         auto scan = [&](auto shift_c) {
             constexpr bool kShift = decltype(shift_c)::value;
             for_each_candidate<DIM>(s, (int64_t)skey[t], cstart, cend, H, stage, xh,
@@ -459,7 +538,7 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
                     if (cnt < stride) nb[(int64_t)cnt * n + t] = j;
                     ++cnt;
                 }
-            }, PRUNE && kShift ? &pr : nullptr);
+            }, PRUNE && kShift ? &pr : nullptr, narrow ? sri : nullptr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});
@@ -500,7 +579,8 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
                                                      int32_t* __restrict__ nb, int32_t* __restrict__ nbcnt, int64_t n,
                                                      int stride, Ctl* ctl, int cap,
                                                      const unsigned long long* __restrict__ hcell,
-                                                     const unsigned long long* __restrict__ hrow, int32_t* queue,
+                                                     const unsigned long long* __restrict__ hrow,
+                                                     const unsigned long long* __restrict__ hreach, int32_t* queue,
                                                      int32_t* arrive) {
     if (ctl->abort) return;
     const int B = blockDim.x;
@@ -508,11 +588,11 @@ __global__ void __launch_bounds__(kBlock) k_nb_build(const double4* __restrict__
         int polls = 0;
         for (int64_t tile; (tile = pull_tile(queue, (n + B - 1) / B, polls)) >= 0;)
             nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
-                                 hrow, tile * B);
+                                 hrow, hreach, tile * B);
         if (last_to_arrive(arrive)) reset_queues(queue);
     } else {
         nb_block<DIM, PRUNE>(xh, skey, cstart, cend, inst, inst_of, nb, nbcnt, n, stride, ctl, cap, hcell,
-                             hrow, xcd_tile((n + B - 1) / B) * B);
+                             hrow, hreach, xcd_tile((n + B - 1) / B) * B);
     }
 }
 
@@ -583,7 +663,7 @@ void launch_pack_density(Dev& d) {
 }
 
 template <int DIM, bool PRUNE>
-static void nb_build(Dev& d, const unsigned long long* hc) {
+static void nb_build(Dev& d, const unsigned long long* hc, const unsigned long long* hreach) {
     const int B = d.blk[LC_LISTS];
     // 2-D blocks stage candidate runs in LDS, cap records sized with the block
     const int cap = DIM == 2 ? kStageCap * B / kBlock : 0;
@@ -593,7 +673,7 @@ static void nb_build(Dev& d, const unsigned long long* hc) {
                             : xcd_grid((d.n + B - 1) / B);
     hipLaunchKernelGGL((k_nb_build<DIM, PRUNE>), dim3(grid), dim3(B), sizeof(double4) * cap, d.stream,
                        (const double4*)d.dpack, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.nb, d.nbcnt, d.n,
-                       d.nb_stride, d.ctl, cap, hc, hc ? hc + d.cell_cap : nullptr, q, d.arrive);
+                       d.nb_stride, d.ctl, cap, hc, hc ? hc + d.cell_cap : nullptr, hreach, q, d.arrive);
 }
 
 void launch_nb_build(Dev& d, bool cell_h) {
@@ -603,12 +683,27 @@ void launch_nb_build(Dev& d, bool cell_h) {
         hipLaunchKernelGGL(k_cell_h, dim3(nblk(d.n)), dim3(kBlock), 0, d.stream, (const double4*)d.dpack, d.keys,
                            d.inst, d.inst_of, d.hcell, d.cell_cap, d.n, d.ctl);
     const unsigned long long* hc = per_cell ? d.hcell : nullptr;
+    // 3-D: the largest h within each cell's h_max stencil (the cell maxima
+    // dilated along x, y, z), which bounds each particle's reach
+    const unsigned long long* hreach = nullptr;
+    if (per_cell && d.dim == 3) {
+        unsigned long long* t1 = d.hcell + 2 * d.cell_cap;
+        unsigned long long* t2 = d.hcell + 3 * d.cell_cap;
+        const unsigned g = nblk(d.n);
+        hipLaunchKernelGGL(k_cell_dilate, dim3(g), dim3(kBlock), 0, d.stream, (const unsigned long long*)d.hcell, t1,
+                           d.inst, d.ninst, 0, d.cell_cap, d.ctl);
+        hipLaunchKernelGGL(k_cell_dilate, dim3(g), dim3(kBlock), 0, d.stream, (const unsigned long long*)t1, t2,
+                           d.inst, d.ninst, 1, d.cell_cap, d.ctl);
+        hipLaunchKernelGGL(k_cell_dilate, dim3(g), dim3(kBlock), 0, d.stream, (const unsigned long long*)t2, t1,
+                           d.inst, d.ninst, 2, d.cell_cap, d.ctl);
+        hreach = t1;
+    }
     if (d.dim == 2) {
-        if (d.prune_lists) nb_build<2, true>(d, hc);
-        else nb_build<2, false>(d, hc);
+        if (d.prune_lists) nb_build<2, true>(d, hc, nullptr);
+        else nb_build<2, false>(d, hc, nullptr);
     } else {
-        if (d.prune_lists) nb_build<3, true>(d, hc);
-        else nb_build<3, false>(d, hc);
+        if (d.prune_lists) nb_build<3, true>(d, hc, hreach);
+        else nb_build<3, false>(d, hc, nullptr);
     }
 }
 

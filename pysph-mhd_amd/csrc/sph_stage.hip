@@ -257,6 +257,14 @@ template <int NR>
 __device__ __forceinline__ void stage_reduce_begin(int k, unsigned long long (&v)[kRed], Inst* inst, int ninst,
                                                    Ctl* ctl, int32_t* arrive, unsigned long long* hblk,
                                                    int32_t* kblk, bool advance) {
+    if (uint32_t* hist = ctl->hist) {  // h levels for the 3-D cell size
+        __shared__ uint32_t lh[kHistSlots];
+        hist_clear(lh);
+        __syncthreads();
+        if (k >= 0) hist_add(lh, bitsd(v[0]));
+        __syncthreads();
+        hist_flush(hist, lh);
+    }
     bool slot = false;
     block_max_runs<NR>(k, v, [&](int q, const unsigned long long* m, bool whole) {
         if (whole) {

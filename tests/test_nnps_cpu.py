@@ -115,3 +115,38 @@ def test_cell_keys_and_stable_sort(kind):
     want_perm = np.argsort(direct, kind="stable")
     assert np.array_equal(perm, want_perm)
     assert np.array_equal(keys.astype(np.int64), direct[want_perm])
+
+
+@pytest.mark.parametrize("bulk_frac", [1.0, 0.97])
+def test_3d_cells_follow_the_bulk_h(bulk_frac):
+    """A single 3-D instance sizes its cells from its h histogram: where
+    most particles share one h the cells are half of R times that h (to the
+    1/16-octave level edge), whatever a few larger h are (here a ball of
+    particles at 2.5 times the bulk h, which the h_max rule would size the
+    cells for: c = R h_max / 4), and the search still equals the all-pairs
+    one."""
+    from pyoracle import OracleSystem
+    from sphmhd.cases.baseline import blast_3d
+
+    nx = 20
+    s = blast_3d(nx)
+    f = {k: v.copy() for k, v in s.fields.items()}
+    hb = f["h"][0]
+    if bulk_frac < 1.0:
+        r = np.sqrt((f["x"] - 0.5) ** 2 + (f["y"] - 0.5) ** 2 + (f["z"] - 0.5) ** 2)
+        ball = r < np.quantile(r, 1.0 - bulk_frac)
+        f["h"][ball] = 2.5 * hb
+    o = OracleSystem(s.params, s.offsets)
+    o.set_fields(f)
+    keys, _ = o.cells()
+    n = int(round((int(keys.max()) + 1) ** (1.0 / 3.0)))
+    assert n ** 3 == int(keys.max()) + 1  # the corner cell holds a lattice site
+    R = 3.0  # quintic
+    c = 1.0 / n
+    assert 0.5 * R * hb * (1.0 - 1e-12) <= c <= 0.5 * R * hb * 2.0 ** (1.0 / 16.0) * 1.07, (n, c / (R * hb))
+    oc, ic = o.neighbours()
+    ob, ib = o.neighbours_brute()
+    assert np.array_equal(oc, ob)
+    for i in range(0, len(f["x"]), 7):
+        assert np.array_equal(np.sort(ic[oc[i]:oc[i + 1]]), np.sort(ib[ob[i]:ob[i + 1]])), i
+    o.close()
