@@ -216,12 +216,15 @@ def test_c3_blast_32_vs_oracle(gpu):
 
 
 def test_c3_blast_128_properties(gpu):
-    """The full 2,097,152-particle blast over 20 steps (the bench's count):
+    """The full 2,097,152-particle blast over 100 steps (the bench's count):
     mass is the sum of fixed masses; total momentum stays at round-off of
     sqrt(2 M E_kin) (every pair term is antisymmetric; the divergence
     correction only acts where div B has grown); the total energy drifts by
     less than 1e-4 of itself; every field stays finite; particles keep
-    about 60 neighbours."""
+    about 60 neighbours.  By then h spans a 3x range, and the cell keys and
+    permutation of the state, whose grid the scan-cost model sizes from the
+    h histogram, equal the oracle's bit for bit."""
+    from pyoracle import OracleSystem
     from sphmhd.cases.baseline import blast_3d
     from sphmhd.device import DeviceSystem
 
@@ -230,7 +233,7 @@ def test_c3_blast_128_properties(gpu):
     dev.set_fields(init_fields(s))
     dev.warm_start()
     t0 = dev.totals()[0]
-    dev.step(20)
+    dev.step(100)
     t1 = dev.totals()[0]
     mass = float(np.sum(s.fields["m"]))
     assert t1[0] == pytest.approx(mass, rel=1e-13)
@@ -245,7 +248,15 @@ def test_c3_blast_128_properties(gpu):
         assert np.all(np.isfinite(dev.get(nm))), nm
     st = dev.stats()
     assert 50 <= st["pairs_force"] / s.n <= 80
-    assert st["steps"] == 20
+    assert st["steps"] == 100
+    state = dev.state(("x", "y", "z", "h"))
+    assert state["h"].max() > 2.5 * state["h"].min()
+    kd, pd = dev.cells()
+    orc = OracleSystem(s.params, s.offsets, threads=16)
+    orc.set_fields(state)
+    ko, po = orc.cells()
+    orc.close()
+    assert np.array_equal(kd, ko) and np.array_equal(pd, po)
 
 
 # ---- C4: ensemble ----------------------------------------------------------------
