@@ -61,15 +61,22 @@ TRAFFIC_FILE = ROOT / "profiles" / "k_force_hbm_traffic.md"
 # profile summaries as "fp64_flops_per_pair[<workload>]: <flops>".
 FLOPS_FILE = ROOT / "profiles" / "k_force_fp64_ops.md"
 
-# Used only for a workload whose counters are not in FLOPS_FILE: the
-# operation count of the fused reference bodies written out by hand
-# (add/sub/mul/div/sqrt 1, fma 2).
+# Algorithmic FP64 work per neighbour entry of the force pass and of a
+# density pass: the oracle's own arithmetic counted over an operation-counting
+# scalar (oracle/flopcount.cpp, tests/test_flops_cpu.py), kept as
+# "alg_flops_per_pair[<workload>]" and "alg_density_flops_per_pair[<workload>]".
+# The Orszag-Tang variants share the 1M vortex's figures.
+ALG_FLOPS_FILE = ROOT / "profiles" / "k_force_alg_flops.md"
+ALG_FLOPS_KEY = {"ot128": "ot1m", "ot128x64": "ot1m", "ensemble64": "ot1m"}
+
+# Used only for a workload with no counted figure: the operation count of the
+# fused reference bodies written out by hand (add/sub/mul/div/sqrt 1, fma 2).
 FLOPS_PER_PAIR_ESTIMATE = {2: 330.0, 3: 390.0}
 
 # Density pass: compulsory bytes per particle of one full pass (reads x, y,
-# (z), h, m, h0, converged; writes rho, h, omega, converged) and FP64
-# operations per neighbour entry (distance, kernel value and h-derivative,
-# two sums), add/sub/mul/div/sqrt 1, fma 2.
+# (z), h, m, h0, converged; writes rho, h, omega, converged) and, where no
+# counted figure exists, FP64 operations per neighbour entry (distance, kernel
+# value and h-derivative, two sums), add/sub/mul/div/sqrt 1, fma 2.
 DENSITY_BYTES_PER_PARTICLE = {2: 80.0, 3: 88.0}
 DENSITY_FLOPS_PER_PAIR = 30.0
 
@@ -93,12 +100,25 @@ def measured_traffic(workload: str):
 
 
 def flops_per_pair(workload: str, dim: int):
-    """(FP64 flops per neighbour entry of the force pass, how it was
-    obtained): the counted value of the profile summary, else the estimate."""
-    v = _profile_value(FLOPS_FILE, f"fp64_flops_per_pair[{workload}]")
+    """(algorithmic FP64 flops per neighbour entry of the force pass, how it
+    was obtained): the oracle's operation count, else the estimate."""
+    key = ALG_FLOPS_KEY.get(workload, workload)
+    v = _profile_value(ALG_FLOPS_FILE, f"alg_flops_per_pair[{key}]")
     if v is not None:
-        return v, "counted (rocprofv3 FP64 VALU instruction counters)"
+        return v, "algorithmic: the oracle's reference bodies over an operation-counting scalar"
     return FLOPS_PER_PAIR_ESTIMATE[dim], "estimate (hand count of the reference bodies)"
+
+
+def executed_flops_per_pair(workload: str):
+    """FP64 operations the GPU force kernel executes per neighbour entry
+    (rocprofv3 VALU instruction counters), or None when not profiled."""
+    return _profile_value(FLOPS_FILE, f"fp64_flops_per_pair[{workload}]")
+
+
+def density_flops_per_pair(workload: str):
+    key = ALG_FLOPS_KEY.get(workload, workload)
+    v = _profile_value(ALG_FLOPS_FILE, f"alg_density_flops_per_pair[{key}]")
+    return v if v is not None else DENSITY_FLOPS_PER_PAIR
 
 
 def parse():
@@ -258,10 +278,12 @@ def run_tangent(steps: int, warmup: int = 2):
 def roofline(res):
     """Roofline of the dominant kernel (the fused force pass).  Its binding
     roof is the FP64 vector rate (no step is a dense contraction, so no MFMA
-    runs): per launch it does flops_per_pair x pairs of FP64 work on
-    bytes_alg of compulsory HBM traffic, about 20-30 flop/B against a ridge
-    of 9.8 flop/B.  The HBM fraction of the same launch is reported beside
-    it, and the density passes' figures after it."""
+    runs): per launch it does flops_per_pair x pairs of algorithmic FP64 work
+    (the reference bodies' operation count) on bytes_alg of compulsory HBM
+    traffic, about 20-30 flop/B against a ridge of 9.8 flop/B.  The HBM
+    fraction of the same launch, the FP64 rate the kernel's executed
+    instructions give, and the density passes' figures are reported beside
+    it."""
     tm = res["timing"]
     n = res["particles"]
     dim = res["dim"]
@@ -281,7 +303,7 @@ def roofline(res):
     dens = None
     if t_dens > 0:
         dbytes = DENSITY_BYTES_PER_PARTICLE[dim] * n
-        dflops = DENSITY_FLOPS_PER_PAIR * pairs
+        dflops = density_flops_per_pair(res["workload"]) * pairs
         dens = {
             "kernel": "k_density (G0 summation density + Newton step for h), all passes of a step",
             "ms_per_step": round(t_dens * 1e3, 4),
@@ -305,6 +327,9 @@ def roofline(res):
         "launches": tm["force_launches"],
         "flops_per_pair": fpp,
         "flops_per_pair_source": fpp_how,
+        "fp64_executed_per_pair": executed_flops_per_pair(res["workload"]),
+        "fp64_executed_TFLOPs": (round(executed_flops_per_pair(res["workload"]) * pairs / t_force / 1e12, 3)
+                                 if executed_flops_per_pair(res["workload"]) else None),
         "pairs_per_launch": pairs,
         "bytes_alg_per_launch": bytes_alg,
         "hbm": {

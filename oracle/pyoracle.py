@@ -13,7 +13,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "libsphmhd_oracle.so"
-_lib = None
+# the same oracle over an operation-counting scalar (flopcount.cpp)
+COUNTED_LIB = HERE / "build" / "libsphmhd_oracle_counted.so"
+_libs: dict = {}
 
 
 def build() -> Path:
@@ -21,12 +23,11 @@ def build() -> Path:
     return LIB
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB.exists():
+def lib(path: Path = LIB):
+    if path not in _libs:
+        if not path.exists():
             build()
-        L = ctypes.CDLL(str(LIB))
+        L = ctypes.CDLL(str(path))
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         dp = ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = vp
@@ -61,8 +62,10 @@ def lib():
         L.orc_kernel_grad.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp]
         L.orc_kernel_radius.restype = ctypes.c_double
         L.orc_kernel_radius.argtypes = [ctypes.c_int]
-        _lib = L
-    return _lib
+        if path == COUNTED_LIB:
+            L.orcflop_get.argtypes = [ctypes.POINTER(ctypes.c_int64)] * 3
+        _libs[path] = L
+    return _libs[path]
 
 
 def prop_names() -> list[str]:
@@ -81,9 +84,23 @@ BODY = {name: k for k, name in enumerate(BODIES)}
 class OracleSystem:
     """CPU oracle over the same instances/offsets as a DeviceSystem."""
 
-    def __init__(self, params, offsets=None, n=None, threads: int | None = None):
+    def flop_counts(self):
+        """(body, sym, pairs) since the last flop_reset (an
+        OracleSystem(counted=True)): per pass (density, Cullen-Dehnen, MHD
+        accelerations, magnetic stress, magnetic state rates) the counted
+        operations of the loop bodies and the pairs visited, and per pair
+        symbol (r, rho_ij, W_i, grad W_i, grad W_j, dW_i/dh) its operations
+        summed over every pass."""
+        body, sym, pairs = (ctypes.c_int64 * 5)(), (ctypes.c_int64 * 6)(), (ctypes.c_int64 * 5)()
+        self.L.orcflop_get(body, sym, pairs)
+        return tuple(np.array(list(a), dtype=np.int64) for a in (body, sym, pairs))
+
+    def flop_reset(self):
+        self.L.orcflop_reset()
+
+    def __init__(self, params, offsets=None, n=None, threads: int | None = None, counted: bool = False):
         from sphmhd._abi import Params  # noqa: F401  (layout shared with the library)
-        self.L = lib()
+        self.L = lib(COUNTED_LIB if counted else LIB)
         plist = [params] if not isinstance(params, (list, tuple)) else list(params)
         self.params = plist
         self.params_arr = (type(plist[0]) * len(plist))(*plist)

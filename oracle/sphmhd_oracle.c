@@ -24,9 +24,12 @@
  *             k = min(floor(h_max/h_min), 4) when h_max >= 2 h_min, else 1
  *             (cells split where h varies, so the small-h particles do not
  *             scan cells sized for the largest h), and in 3-D k doubled (at
- *             most 4: half-width cells); n_a = max(1, floor(L_a/c)),
- *             while prod n_a > max(1, 8 N) the split is undone first and c
- *             then doubled; c_eff_a = L_a / n_a over the box: the domain on
+ *             most 4: half-width cells); a single 3-D instance whose h spans
+ *             fewer than 64 levels (16 per octave) takes the cell size of
+ *             cell_model_3d instead, at least R h_max / 8;
+ *             n_a = max(1, floor(L_a/c)), while prod n_a > max(1, 8 N) the
+ *             split is undone first and c then doubled (the model's c is
+ *             doubled); c_eff_a = L_a / n_a over the box: the domain on
  *             periodic axes, else the particles' box at this evaluation
  *             widened by box_pad on each side;
  *             key = ix + n_x (iy + n_y iz) with i_a = floor((x_a - lo_a)/c_eff_a)
@@ -1146,23 +1149,45 @@ static void for_neighbours(orc_t* o, int k, int64_t i, const int* srad, visit_fn
 
 enum { PASS_DENSITY, PASS_CD, PASS_MHDACC, PASS_MAGSTRESS, PASS_MAGRATES };
 
+/* Hooks of the operation-counting build: each pair symbol counts apart. */
+#ifndef ORC_SYM_PART
+#define ORC_SYM_PART(k)
+#endif
+
 static void make_sym(orc_t* o, const sphmhd_params* P, int64_t d, int64_t s, const double* xij, double r2, sym_t* y) {
     y->xij[0] = xij[0];
     y->xij[1] = xij[1];
     y->xij[2] = xij[2];
     y->r2ij = r2;
+    ORC_SYM_PART(0);
     y->rij = sqrt(r2);
+    ORC_SYM_PART(1);
     y->rhoij = 0.5 * (D(rho) + S(rho));
+    ORC_SYM_PART(2);
     y->wi = orc_kernel_w(P->kernel, P->dim, y->rij, D(h));
+    ORC_SYM_PART(3);
     orc_kernel_grad(P->kernel, P->dim, y->xij, y->rij, D(h), y->dwi);
+    ORC_SYM_PART(4);
     orc_kernel_grad(P->kernel, P->dim, y->xij, y->rij, S(h), y->dwj);
+    ORC_SYM_PART(5);
     y->ghi = orc_kernel_gradh(P->kernel, P->dim, y->rij, D(h));
 }
+
+/* Hooks of the operation-counting build (flopcount.cpp): a pair's symbols
+   (kernel values and gradients) and its pass's body are counted apart;
+   empty in the plain build. */
+#ifndef ORC_PAIR_BEGIN
+#define ORC_PAIR_BEGIN(pass)
+#define ORC_PAIR_BODY(pass)
+#define ORC_PAIR_END()
+#endif
 
 static void visit_pass(orc_t* o, const sphmhd_params* P, int64_t d, int64_t s, const double* xij, double r2, void* ctx) {
     int pass = *(int*)ctx;
     sym_t y;
+    ORC_PAIR_BEGIN(pass);
     make_sym(o, P, d, s, xij, r2, &y);
+    ORC_PAIR_BODY(pass);
     switch (pass) {
     case PASS_DENSITY: density_loop(o, d, s, &y); break;
     case PASS_CD: cd_loop(o, d, s, &y); break;
@@ -1170,6 +1195,7 @@ static void visit_pass(orc_t* o, const sphmhd_params* P, int64_t d, int64_t s, c
     case PASS_MAGSTRESS: magstress_loop(o, P, d, s, &y); break;
     case PASS_MAGRATES: magrates_loop(o, P, d, s, &y); break;
     }
+    ORC_PAIR_END();
 }
 
 static void pair_pass(orc_t* o, int k, int pass) {
