@@ -804,6 +804,26 @@ struct Prune {
     // false when nothing is left
     __device__ __forceinline__ bool trim(double g2yz, int64_t row, int& xl, int& xh) const {
         if (far(g2yz, hrow ? bitsd(hrow[row]) : hb)) return false;
+        if (!hcell) {
+            // start from the span the reach leaves along x, widened by a
+            // cell each side (the gap tests below trim it exactly: a cell's
+            // gap grows away from the particle), keeping the edge cells of
+            // a periodic axis, whose gap counts as zero
+            const double r = R * ::fmax(hi, hb);
+            const double rx = ::sqrt(::fmax(r * r * (1.0 + 1.0e-12) - g2yz, 0.0));
+            const int a = (int)::floor((x[0] - rx - lo[0]) / c[0]) - 1;
+            const int b = (int)::floor((x[0] + rx - lo[0]) / c[0]) + 1;
+            int nl = xl > a ? xl : a, nh = xh < b ? xh : b;
+            if (per) {
+                for (int q : {-1, 0, n[0] - 1, n[0]})
+                    if (q >= xl && q <= xh) {
+                        nl = q < nl ? q : nl;
+                        nh = q > nh ? q : nh;
+                    }
+            }
+            xl = nl;
+            xh = nh;
+        }
         auto skip = [&](int q) {
             const double g = gap(0, q);
             if (!hcell) return far(g2yz + g * g, hb);
