@@ -898,8 +898,10 @@ static double level_value(int l) { return ldexp(1.0 + (double)(l & 15) / 16.0, (
    edge (capped at h_max) of each occupied level that minimises the modelled
    scan cost: per particle of level edge u, s = max(1, ceil(R u / c)) cells
    per axis, (2s+1)^3 cells of (c/u)^3 h-volumes holding hfact^3 particles
-   each, plus 6 candidates' worth per stencil row ((2s+1)^2); ties keep the
-   larger cell.  cnt[j]: particles of level l0 + j, j < nl. */
+   each, plus 6 candidates' worth per stencil row ((2s+1)^2), times 1.15
+   where the h_max stencil spans more than two cells (the lists then prune
+   and bound each particle's reach); ties keep the larger cell.  cnt[j]:
+   particles of level l0 + j, j < nl. */
 static double cell_model_3d(double R, double hfact, double hm, int l0, int nl, const int64_t* cnt, double c0) {
     const double k3 = hfact * hfact * hfact;
     double best = c0, bcost = 0.0;
@@ -921,6 +923,7 @@ static double cell_model_3d(double R, double hfact, double hm, int l0, int nl, c
             double x = w * c / u;
             cost += (double)cnt[j] * (k3 * (x * x * x) + 6.0 * (w * w));
         }
+        if (ceil(R * hm / c) > 2.0) cost = 1.15 * cost;
         if (cand < 0 || cost < bcost || (cost == bcost && c > best)) {
             best = c;
             bcost = cost;

@@ -400,9 +400,14 @@ __device__ __forceinline__ void hist_flush(uint32_t* hist, const uint32_t* lh) {
 // largest such cell on ties.  Per particle of level upper edge u (capped at
 // h_max): s = max(1, ceil(R u / c)) stencil cells per axis, (2s+1)^3 cells of
 // (c/u)^3 h-volumes holding hfact^3 particles each, plus kRowCost
-// candidates' worth per stencil row ((2s+1)^2).  Contraction off: the
-// oracle makes the same choice from the same counts.
+// candidates' worth per stencil row ((2s+1)^2); a cell whose h_max stencil
+// spans more than two cells makes the lists prune and bound each particle's
+// reach, which costs kSplitCost times as much per modelled candidate (the
+// factor that times best on the 128^3 blast: its grid splits once h_max
+// passes about 1.2 times the bulk h).  Contraction off: the oracle makes the
+// same choice from the same counts.
 constexpr double kRowCost = 6.0;
+constexpr double kSplitCost = 1.15;
 
 // candidate cand (-1: c0) and its modelled cost
 __host__ __device__ inline double cell_candidate(double R, double hm, int l0, int cand, double c0) {
@@ -473,7 +478,9 @@ __device__ inline void grid_setup(Inst* inst, int ninst, Ctl* ctl) {
             const bool use = cand < 0 || hring[(l0 + cand) & (kHistSlots - 1)] != 0u;
             const double c = cell_candidate(R, hm, l0, cand, c0);
             cand_c[t] = c;
-            cand_cost[t] = use ? cell_cost_3d(R, s0.hfact, hm, l0, nl, hring, c) : -1.0;
+            double q = use ? cell_cost_3d(R, s0.hfact, hm, l0, nl, hring, c) : -1.0;
+            if (use && ceil(R * hm / c) > 2.0) q = kSplitCost * q;
+            cand_cost[t] = q;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
