@@ -239,13 +239,19 @@ static void tune_end_step(sphmhd_ctx* c) {
         }
     }
     if (++T.step < ncfg * kTuneRounds) return;
-    // the default (256, plain) unless another configuration's slowest sample
-    // beats the default's fastest by 5% (the early steps the samples come
-    // from differ from each other by a few percent)
+    // the default unless another configuration's slowest sample beats the
+    // default's fastest by 5% (the early steps the samples come from differ
+    // from each other by a few percent).  The default is 256 plain, and 256
+    // persistent in 3-D: a 3-D state that develops a spread of h (the blast's
+    // cavity and shell) loads the pair kernels unevenly, which persistent
+    // workgroups pulling tiles even out (blast 128^3: 6.8 -> 6.2 ms per step),
+    // while the early steps the tuner times show no difference yet
+    const int def = (c->d.dim == 3 && ncfg == kTuneConfigs) ? 5 : 2;
     for (int q = 0; q < kLaunchClasses; ++q) {
-        int best = 2;
+        int best = def;
         for (int k = 0; k < ncfg; ++k)
-            if (k != 2 && T.worst_ms[q][k] < 0.95 * (best == 2 ? T.best_ms[q][2] : T.worst_ms[q][best])) best = k;
+            if (k != def && T.worst_ms[q][k] < 0.95 * (best == def ? T.best_ms[q][def] : T.worst_ms[q][best]))
+                best = k;
         c->d.blk[q] = kShapes[best % 3];
         if (!c->persist_fixed) c->d.persist[q] = best >= 3 ? 1 : 0;
     }
