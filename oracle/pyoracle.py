@@ -15,6 +15,8 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "libsphmhd_oracle.so"
 # the same oracle over an operation-counting scalar (flopcount.cpp)
 COUNTED_LIB = HERE / "build" / "libsphmhd_oracle_counted.so"
+# the same oracle over a taped reverse-mode scalar (tape.cpp)
+TAPED_LIB = HERE / "build" / "libsphmhd_oracle_taped.so"
 _libs: dict = {}
 
 
@@ -64,6 +66,10 @@ def lib(path: Path = LIB):
         L.orc_kernel_radius.argtypes = [ctypes.c_int]
         if path == COUNTED_LIB:
             L.orcflop_get.argtypes = [ctypes.POINTER(ctypes.c_int64)] * 3
+        if path == TAPED_LIB:
+            pp = ctypes.POINTER(dp)
+            L.tape_rollout.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64), pp, ctypes.POINTER(ctypes.c_int32),
+                                       i64, i32, dp, dp, pp, dp, ctypes.POINTER(ctypes.c_int64)]
         _libs[path] = L
     return _libs[path]
 
@@ -259,3 +265,56 @@ def kernel_grad(kernel: int, dim: int, xij, r: float, h: float) -> np.ndarray:
 
 def kernel_radius(kernel: int) -> float:
     return lib().orc_kernel_radius(kernel)
+
+
+def tape_rollout(params, offsets, fields, itype=None, nsteps: int = 0, warm: bool = True, weights=None):
+    """Reference gradient from the taped oracle (tape.cpp): the objective
+    sum_q weights[q] * total_q per instance after the warm start (``warm``)
+    and ``nsteps`` PEC steps from the initial state ``fields`` (property name
+    -> array; the rest zero), and the gradient of the objectives' sum with
+    respect to every initial property and to the parameters of TPARAMS.
+
+    Returns (values [ninst], {property: gradient array}, grad_params
+    [ninst, 6], tape nodes)."""
+    L = lib(TAPED_LIB)
+    names = prop_names()
+    plist = [params] if not isinstance(params, (list, tuple)) else list(params)
+    parr = (type(plist[0]) * len(plist))(*plist)
+    offsets = np.asarray(offsets, dtype=np.int64)
+    ninst, n = len(plist), int(offsets[-1])
+    keep = []
+    fptr = (ctypes.POINTER(ctypes.c_double) * len(names))()
+    gptr = (ctypes.POINTER(ctypes.c_double) * len(names))()
+    grads = {}
+    for q, nm in enumerate(names):
+        if nm in fields:
+            a = np.ascontiguousarray(np.asarray(fields[nm], dtype=np.float64))
+            keep.append(a)
+            fptr[q] = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        g = np.zeros(n)
+        grads[nm] = g
+        gptr[q] = g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    it = None
+    if itype is not None:
+        it = np.ascontiguousarray(np.asarray(itype, dtype=np.int32))
+    elif "itype" in fields:
+        it = np.ascontiguousarray(np.asarray(fields["itype"], dtype=np.int32))
+    else:
+        it = np.ones(n, dtype=np.int32)
+    w = np.zeros(9)
+    if weights is None:
+        w[4] = w[6] = 1.0  # E_kin + E_mag
+    else:
+        w[:] = np.asarray(weights, dtype=np.float64)
+    vals = np.zeros(ninst)
+    gpar = np.zeros((ninst, 6))
+    nodes = ctypes.c_int64(0)
+    rc = L.tape_rollout(ctypes.cast(parr, ctypes.c_void_p), ninst,
+                        offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), fptr,
+                        it.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(nsteps), 1 if warm else 0,
+                        w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), gptr,
+                        gpar.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(nodes))
+    if rc:
+        raise RuntimeError(f"taped oracle rollout failed: {rc}")
+    return vals, grads, gpar, int(nodes.value)
