@@ -1147,7 +1147,7 @@ __host__ __device__ inline double kernel_sigma(int k, int dim) {
 // generic in the scalar type so the tangent step differentiates the same
 // arithmetic
 template <int K, int DIM, class T>
-__device__ __forceinline__ void shape(T q, T& w, T& dw) {
+__host__ __device__ __forceinline__ void shape(T q, T& w, T& dw) {
     if constexpr (K == SPHMHD_CUBIC) {
         T t = fmax(2.0 - q, 0.0);
         if (q > 1.0) {
@@ -1194,7 +1194,7 @@ __device__ __forceinline__ void shape(T q, T& w, T& dw) {
 }
 
 template <int K, int DIM, class T>
-__device__ __forceinline__ T shape_dw(T q) {
+__host__ __device__ __forceinline__ T shape_dw(T q) {
     T w, dw;
     shape<K, DIM>(q, w, dw);
     return dw;
