@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SPHMHD_ABI_VERSION 7
+#define SPHMHD_ABI_VERSION 8
 
 typedef enum sphmhd_status {
     SPHMHD_OK = 0,
@@ -367,6 +367,45 @@ int sphmhd_tangent_step(sphmhd_ctx* ctx, int64_t nsteps);
 int sphmhd_tangent_totals(sphmhd_ctx* ctx, double* out);
 /* Tangents of the current dt, host array n_instances * SPHMHD_NTANGENTS. */
 int sphmhd_tangent_get_dt(sphmhd_ctx* ctx, double* dt_tangent);
+
+/* ---- Adjoint (reverse-mode) step with checkpointed recomputation --------
+ * No counterpart in the reference, which has no derivatives.  It gives the
+ * gradient of a scalar function of the final state of a rollout (the warm
+ * start of application.py:110-147 and PEC steps of integrator.py:12-82 with
+ * the solver's dt of solver.py:57-61) with respect to every property of the
+ * initial state and to the parameters of sphmhd_tparam, in one reverse sweep
+ * whatever their number.  The derivative is that of the discrete map the
+ * forward step computes, as for the tangent-linear step: neighbour sets and
+ * cells fixed, the density Newton iteration as run, branches and min/max as
+ * taken, the adaptive dt through the particle attaining the minimum, a step
+ * clipped to tf - t without derivative.  The rollout's states are kept every
+ * `every` steps; the reverse sweep recomputes each segment from its
+ * checkpoint, then each step's evaluation before reversing it, so memory
+ * grows with steps / every + every states, not with the rollout.
+ *
+ * Bar rows: SPHMHD_NFIELDS device pointers (null where the bound field is
+ * null), n doubles each in storage order, borrowed like the fields; bar row f
+ * holds d(objective)/d(property f) of the current state. */
+int sphmhd_adjoint_bind(sphmhd_ctx* ctx, double* const* bar_fields);
+/* From the current state (the rollout's initial state): the warm start when
+ * warm != 0, then nsteps PEC steps (fewer when the clocks reach tf), keeping
+ * the state every `every` steps.  The fields then hold the final state.
+ * Returns the largest warning of the forward calls. */
+int sphmhd_adjoint_record(sphmhd_ctx* ctx, int32_t warm, int64_t nsteps, int32_t every);
+/* Steps the last record took (-1: none recorded). */
+int sphmhd_adjoint_steps(sphmhd_ctx* ctx, int64_t* steps);
+/* Adds to the bar rows the gradient of sum_k sum_q weights[q] total_q(k)
+ * (the totals of sphmhd_totals, SPHMHD_NTOTALS weights, the divB total's
+ * weight 0) at the current state: seeds the reverse sweep after a record. */
+int sphmhd_adjoint_seed_totals(sphmhd_ctx* ctx, const double* weights);
+/* The reverse sweep of the recorded rollout: the bar rows, holding the
+ * objective's gradient with respect to the final state, end holding it with
+ * respect to the initial state, to which the fields are restored;
+ * param_bars (host, n_instances * SPHMHD_NTPARAMS, or null) receives the
+ * gradient with respect to the parameters of sphmhd_tparam.  SPHMHD_ECAP: a
+ * density iteration ran more than 32 passes; SPHMHD_ESTATE: a recomputed
+ * density iteration ended elsewhere than the recorded one. */
+int sphmhd_adjoint_reverse(sphmhd_ctx* ctx, double* param_bars);
 
 #ifdef __cplusplus
 }

@@ -19,8 +19,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 OUT = HERE / "sphmhd" / "libsphmhd.so"
-SOURCES = ["sph_nnps.hip", "sph_pairs.hip", "sph_stage.hip", "sph_tangent.hip", "sph_lattice.hip", "sph_capi.hip",
-           "sph_totals_static.hip", "sph_cases.hip"]
+SOURCES = ["sph_nnps.hip", "sph_pairs.hip", "sph_stage.hip", "sph_tangent.hip", "sph_adjoint.hip", "sph_lattice.hip",
+           "sph_capi.hip", "sph_totals_static.hip", "sph_cases.hip"]
 ARCH = os.environ.get("SPHMHD_ARCH", "gfx950")
 
 

@@ -29,8 +29,10 @@
 
 #include <hip/hip_runtime.h>
 
-#include "sph_dual.h"
+// sph_internal.h first: its generic kernel shapes must see the double
+// overloads of fmax/exp, the duals' are found through their namespace
 #include "sph_internal.h"
+#include "sph_dual.h"
 
 #define ADJ_HD __host__ __device__ __forceinline__
 
@@ -44,7 +46,6 @@ using ::fmax;
 using ::fmin;
 using ::sqrt;
 
-constexpr int kPassMax = 32;  // density passes one evaluation's reverse sweep holds per particle
 constexpr double kFloatTiny = 2.2250738585072014e-308;
 
 // differentiated parameters (sphmhd_tparam order) and the instance's constants
@@ -82,7 +83,7 @@ struct Acc {
     T vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz;
     T axx, axy, axz, ayx, ayy, ayz, azx, azy, azz;
 };
-constexpr int kNAcc = 35;
+constexpr int kNAcc = kAdjAcc;
 template <class T>
 ADJ_HD T* acc_ptr(Acc<T>& a) { return &a.ax; }
 template <class T>

@@ -120,6 +120,22 @@ struct sphmhd_ctx {
     // tangent-linear step
     TanDev tg;
     bool tbound = false;
+    // adjoint step: bar rows and work arrays, and the recorded rollout
+    // (checkpoints of the state every `every` steps, each with its clocks)
+    struct Adj {
+        bool bound = false;
+        AdjDev a{};
+        bool recorded = false;
+        bool warm = false;
+        int every = 1;
+        int64_t steps = 0;
+        double* init = nullptr;      // SPHMHD_NFIELDS x n + 2 x ninst: the state before the rollout
+        std::vector<double*> ckpt;   // per segment: kNState x n + 2 x ninst
+        double* final_clk = nullptr; // 2 x ninst: the clock after the last step
+        double* seg = nullptr;       // every x kNState x n: the states of one segment's steps
+        double* clk = nullptr;       // (every + 1) x 2 x ninst: their clocks, then the clock after
+        int seg_cap = 0;
+    } adj;
     // launch-shape tuner: the first kTuneSteps steps of sphmhd_step rotate
     // the tuned launches through the configurations (workgroup size, which is
     // also the tile of particles a work item covers, and plain or persistent
@@ -620,6 +636,144 @@ static int count_unconverged(sphmhd_ctx* c, int64_t* out) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// adjoint step: host sequencing
+// ---------------------------------------------------------------------------
+
+static void adj_free(sphmhd_ctx* c) {
+    sphmhd_ctx::Adj& A = c->adj;
+    AdjDev& a = A.a;
+    for (void* p : {(void*)a.acc, (void*)a.argmax, (void*)a.par, (void*)a.dtrow, (void*)a.ain, (void*)a.pass,
+                    (void*)a.npass, (void*)a.bsave, (void*)a.ai, (void*)A.init, (void*)A.final_clk, (void*)A.seg,
+                    (void*)A.clk})
+        if (p) (void)hipFree(p);
+    for (double* p : A.ckpt)
+        if (p) (void)hipFree(p);
+    A.ckpt.clear();
+    double* keep[SPHMHD_NFIELDS];
+    std::memcpy(keep, a.b, sizeof keep);
+    a = AdjDev{};
+    std::memcpy(a.b, keep, sizeof keep);
+    A.init = A.final_clk = A.seg = A.clk = nullptr;
+    A.seg_cap = 0;
+    A.recorded = false;
+}
+
+static RowList rows_state() {
+    RowList L;
+    L.count = kNState;
+    for (int r = 0; r < kNState; ++r) L.f[r] = kStateFields[r];
+    return L;
+}
+
+static RowList rows_all() {
+    RowList L;
+    L.count = SPHMHD_NFIELDS;
+    for (int r = 0; r < SPHMHD_NFIELDS; ++r) L.f[r] = r;
+    return L;
+}
+
+static int adj_check(sphmhd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->adj.bound) return fail(SPHMHD_ESTATE, "no adjoint rows bound");
+    return 0;
+}
+
+// the work arrays of the reverse sweep (once per context)
+static int adj_alloc_work(sphmhd_ctx* c) {
+    Dev& d = c->d;
+    AdjDev& a = c->adj.a;
+    if (a.acc) return 0;
+    const int64_t n = d.n;
+    a.passmax = std::min(std::max(c->max_iter_all, 1), kAdjPassMax);
+    int rc;
+    if ((rc = dalloc(&a.acc, (int64_t)kAdjAcc * n))) return rc;
+    if ((rc = dalloc(&a.argmax, n))) return rc;
+    if ((rc = dalloc(&a.par, (int64_t)SPHMHD_NTPARAMS * n))) return rc;
+    if ((rc = dalloc(&a.dtrow, n))) return rc;
+    if ((rc = dalloc(&a.ain, 5 * n))) return rc;
+    if ((rc = dalloc(&a.pass, (int64_t)5 * a.passmax * n))) return rc;
+    if ((rc = dalloc(&a.npass, n))) return rc;
+    if ((rc = dalloc(&a.bsave, 3 * n))) return rc;
+    if ((rc = dalloc(&a.ai, d.ninst))) return rc;
+    if ((rc = dalloc(&c->adj.final_clk, 2 * (int64_t)d.ninst))) return rc;
+    return 0;
+}
+
+// one PEC step launch by launch: initialize, stage1, the evaluation, stage2,
+// the clock (equal to a step of sphmhd_step bit for bit)
+static int adj_plain_step(sphmhd_ctx* c) {
+    Dev& d = c->d;
+    launch_initialize(d);
+    launch_stage1(d);
+    int rc = do_eval(c);
+    if (rc < 0) return rc;
+    launch_stage2(d);
+    launch_clock(d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// The warm start of sphmhd_warm_start up to a point, from the recorded
+// initial state: upto 0 = the first sync, 1 = the first evaluation, 2 = the
+// field restored and the second sync, 3 = the second evaluation, 4 = the
+// third; the entry of the last evaluation run is saved for its reverse.
+static int adj_warm_forward(sphmhd_ctx* c, int upto) {
+    Dev& d = c->d;
+    AdjDev& a = c->adj.a;
+    double** f = d.fields.f;
+    const int64_t n = d.n;
+    launch_a_rows(d, rows_all(), c->adj.init, true);
+    launch_a_clock(d, c->adj.init + (int64_t)SPHMHD_NFIELDS * n, true);
+    launch_copy(d, d.scratch3, f[SPHMHD_F_BX], n);
+    launch_copy(d, d.scratch3 + n, f[SPHMHD_F_BY], n);
+    launch_copy(d, d.scratch3 + 2 * n, f[SPHMHD_F_BZ], n);
+    launch_copy(d, f[SPHMHD_F_H0], f[SPHMHD_F_H], n);
+    launch_sync_initial(d);
+    int rc;
+    for (int stage = 1; stage <= upto; ++stage) {
+        if (stage == 2) {
+            launch_copy(d, f[SPHMHD_F_BX], d.scratch3, n);
+            launch_copy(d, f[SPHMHD_F_BY], d.scratch3 + n, n);
+            launch_copy(d, f[SPHMHD_F_BZ], d.scratch3 + 2 * n, n);
+            launch_copy(d, f[SPHMHD_F_H0], f[SPHMHD_F_H], n);
+            launch_sync_initial(d);
+            continue;
+        }
+        if (stage == upto) launch_a_save_entry(d, a);
+        if ((rc = do_eval(c)) < 0) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int adj_warm_reverse(sphmhd_ctx* c, const double* clk_next) {
+    Dev& d = c->d;
+    AdjDev& a = c->adj.a;
+    int rc;
+    // the third and second evaluations (the first dt is the third's pending minimum)
+    for (int upto = 4; upto >= 3; --upto) {
+        if ((rc = adj_warm_forward(c, upto))) return rc;
+        launch_a_eval_begin(d, a, clk_next, upto == 4 ? 1 : 2);
+        launch_a_eval(d, a);
+    }
+    // the field restored and the second sync
+    if ((rc = adj_warm_forward(c, 2))) return rc;
+    launch_a_sync(d, a);
+    launch_a_bsave(d, a, true);
+    // the first evaluation
+    if ((rc = adj_warm_forward(c, 1))) return rc;
+    launch_a_eval_begin(d, a, clk_next, 2);
+    launch_a_eval(d, a);
+    // the first sync, and the saved field
+    if ((rc = adj_warm_forward(c, 0))) return rc;
+    launch_a_sync(d, a);
+    launch_a_bsave(d, a, false);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" {
 
 int sphmhd_abi_version(void) { return SPHMHD_ABI_VERSION; }
@@ -799,6 +953,7 @@ int sphmhd_destroy(sphmhd_ctx* c) {
     drop_graphs(c);
     for (void* p : {(void*)c->tg.itan, (void*)c->tg.aprev, (void*)c->tg.bsave, (void*)c->tg.totals})
         if (p) (void)hipFree(p);
+    adj_free(c);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->gev) (void)hipEventDestroy(c->gev);
     if (c->sev) (void)hipEventDestroy(c->sev);
@@ -1427,6 +1582,162 @@ int sphmhd_tangent_get_dt(sphmhd_ctx* c, double* dt_tangent) {
     HIPCHK(hipStreamSynchronize(c->d.stream));
     for (int k = 0; k < c->d.ninst; ++k)
         for (int q = 0; q < kTanDirs; ++q) dt_tangent[(int64_t)k * kTanDirs + q] = h[k].dt[q];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (reverse-mode) step
+// ---------------------------------------------------------------------------
+
+int sphmhd_adjoint_bind(sphmhd_ctx* c, double* const* bar_fields) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!bar_fields) return fail(SPHMHD_EINVAL, "null argument");
+    for (int f = 0; f < SPHMHD_NFIELDS; ++f)
+        if (!bar_fields[f] && c->d.fields.f[f])
+            return fail(SPHMHD_EINVAL, "adjoint row " + std::to_string(f) + " is null");
+    for (int f = 0; f < SPHMHD_NFIELDS; ++f) c->adj.a.b[f] = c->d.fields.f[f] ? bar_fields[f] : nullptr;
+    c->adj.bound = true;
+    return 0;
+}
+
+int sphmhd_adjoint_record(sphmhd_ctx* c, int32_t warm, int64_t nsteps, int32_t every) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (nsteps < 0 || every < 1) return fail(SPHMHD_EINVAL, "nsteps < 0 or every < 1");
+    Dev& d = c->d;
+    sphmhd_ctx::Adj& A = c->adj;
+    const int64_t n = d.n;
+    const int64_t sn = (int64_t)kNState * n + 2 * d.ninst;
+    if ((rc = adj_alloc_work(c))) return rc;
+    for (double* p : A.ckpt)
+        if (p) (void)hipFree(p);
+    A.ckpt.clear();
+    A.recorded = false;
+    if (!A.init && (rc = dalloc(&A.init, (int64_t)SPHMHD_NFIELDS * n + 2 * d.ninst))) return rc;
+    if (A.seg_cap < every) {
+        if (A.seg) (void)hipFree(A.seg);
+        if (A.clk) (void)hipFree(A.clk);
+        A.seg = A.clk = nullptr;
+        if ((rc = dalloc(&A.seg, (int64_t)every * kNState * n))) return rc;
+        if ((rc = dalloc(&A.clk, (int64_t)(every + 1) * 2 * d.ninst))) return rc;
+        A.seg_cap = every;
+    }
+    launch_a_rows(d, rows_all(), A.init, false);
+    launch_a_clock(d, A.init + (int64_t)SPHMHD_NFIELDS * n, false);
+    int worst = 0;
+    if (warm) {
+        rc = sphmhd_warm_start(c);
+        if (rc < 0) return rc;
+        worst = std::max(worst, rc);
+    }
+    int64_t taken = 0;
+    while (taken < nsteps) {
+        double* ck = nullptr;
+        if ((rc = dalloc(&ck, sn))) return rc;
+        A.ckpt.push_back(ck);
+        launch_a_rows(d, rows_state(), ck, false);
+        launch_a_clock(d, ck + (int64_t)kNState * n, false);
+        const int64_t len = std::min<int64_t>(every, nsteps - taken);
+        const int64_t before = c->stats.steps;
+        rc = sphmhd_step(c, len);
+        if (rc < 0) return rc;
+        worst = std::max(worst, rc);
+        const int64_t got = c->stats.steps - before;
+        taken += got;
+        if (got < len) break;  // the clocks reached tf
+    }
+    launch_a_clock(d, A.final_clk, false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(d.stream));
+    A.warm = warm != 0;
+    A.every = every;
+    A.steps = taken;
+    A.recorded = true;
+    return worst;
+}
+
+int sphmhd_adjoint_steps(sphmhd_ctx* c, int64_t* steps) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!steps) return fail(SPHMHD_EINVAL, "null argument");
+    *steps = c->adj.recorded ? c->adj.steps : -1;
+    return 0;
+}
+
+int sphmhd_adjoint_seed_totals(sphmhd_ctx* c, const double* weights) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!weights) return fail(SPHMHD_EINVAL, "null argument");
+    if (weights[SPHMHD_T_DIVB] != 0.0) return fail(SPHMHD_EINVAL, "the divB total has no derivative: weight 0");
+    Weights w;
+    for (int q = 0; q < SPHMHD_NTOTALS; ++q) w.w[q] = weights[q];
+    launch_a_seed_totals(c->d, c->adj.a, w);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sphmhd_adjoint_reverse(sphmhd_ctx* c, double* param_bars) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    sphmhd_ctx::Adj& A = c->adj;
+    if (!A.recorded) return fail(SPHMHD_ESTATE, "no recorded rollout");
+    Dev& d = c->d;
+    AdjDev& a = A.a;
+    const int64_t n = d.n;
+    const int ninst = d.ninst;
+    HIPCHK(hipMemsetAsync(a.ai, 0, sizeof(AdjInst) * ninst, d.stream));
+    launch_a_zero(d, a.par, (int64_t)SPHMHD_NTPARAMS * n);
+    const int64_t nseg = (int64_t)A.ckpt.size();
+    const int64_t sn = (int64_t)kNState * n;
+    for (int64_t s = nseg - 1; s >= 0; --s) {
+        const int64_t len = std::min<int64_t>(A.every, A.steps - s * A.every);
+        // the segment again from its checkpoint, keeping each step's state and clock
+        launch_a_rows(d, rows_state(), A.ckpt[s], true);
+        launch_a_clock(d, A.ckpt[s] + sn, true);
+        for (int64_t j = 0; j < len; ++j) {
+            launch_a_rows(d, rows_state(), A.seg + j * sn, false);
+            launch_a_clock(d, A.clk + 2 * ninst * j, false);
+            if ((rc = adj_plain_step(c))) return rc;
+        }
+        launch_a_clock(d, A.clk + 2 * ninst * len, false);
+        // its steps backwards: initialize, stage1 and the evaluation again, then reversed
+        for (int64_t j = len - 1; j >= 0; --j) {
+            launch_a_rows(d, rows_state(), A.seg + j * sn, true);
+            launch_a_clock(d, A.clk + 2 * ninst * j, true);
+            launch_initialize(d);
+            launch_stage1(d);
+            launch_a_save_entry(d, a);
+            if ((rc = do_eval(c)) < 0) return rc;
+            a.snap = A.seg + j * sn;
+            launch_a_eval_begin(d, a, A.clk + 2 * ninst * (j + 1), 0);
+            launch_a_stage2(d, a);
+            launch_a_eval(d, a);
+            launch_a_stage1_init(d, a);
+            launch_a_reduce_dt(d, a);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    if (A.warm) {
+        const double* clk_next = nseg ? A.ckpt[0] + sn : A.final_clk;
+        if ((rc = adj_warm_reverse(c, clk_next))) return rc;
+    }
+    launch_a_reduce_par(d, a);
+    // leave the fields at the recorded initial state (the bars are its gradient)
+    launch_a_rows(d, rows_all(), A.init, true);
+    launch_a_clock(d, A.init + (int64_t)SPHMHD_NFIELDS * n, true);
+    HIPCHK(hipGetLastError());
+    std::vector<AdjInst> h(ninst);
+    HIPCHK(hipMemcpyAsync(h.data(), a.ai, sizeof(AdjInst) * ninst, hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    for (int k = 0; k < ninst; ++k) {
+        if (h[k].status & 2)
+            return fail(SPHMHD_ECAP, "adjoint: a density iteration ran more passes than the reverse sweep holds");
+        if (h[k].status & 1)
+            return fail(SPHMHD_ESTATE, "adjoint: the recomputed density iteration differs from the forward one");
+        if (param_bars)
+            for (int p = 0; p < SPHMHD_NTPARAMS; ++p) param_bars[(int64_t)k * SPHMHD_NTPARAMS + p] = h[k].par[p];
+    }
     return 0;
 }
 

@@ -1295,6 +1295,79 @@ void launch_t_save(Dev& d, TanDev& g, int field, double* buf, bool restore);
 void launch_t_copy(Dev& d, TanDev& g, int dst, int src);
 void launch_t_totals(Dev& d, TanDev& g, double* out);
 
+// adjoint (reverse-mode) step (sph_adjoint.hip)
+constexpr int kAdjBlock = 128;    // workgroup of the reverse pair and density kernels
+constexpr int kAdjAcc = 35;       // sums of the force pass (their bars: AdjDev::acc rows)
+constexpr int kAdjPassMax = 32;   // density passes one evaluation's reverse sweep holds per particle
+
+// the properties a PEC step reads from the step before it (the state a step
+// is recomputed from); the rest is written by initialize or the evaluation
+// before it is read
+enum {
+    kSX = 0, kSY, kSZ, kSU, kSV, kSW, kSE, kSH, kSALPHA1, kSBX, kSBY, kSBZ, kSPSI,
+    kSAU, kSAV, kSAW, kSAE, kSAH, kSCS, kSALPHALOC, kSABX, kSABY, kSABZ, kSAPSI, kNState
+};
+constexpr int kStateFields[kNState] = {
+    SPHMHD_F_X, SPHMHD_F_Y, SPHMHD_F_Z, SPHMHD_F_U, SPHMHD_F_V, SPHMHD_F_W, SPHMHD_F_E, SPHMHD_F_H,
+    SPHMHD_F_ALPHA1, SPHMHD_F_BEVOLX, SPHMHD_F_BEVOLY, SPHMHD_F_BEVOLZ, SPHMHD_F_PSI,
+    SPHMHD_F_AU, SPHMHD_F_AV, SPHMHD_F_AW, SPHMHD_F_AE, SPHMHD_F_AH, SPHMHD_F_CS, SPHMHD_F_ALPHALOC,
+    SPHMHD_F_ABEVOLX, SPHMHD_F_ABEVOLY, SPHMHD_F_ABEVOLZ, SPHMHD_F_APSI};
+
+// property rows to copy (kernel argument)
+struct RowList {
+    int count;
+    int f[SPHMHD_NFIELDS];
+};
+struct Weights {
+    double w[SPHMHD_NTOTALS];
+};
+
+// per instance: bars of the dt of the step being reversed and of the
+// pending dt of the evaluation being reversed, the particle attaining that
+// pending minimum, the largest h of the evaluation's density passes, the
+// parameter bars and the status (1: the recomputed density iteration ended
+// elsewhere than the forward one, 2: more passes than the sweep holds)
+struct AdjInst {
+    double dtbar;
+    double pendbar;
+    double carry;  // fixed steps: the next dt's bar carried to this step's dt
+    double par[SPHMHD_NTPARAMS];
+    unsigned long long hmax_bits;
+    int32_t argmin;
+    int32_t status;
+};
+
+struct AdjDev {
+    double* b[SPHMHD_NFIELDS];  // bar rows (the caller's), null where the field is null
+    double* acc;        // 35 x n: bars of the force pass's sums (storage order)
+    int32_t* argmax;    // n: the neighbour attaining dt_cfl's maximum (-1: none)
+    double* par;        // SPHMHD_NTPARAMS x n: per-particle parameter bars
+    double* dtrow;      // n: per-particle bars of the step's dt
+    double* ain;        // 5 x n: h, converged, au, av, aw entering the evaluation
+    double* pass;       // 5 x passmax x n: per density pass h, sw, sg (values, then bars), dsw/dh, dsg/dh
+    int32_t* npass;     // n: density passes the particle ran
+    double* bsave;      // 3 x n: warm start: the bar of the saved initial B
+    AdjInst* ai;        // ninst
+    const double* snap; // kNState x n: the state the step being reversed started from
+    int32_t passmax;
+};
+
+void launch_a_rows(Dev& d, const RowList& L, double* buf, bool restore);
+void launch_a_clock(Dev& d, double* clk, bool restore);
+void launch_a_save_entry(Dev& d, AdjDev& a);
+void launch_a_seed_totals(Dev& d, AdjDev& a, const Weights& w);
+void launch_a_zero(Dev& d, double* p, int64_t count);
+// mode 0: the step clock (adaptive: the pending minimum; fixed: the current
+// dt), 1: the warm start's first dt (the pending minimum), 2: none
+void launch_a_eval_begin(Dev& d, AdjDev& a, const double* clk_next, int mode);
+void launch_a_stage2(Dev& d, AdjDev& a);
+void launch_a_eval(Dev& d, AdjDev& a);
+void launch_a_stage1_init(Dev& d, AdjDev& a);
+void launch_a_reduce_dt(Dev& d, AdjDev& a);
+void launch_a_reduce_par(Dev& d, AdjDev& a);
+void launch_a_sync(Dev& d, AdjDev& a);
+void launch_a_bsave(Dev& d, AdjDev& a, bool take);
+
 void launch_eval_begin(Dev& d, bool reset_dt);
 void launch_pack_xh(Dev& d);
 void launch_reset_nb_stats(Dev& d);

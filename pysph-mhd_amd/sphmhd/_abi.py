@@ -9,7 +9,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 # sphmhd_field, in header order (names are the reference's property names,
 # particles.py:1-162)
@@ -245,6 +245,11 @@ def load_library() -> ctypes.CDLL:
         "sphmhd_tangent_step": ([vp, i64], ctypes.c_int),
         "sphmhd_tangent_totals": ([vp, dp], ctypes.c_int),
         "sphmhd_tangent_get_dt": ([vp, dp], ctypes.c_int),
+        "sphmhd_adjoint_bind": ([vp, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "sphmhd_adjoint_record": ([vp, i32, i64, i32], ctypes.c_int),
+        "sphmhd_adjoint_steps": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "sphmhd_adjoint_seed_totals": ([vp, dp], ctypes.c_int),
+        "sphmhd_adjoint_reverse": ([vp, dp], ctypes.c_int),
         "sphmhd_set_timing": ([vp, i32], ctypes.c_int),
         "sphmhd_set_launch": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "sphmhd_hcp_lattice": ([ctypes.POINTER(ctypes.c_double), ctypes.c_double, vp, vp, vp, i64,
