@@ -131,6 +131,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the C5 gradient by the adjoint step against the plain rollout (one JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 512 MiB FP64 buffer 5 times (scales HBM byte counters)")
     return ap.parse_args()
@@ -272,6 +274,69 @@ def run_tangent(steps: int, warmup: int = 2):
         del d
     torch.cuda.empty_cache()
     out["tangent_over_plain"] = round(out["tangent_ms_per_step"] / out["plain_ms_per_step"], 3)
+    return out
+
+
+def run_adjoint(steps: int = 400, every: int = 20):
+    """The gradient of the final E_kin + E_mag of the C5 rollout (the
+    travelling Alfven wave, 97x168, warm start + `steps` steps) with respect
+    to every initial property and the six parameters, by the adjoint step:
+    the forward rollout recorded with checkpoints every `every` steps, then
+    one reverse sweep (each segment recomputed from its checkpoint, each
+    step's evaluation recomputed before it is reversed).  Beside it, the plain
+    rollout (warm start + `steps` steps) on the same context type."""
+    import torch
+
+    from sphmhd.adjoint import ENERGY, AdjointSystem
+    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.device import DeviceSystem
+
+    setup = alfven_2d(97, 168)
+    out = {"desc": "alfven_2d 97x168 (16,296 particles), wendland_c6, h=1.9dx: gradient of the final E_kin + E_mag "
+                   "with respect to every initial property and the six parameters",
+           "particles": setup.n, "steps": steps, "checkpoint_every": every}
+    d = DeviceSystem(setup.params, setup.offsets, device="cuda:0")
+    d.set_fields(dict(setup.fields))
+    d.warm_start()
+    d.step(TUNE_STEPS + 2)
+    d.set_fields(dict(setup.fields))
+    d.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.warm_start()
+    d.step(steps)
+    d.synchronize()
+    fwd = time.perf_counter() - t0
+    d.close()
+    a = AdjointSystem(setup.params, setup.offsets, device="cuda:0")
+    a.set_fields(dict(setup.fields))
+    a.record(TUNE_STEPS + 2, every)  # launch-shape tuning and first-call costs, untimed
+    a.zero_bars()
+    a.seed_totals(ENERGY)
+    a.reverse()
+    a.set_fields(dict(setup.fields))
+    a.zero_bars()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a.record(steps, every)
+    torch.cuda.synchronize()
+    rec = time.perf_counter() - t0
+    a.seed_totals(ENERGY)
+    t0 = time.perf_counter()
+    a.reverse()
+    torch.cuda.synchronize()
+    rev = time.perf_counter() - t0
+    a.close()
+    torch.cuda.empty_cache()
+    out["forward_ms"] = round(1e3 * fwd, 2)
+    out["forward_ms_per_step"] = round(1e3 * fwd / steps, 4)
+    out["record_ms"] = round(1e3 * rec, 2)
+    out["reverse_ms"] = round(1e3 * rev, 2)
+    out["reverse_ms_per_step"] = round(1e3 * rev / steps, 4)
+    out["gradient_over_forward"] = round((rec + rev) / fwd, 3)
+    out["reverse_over_forward"] = round(rev / fwd, 3)
+    out["note"] = ("gradient_over_forward: (record + reverse sweep) / plain rollout, warm start included in both; "
+                   "the reverse sweep recomputes each segment and each step's evaluation from the checkpoints")
     return out
 
 
@@ -421,6 +486,9 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.adjoint_only:
+        print(json.dumps(run_adjoint(400, 20)))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs one GPU per context; sharding a workload over several GPUs "
                                    "is not implemented, so --gpus must be 1"}))
@@ -457,6 +525,7 @@ def main():
                 "what": "particle-steps/s of 64 OT 128x128 instances in one batch over one OT 128x128 instance alone",
             }
         extras["fwd_tangent"] = run_tangent(100)
+        extras["fwd_adjoint"] = run_adjoint(400, 20)
     line = {
         "metric": METRIC,
         "value": round(head["particle_steps_per_s"], 1),

@@ -91,17 +91,74 @@ __device__ __forceinline__ Raw<double> load_raw(const Fields& F, const AdjDev& A
     return r;
 }
 
+// a neighbour's side from its force-pack record (sorted position j): the
+// values the primal pair loop reads, the derived ones rebuilt with side_of's
+// operations
 template <int DIM>
-__device__ __forceinline__ void sep(const Inst& s, const Raw<double>& a, const Raw<double>& b, double* d) {
-    d[0] = min_image(a.x - b.x, s.L[0], s.halfL[0], s.periodic);
-    d[1] = min_image(a.y - b.y, s.L[1], s.halfL[1], s.periodic);
-    d[2] = DIM == 3 ? min_image(a.z - b.z, s.L[2], s.halfL[2], s.periodic) : 0.0;
+__device__ __forceinline__ Side<double> side_of_rec(const double* __restrict__ fpack, int64_t j, const Par<double>& P,
+                                                    double* pos) {
+    using R = FRec<DIM>;
+    const double* r = fpack + (int64_t)R::N * j;
+    Side<double> s;
+    pos[0] = r[R::X];
+    pos[1] = r[R::Y];
+    pos[2] = DIM == 3 ? r[R::Z] : 0.0;
+    s.h1 = r[R::H1];
+    s.m = r[R::M];
+    s.rho = r[R::RHO];
+    s.e = r[R::E];
+    s.p = P.gamma1 * s.rho * s.e;
+    s.u = r[R::U];
+    s.v = r[R::V];
+    s.w = r[R::W];
+    s.om = r[R::OM];
+    s.bx = r[R::BX];
+    s.by = r[R::BY];
+    s.bz = r[R::BZ];
+    s.al = r[R::AL];
+    s.au = r[R::AU];
+    s.av = r[R::AV];
+    s.aw = DIM == 3 ? r[FRec<3>::AW] : 0.0;
+    s.psich = r[R::PSICH];
+    s.vw = r[R::VW];
+    s.hir = 0.5 * (1.0 / s.rho);
+    return s;
 }
 
-__device__ __forceinline__ void load_acc(const AdjDev& A, int dim, int64_t i, int64_t n, Acc<double>& B) {
+template <int DIM>
+__device__ __forceinline__ void sep_pos(const Inst& s, const double* a, const double* b, double* d) {
+    d[0] = min_image(a[0] - b[0], s.L[0], s.halfL[0], s.periodic);
+    d[1] = min_image(a[1] - b[1], s.L[1], s.halfL[1], s.periodic);
+    d[2] = DIM == 3 ? min_image(a[2] - b[2], s.L[2], s.halfL[2], s.periodic) : 0.0;
+}
+
+// the bars of particle t's sums (rows by sorted position)
+template <int DIM>
+__device__ __forceinline__ void load_acc(const AdjDev& A, int64_t t, int64_t n, Acc<double>& B) {
     double* p = acc_ptr(B);
 #pragma unroll
-    for (int k = 0; k < kNAcc; ++k) p[k] = acc_used(k, dim) ? A.acc[(int64_t)k * n + i] : 0.0;
+    for (int k = 0; k < kNAcc; ++k) p[k] = acc_used(k, DIM) ? A.acc[(int64_t)k * n + t] : 0.0;
+}
+
+// A group of kAdjLanes consecutive lanes shares one particle: each lane takes
+// every kAdjLanes-th neighbour (or candidate), and the lanes' partial sums
+// are combined by a butterfly of shuffles (a fixed order: the results do not
+// depend on the launch).
+__device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+    for (int off = kAdjLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ double gmax(double v) {
+#pragma unroll
+    for (int off = kAdjLanes / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+template <class S>
+__device__ __forceinline__ void gsum_all(S& x) {
+    double* p = reinterpret_cast<double*>(&x);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(S) / sizeof(double)); ++k) p[k] = gsum(p[k]);
 }
 
 }  // namespace
@@ -293,34 +350,56 @@ __global__ void k_a_stage2(Fields F, AdjDev A, const Inst* __restrict__ inst, co
 // ---------------------------------------------------------------------------
 
 template <int K, int DIM>
-__global__ void __launch_bounds__(kAdjBlock) k_a_force_epi(Fields F, AdjDev A, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(kAdjBlock) k_a_force_epi(Fields F, AdjDev A, const double* __restrict__ fpack,
+                                                          const int32_t* __restrict__ perm,
                                                           const int32_t* __restrict__ nb,
                                                           const int32_t* __restrict__ nbcnt,
                                                           const Inst* __restrict__ inst,
                                                           const int32_t* __restrict__ inst_of, int64_t n,
                                                           int stride) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = gid / kAdjLanes;
+    const int lane = (int)(gid % kAdjLanes);
+    if (t >= n) return;  // the whole group
     const int32_t i = perm[t];
     const int ki = inst_of[t];
     const Inst& s = inst[ki];
     const Par<double> P = par_of<K, DIM>(s);
     const Raw<double> me = load_raw<DIM>(F, A, i, n);
     const Side<double> S = side_of<DIM>(me, P);
+    const double xi[3] = {me.x, me.y, me.z};
     Acc<double> acc;
     acc_zero(acc);
-    int32_t amax = -1;
+    // the maximum pair candidate of dt_cfl and the first list slot attaining
+    // it (the forward max keeps its first operand on ties; -1: the initial 0)
+    double best = 0.0;
+    int qbest = -1;
     const int cnt = min(nbcnt[t], stride);
-    for (int q = 0; q < cnt; ++q) {
-        const int32_t jj = perm[nb[(int64_t)q * n + t]];
-        const Raw<double> o = load_raw<DIM>(F, A, jj, n);
-        const Side<double> So = side_of<DIM>(o, P);
+    for (int q = lane; q < cnt; q += kAdjLanes) {
+        const int32_t j = nb[(int64_t)q * n + t];
+        double xj[3];
+        const Side<double> So = side_of_rec<DIM>(fpack, j, P, xj);
         double d[3];
-        sep<DIM>(s, me, o, d);
-        const double old = acc.dtcfl;
+        sep_pos<DIM>(s, xi, xj, d);
         const double c = pair_fwd<K, DIM>(S, So, d[0], d[1], d[2], P, acc);
-        if (c > old) amax = jj;
+        if (c > best) {
+            best = c;
+            qbest = q;
+        }
     }
+    double* ap0 = acc_ptr(acc);
+#pragma unroll
+    for (int k = 0; k < kNAcc; ++k)
+        if (k != 5 && acc_used(k, DIM)) ap0[k] = gsum(ap0[k]);
+    acc.dtcfl = gmax(acc.dtcfl);
+    {
+        int qb = (best == acc.dtcfl && qbest >= 0) ? qbest : INT_MAX;
+#pragma unroll
+        for (int off = kAdjLanes / 2; off > 0; off >>= 1) qb = min(qb, __shfl_xor(qb, off));
+        qbest = qb;
+    }
+    if (lane != 0) return;
+    const int32_t amax = qbest == INT_MAX ? -1 : nb[(int64_t)qbest * n + t];
     Epi<double> O;
     O.ax = BAR(AU);
     O.ay = BAR(AV);
@@ -346,8 +425,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_a_force_epi(Fields F, AdjDev A, c
     const double* ap = acc_ptr(Ab);
 #pragma unroll
     for (int k = 0; k < kNAcc; ++k)
-        if (acc_used(k, DIM)) A.acc[(int64_t)k * n + i] = ap[k];
-    A.argmax[i] = amax;
+        if (acc_used(k, DIM)) A.acc[(int64_t)k * n + t] = ap[k];
+    A.argmax[t] = amax;
     // the sums' outputs are consumed; what the epilogue read of the particle
     // adds to the bars of those values
     BAR(AE) = 0.0;
@@ -372,59 +451,68 @@ __global__ void __launch_bounds__(kAdjBlock) k_a_force_epi(Fields F, AdjDev A, c
 }
 
 template <int K, int DIM>
-__global__ void __launch_bounds__(kAdjBlock) k_a_force_pair(Fields F, AdjDev A, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(kAdjBlock) k_a_force_pair(Fields F, AdjDev A, const double* __restrict__ fpack,
+                                                           const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ nb,
                                                            const int32_t* __restrict__ nbcnt,
                                                            const Inst* __restrict__ inst,
                                                            const int32_t* __restrict__ inst_of, int64_t n,
                                                            int stride) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = gid / kAdjLanes;
+    const int lane = (int)(gid % kAdjLanes);
+    if (t >= n) return;  // the whole group
     const int32_t i = perm[t];
     const Inst& s = inst[inst_of[t]];
     const Par<double> P = par_of<K, DIM>(s);
     const Raw<double> me = load_raw<DIM>(F, A, i, n);
     const Side<double> S = side_of<DIM>(me, P);
+    const double xi[3] = {me.x, me.y, me.z};
     Acc<double> Bi;
-    load_acc(A, DIM, i, n, Bi);
-    const int32_t amax_i = A.argmax[i];
-    SideBar own{};
-    double xb[3] = {0.0, 0.0, 0.0};
-    PairParBar pb{0.0, 0.0, 0.0};
+    load_acc<DIM>(A, t, n, Bi);
+    const int32_t amax_i = A.argmax[t];
+    struct Part {
+        SideBar own;
+        double xb[3];
+        PairParBar pb;
+    } part{};
     const int cnt = min(nbcnt[t], stride);
-    for (int q = 0; q < cnt; ++q) {
-        const int32_t jj = perm[nb[(int64_t)q * n + t]];
-        const Raw<double> o = load_raw<DIM>(F, A, jj, n);
-        const Side<double> So = side_of<DIM>(o, P);
+    for (int q = lane; q < cnt; q += kAdjLanes) {
+        const int32_t j = nb[(int64_t)q * n + t];
+        double xj[3];
+        const Side<double> So = side_of_rec<DIM>(fpack, j, P, xj);
         double d[3];
-        sep<DIM>(s, me, o, d);
-        // i as the destination: the bars of i's sums
+        sep_pos<DIM>(s, xi, xj, d);
+        // t as the destination: the bars of t's sums
         {
             SideBar jb{};
             double db[3] = {0.0, 0.0, 0.0};
-            pair_rev<K, DIM>(S, So, d[0], d[1], d[2], P, Bi, amax_i == jj, own, jb, db, pb);
-            xb[0] += db[0];
-            xb[1] += db[1];
-            xb[2] += db[2];
+            pair_rev<K, DIM>(S, So, d[0], d[1], d[2], P, Bi, amax_i == j, part.own, jb, db, part.pb);
+            part.xb[0] += db[0];
+            part.xb[1] += db[1];
+            part.xb[2] += db[2];
         }
-        // i as a source of jj: the bars of jj's sums (the separation is -d)
+        // t as a source of j: the bars of j's sums (the separation is -d)
         {
             Acc<double> Bj;
-            load_acc(A, DIM, jj, n, Bj);
+            load_acc<DIM>(A, j, n, Bj);
             SideBar ib{};
             double db[3] = {0.0, 0.0, 0.0};
-            PairParBar pdrop{0.0, 0.0, 0.0};  // counted by jj's own thread
-            pair_rev<K, DIM>(So, S, -d[0], -d[1], -d[2], P, Bj, A.argmax[jj] == i, ib, own, db, pdrop);
-            xb[0] -= db[0];
-            xb[1] -= db[1];
-            xb[2] -= db[2];
+            PairParBar pdrop{0.0, 0.0, 0.0};  // counted by j's own group
+            pair_rev<K, DIM>(So, S, -d[0], -d[1], -d[2], P, Bj, A.argmax[j] == t, ib, part.own, db, pdrop);
+            part.xb[0] -= db[0];
+            part.xb[1] -= db[1];
+            part.xb[2] -= db[2];
         }
     }
+    gsum_all(part);
+    if (lane != 0) return;
+    const SideBar& own = part.own;
     RawBar rb{};
     side_rev(me, S, P, own, rb);
-    BAR(X) += xb[0];
-    BAR(Y) += xb[1];
-    if (DIM == 3) BAR(Z) += xb[2];
+    BAR(X) += part.xb[0];
+    BAR(Y) += part.xb[1];
+    if (DIM == 3) BAR(Z) += part.xb[2];
     BAR(H) += rb.h;
     BAR(RHO) += rb.rho;
     BAR(EPRED) += rb.e;
@@ -443,9 +531,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_a_force_pair(Fields F, AdjDev A, 
     BAR(AW) = rb.aw;
     BAR(PSIPRED) += rb.psi;
     BAR(CS) += rb.cs;
-    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_VISCOSITY * n + i] += pb.av;
-    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_THERMAL_CONDUCTIVITY * n + i] += pb.ac;
-    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_MAGNETIC_DISSIPATION * n + i] += pb.ar;
+    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_VISCOSITY * n + i] += part.pb.av;
+    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_THERMAL_CONDUCTIVITY * n + i] += part.pb.ac;
+    A.par[(int64_t)SPHMHD_TP_ARTIFICIAL_MAGNETIC_DISSIPATION * n + i] += part.pb.ar;
     // psich = psipred chf vw: every pair's psich of this particle, in either role
     A.par[(int64_t)SPHMHD_TP_CLEANING_SPEED_FACTOR * n + i] += own.psich * me.psi * S.vw;
 }
@@ -494,12 +582,13 @@ __global__ void k_a_prep(Fields F, AdjDev A, const Inst* __restrict__ inst, cons
 // ---------------------------------------------------------------------------
 
 // the density sums of particle t at h, with their derivative in h: the
-// primal pass's scan (sph_pairs.hip density_block) over a dual h
+// primal pass's scan (sph_pairs.hip density_block) over a dual h, reading
+// the density pack {x, y, z, m} in sorted order
 template <int K, int DIM>
-__device__ __forceinline__ void density_sums(const Fields& F, const Inst& s, const int32_t* __restrict__ perm,
+__device__ __forceinline__ void density_sums(const Inst& s, const double4* __restrict__ dpack,
                                              const uint32_t* __restrict__ skey, const int32_t* __restrict__ cstart,
-                                             const int32_t* __restrict__ cend, int64_t t, double xi, double yi,
-                                             double zi, double hi, Dual<1>& sw, Dual<1>& sg) {
+                                             const int32_t* __restrict__ cend, int64_t t, int lane,
+                                             const double4& me, double hi, Dual<1>& sw, Dual<1>& sg) {
     const double R = kernel_radius(K);
     Dual<1> h(hi);
     h.d[0] = 1.0;
@@ -511,110 +600,145 @@ __device__ __forceinline__ void density_sums(const Fields& F, const Inst& s, con
         const double q = ceil(Rh / s.ceff[a]);
         sr[a] = q < 1.0 ? 1 : (q < 1.0e6 ? (int)q : 1000000);
     }
+    const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+    const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
+    const int per = s.periodic;
     sw = Dual<1>(0.0);
     sg = Dual<1>(0.0);
     for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double, double, double) {
-        for (int j = a; j < b; ++j) {
-            const int32_t jj = perm[j];
-            const double dx = min_image(xi - F.f[SPHMHD_F_X][jj], s.L[0], s.halfL[0], s.periodic);
-            const double dy = min_image(yi - F.f[SPHMHD_F_Y][jj], s.L[1], s.halfL[1], s.periodic);
-            const double dz = DIM == 3 ? min_image(zi - F.f[SPHMHD_F_Z][jj], s.L[2], s.halfL[2], s.periodic) : 0.0;
+        for (int j = a + lane; j < b; j += kAdjLanes) {
+            const double4 o = dpack[j];
+            const double dx = min_image(me.x - o.x, L0, H0, per);
+            const double dy = min_image(me.y - o.y, L1, H1, per);
+            const double dz = DIM == 3 ? min_image(me.z - o.z, L2, H2, per) : 0.0;
             const double r2 = r2_exact(dx, dy, dz);
             if (r2 < Rh2) {
-                const double mj = F.f[SPHMHD_F_M][jj];
                 const Dual<1> q = sqrt(r2) * h1;
                 Dual<1> w, dw;
                 shape<K, DIM>(q, w, dw);
-                sw += mj * w;
-                sg += mj * ((double)DIM * w + q * dw);
+                sw += o.w * w;
+                sg += o.w * ((double)DIM * w + q * dw);
             }
         }
     }, sr);
+    sw.v = gsum(sw.v);
+    sw.d[0] = gsum(sw.d[0]);
+    sg.v = gsum(sg.v);
+    sg.d[0] = gsum(sg.d[0]);
 }
 
+// per particle (sorted position t): the passes of the forward iteration from
+// the h entering the evaluation (rows of A.pass by sorted position), then
+// reversed; A.pass rows: 0 h, 1 sw then its bar, 2 sg then its bar, 3 dsw/dh,
+// 4 dsg/dh
 template <int K, int DIM>
-__global__ void __launch_bounds__(kAdjBlock) k_a_density(Fields F, AdjDev A, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(kAdjBlock) k_a_density(Fields F, AdjDev A, const double4* __restrict__ dpack,
+                                                        const int32_t* __restrict__ perm,
                                                         const uint32_t* __restrict__ skey,
                                                         const int32_t* __restrict__ cstart,
                                                         const int32_t* __restrict__ cend,
                                                         const Inst* __restrict__ inst,
                                                         const int32_t* __restrict__ inst_of, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int32_t i = perm[t];
-    const int ki = inst_of[t];
-    const Inst& s = inst[ki];
-    if (A.ain[n + i] == 1.0) {  // converged entering the evaluation: h and omega pass through
-        A.npass[i] = 0;
-        return;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = gid / kAdjLanes;
+    const int lane = (int)(gid % kAdjLanes);
+    const bool valid = t < n;
+    const int32_t i = valid ? perm[t] : 0;
+    const int ki = valid ? inst_of[t] : -1;
+    // converged entering the evaluation: h and omega pass through
+    const bool active = valid && A.ain[n + i] != 1.0;
+    if (valid && !active && lane == 0) A.npass[t] = 0;
+    double hmax = 0.0;
+    if (active) {
+        const Inst& s = inst[ki];
+        const double ksig = kernel_sigma(K, DIM);
+        const double4 me = dpack[t];
+        const double mi = me.w, hi0 = FLD(H0);
+        const int maxp = min(s.max_iter, A.passmax);
+        const int64_t pn = (int64_t)A.passmax * n;  // rows of one pass quantity
+        double h = A.ain[i];
+        hmax = h;
+        int np = 0;
+        double hout = h, omega = 1.0;
+        bool fin = false;
+        for (int p = 0; p < maxp; ++p) {
+            Dual<1> sw, sg;
+            density_sums<K, DIM>(s, dpack, skey, cstart, cend, t, lane, me, h, sw, sg);
+            if (lane == 0) {
+                A.pass[(int64_t)p * n + t] = h;
+                A.pass[pn + (int64_t)p * n + t] = sw.v;
+                A.pass[2 * pn + (int64_t)p * n + t] = sg.v;
+                A.pass[3 * pn + (int64_t)p * n + t] = sw.d[0];
+                A.pass[4 * pn + (int64_t)p * n + t] = sg.d[0];
+            }
+            newton_post<DIM>(h, sw.v, sg.v, mi, hi0, s.hfact, ksig, s.density_iterate, hout, omega, fin);
+            np = p + 1;
+            if (fin) break;
+            h = hout;
+            hmax = fmax(hmax, h);
+        }
+        if (lane == 0) {
+            A.npass[t] = np;
+            if (!fin && np == A.passmax && s.max_iter > A.passmax) atomicOr(&A.ai[ki].status, 2);
+            // the recomputed iteration must end where the forward one did
+            if (!(fabs(hout - FLD(H)) <= 1.0e-10 * fabs(FLD(H)))) atomicOr(&A.ai[ki].status, 1);
+            // reverse, pass by pass: (h_next, omega) = post_loop(h, sw(h), sg(h))
+            double hbar = BAR(H), wbar = BAR(OMEGA);
+            for (int p = np - 1; p >= 0; --p) {
+                const double hp = A.pass[(int64_t)p * n + t];
+                Dual<3> hd(hp), swd(A.pass[pn + (int64_t)p * n + t]), sgd(A.pass[2 * pn + (int64_t)p * n + t]);
+                hd.d[0] = 1.0;
+                swd.d[1] = 1.0;
+                sgd.d[2] = 1.0;
+                Dual<3> ho, om;
+                bool f;
+                newton_post<DIM>(hd, swd, sgd, mi, hi0, s.hfact, ksig, s.density_iterate, ho, om, f);
+                const double bsw = hbar * ho.d[1] + wbar * om.d[1];
+                const double bsg = hbar * ho.d[2] + wbar * om.d[2];
+                const double bh = hbar * ho.d[0] + wbar * om.d[0] + bsw * A.pass[3 * pn + (int64_t)p * n + t] +
+                                  bsg * A.pass[4 * pn + (int64_t)p * n + t];
+                A.pass[pn + (int64_t)p * n + t] = bsw;
+                A.pass[2 * pn + (int64_t)p * n + t] = bsg;
+                hbar = bh;
+                wbar = 0.0;  // an earlier pass's omega is overwritten
+            }
+            BAR(H) = hbar;  // the bar of h entering the evaluation
+            BAR(OMEGA) = 0.0;
+        }
     }
-    const double ksig = kernel_sigma(K, DIM);
-    const double xi = FLD(X), yi = FLD(Y), zi = DIM == 3 ? FLD(Z) : 0.0;
-    const double mi = FLD(M), hi0 = FLD(H0);
-    const int maxp = min(s.max_iter, A.passmax);
-    const int64_t pn = (int64_t)A.passmax * n;  // rows of one pass quantity
-    double h = A.ain[i];
-    double hmax = h;
-    int np = 0;
-    double hout = h, omega = 1.0;
-    bool fin = false;
-    for (int p = 0; p < maxp; ++p) {
-        Dual<1> sw, sg;
-        density_sums<K, DIM>(F, s, perm, skey, cstart, cend, t, xi, yi, zi, h, sw, sg);
-        A.pass[(int64_t)p * n + i] = h;
-        A.pass[pn + (int64_t)p * n + i] = sw.v;
-        A.pass[2 * pn + (int64_t)p * n + i] = sg.v;
-        A.pass[3 * pn + (int64_t)p * n + i] = sw.d[0];
-        A.pass[4 * pn + (int64_t)p * n + i] = sg.d[0];
-        newton_post<DIM>(h, sw.v, sg.v, mi, hi0, s.hfact, ksig, s.density_iterate, hout, omega, fin);
-        np = p + 1;
-        if (fin) break;
-        h = hout;
-        hmax = fmax(hmax, h);
+    // the largest h of any pass, per instance: combined over the wave first
+    // (same-address atomics serialise), one atomic per wave and instance
+    unsigned long long hb = active ? dbits(hmax) : 0ull;
+    const int k0 = __shfl(ki, 0);
+    const bool uniform = __all(ki == k0 || ki < 0);
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(hb, off);
+            hb = o > hb ? o : hb;
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0 && k0 >= 0 && hb) atomicMax(&A.ai[k0].hmax_bits, hb);
+    } else if (active && lane == 0) {
+        atomicMax(&A.ai[ki].hmax_bits, hb);
     }
-    A.npass[i] = np;
-    if (!fin && np == A.passmax && s.max_iter > A.passmax) atomicOr(&A.ai[ki].status, 2);
-    // the recomputed iteration must end where the forward one did
-    if (!(fabs(hout - FLD(H)) <= 1.0e-10 * fabs(FLD(H)))) atomicOr(&A.ai[ki].status, 1);
-    atomicMax(&A.ai[ki].hmax_bits, dbits(hmax));
-    // reverse, pass by pass: (h_next, omega) = post_loop(h, sw(h), sg(h))
-    double hbar = BAR(H), wbar = BAR(OMEGA);
-    for (int p = np - 1; p >= 0; --p) {
-        const double hp = A.pass[(int64_t)p * n + i];
-        Dual<3> hd(hp), swd(A.pass[pn + (int64_t)p * n + i]), sgd(A.pass[2 * pn + (int64_t)p * n + i]);
-        hd.d[0] = 1.0;
-        swd.d[1] = 1.0;
-        sgd.d[2] = 1.0;
-        Dual<3> ho, om;
-        bool f;
-        newton_post<DIM>(hd, swd, sgd, mi, hi0, s.hfact, ksig, s.density_iterate, ho, om, f);
-        const double bsw = hbar * ho.d[1] + wbar * om.d[1];
-        const double bsg = hbar * ho.d[2] + wbar * om.d[2];
-        const double bh = hbar * ho.d[0] + wbar * om.d[0] + bsw * A.pass[3 * pn + (int64_t)p * n + i] +
-                          bsg * A.pass[4 * pn + (int64_t)p * n + i];
-        A.pass[pn + (int64_t)p * n + i] = bsw;
-        A.pass[2 * pn + (int64_t)p * n + i] = bsg;
-        hbar = bh;
-        wbar = 0.0;  // an earlier pass's omega is overwritten
-    }
-    BAR(H) = hbar;  // the bar of h entering the evaluation
-    BAR(OMEGA) = 0.0;
 }
 
 // The positions' share: every pass p of every particle a sums
 // m_b w(r_ab / h_a^p) and m_b (d w + q w') over r_ab < R h_a^p; with the
 // pass's bars (swbar, sgbar) their gradient in the separation x_a - x_b is
 // m_b (swbar w' + sgbar ((d + 1) w' + q w'')) / h (x_a - x_b) / r.  Particle
-// k gathers its own passes (destination) and every candidate's (source).
+// t gathers its own passes (destination) and every candidate's (source).
 template <int K, int DIM>
-__global__ void __launch_bounds__(kAdjBlock) k_a_density_pos(Fields F, AdjDev A, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(kAdjBlock) k_a_density_pos(Fields F, AdjDev A, const double4* __restrict__ dpack,
+                                                            const int32_t* __restrict__ perm,
                                                             const uint32_t* __restrict__ skey,
                                                             const int32_t* __restrict__ cstart,
                                                             const int32_t* __restrict__ cend,
                                                             const Inst* __restrict__ inst,
                                                             const int32_t* __restrict__ inst_of, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = gid / kAdjLanes;
+    const int lane = (int)(gid % kAdjLanes);
+    if (t >= n) return;  // the whole group
     const int32_t i = perm[t];
     const int ki = inst_of[t];
     const Inst& s = inst[ki];
@@ -627,44 +751,55 @@ __global__ void __launch_bounds__(kAdjBlock) k_a_density_pos(Fields F, AdjDev A,
         const double q = ceil(R * hm / s.ceff[a]);
         sr[a] = q < 1.0 ? 1 : (q < 1.0e6 ? (int)q : 1000000);
     }
-    const double xi = FLD(X), yi = FLD(Y), zi = DIM == 3 ? FLD(Z) : 0.0, mi = FLD(M);
-    const int npi = A.npass[i];
+    const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+    const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
+    const int per = s.periodic;
+    const double4 me = dpack[t];
+    const int npi = A.npass[t];
+    // the reach of this particle's passes
+    double Ri = 0.0;
+    for (int p = 0; p < npi; ++p) Ri = fmax(Ri, R * A.pass[(int64_t)p * n + t]);
+    const double Rm = R * hm;
+    const double Rm2 = Rm * Rm;
     double xb[3] = {0.0, 0.0, 0.0};
-    // d phi / d(x_a - x_b) of pass p of a against b at separation d (r > 0)
-    auto grad = [&](int64_t a, int p, double mb, const double* d, double r) {
+    // d phi / d(x_a - x_b) / (x_a - x_b) of pass p of a (sorted position) against b
+    auto coef = [&](int64_t a, int p, double mb, double r) {
         const double h = A.pass[(int64_t)p * n + a];
         const double bsw = A.pass[pn + (int64_t)p * n + a], bsg = A.pass[2 * pn + (int64_t)p * n + a];
         const double q = r / h;
         double dw, d2w;
         shape_d12<K, DIM>(q, dw, d2w);
-        const double c = mb * (bsw * dw + bsg * ((double)(DIM + 1) * dw + q * d2w)) / (h * r);
-        return c;
+        return mb * (bsw * dw + bsg * ((double)(DIM + 1) * dw + q * d2w)) / (h * r);
     };
     for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double, double, double) {
-        for (int j = a; j < b; ++j) {
-            const int32_t jj = perm[j];
-            double d[3];
-            d[0] = min_image(xi - F.f[SPHMHD_F_X][jj], s.L[0], s.halfL[0], s.periodic);
-            d[1] = min_image(yi - F.f[SPHMHD_F_Y][jj], s.L[1], s.halfL[1], s.periodic);
-            d[2] = DIM == 3 ? min_image(zi - F.f[SPHMHD_F_Z][jj], s.L[2], s.halfL[2], s.periodic) : 0.0;
-            const double r2 = r2_exact(d[0], d[1], d[2]);
-            if (!(r2 > 0.0)) continue;
+        for (int j = a + lane; j < b; j += kAdjLanes) {
+            const double4 o = dpack[j];
+            const double dx = min_image(me.x - o.x, L0, H0, per);
+            const double dy = min_image(me.y - o.y, L1, H1, per);
+            const double dz = DIM == 3 ? min_image(me.z - o.z, L2, H2, per) : 0.0;
+            const double r2 = r2_exact(dx, dy, dz);
+            if (!(r2 > 0.0) || !(r2 < Rm2)) continue;
             const double r = sqrt(r2);
             double c = 0.0;
-            for (int p = 0; p < npi; ++p) {
-                const double Rh = R * A.pass[(int64_t)p * n + i];
-                if (r2 < Rh * Rh) c += grad(i, p, F.f[SPHMHD_F_M][jj], d, r);
-            }
-            const int npj = A.npass[jj];
+            if (r < Ri)
+                for (int p = 0; p < npi; ++p) {
+                    const double Rh = R * A.pass[(int64_t)p * n + t];
+                    if (r2 < Rh * Rh) c += coef(t, p, o.w, r);
+                }
+            const int npj = A.npass[j];
             for (int p = 0; p < npj; ++p) {
-                const double Rh = R * A.pass[(int64_t)p * n + jj];
-                if (r2 < Rh * Rh) c += grad(jj, p, mi, d, r);  // (x_jj - x_i) = -d: the two signs cancel
+                const double Rh = R * A.pass[(int64_t)p * n + j];
+                if (r2 < Rh * Rh) c += coef(j, p, me.w, r);  // (x_j - x_t) = -d: the two signs cancel
             }
-            xb[0] += c * d[0];
-            xb[1] += c * d[1];
-            xb[2] += c * d[2];
+            xb[0] += c * dx;
+            xb[1] += c * dy;
+            xb[2] += c * dz;
         }
     }, sr);
+    xb[0] = gsum(xb[0]);
+    xb[1] = gsum(xb[1]);
+    xb[2] = gsum(xb[2]);
+    if (lane != 0) return;
     BAR(X) += xb[0];
     BAR(Y) += xb[1];
     if (DIM == 3) BAR(Z) += xb[2];
@@ -955,19 +1090,20 @@ void launch_a_stage2(Dev& d, AdjDev& a) {
 
 void launch_a_eval(Dev& d, AdjDev& a) {
     if (d.n == 0) return;
-    const unsigned g = ablk(d.n, kAdjBlock);
+    const unsigned g = ablk(d.n * kAdjLanes, kAdjBlock);
 #define CALL_A(KK, DD)                                                                                            \
     {                                                                                                             \
-        hipLaunchKernelGGL((k_a_force_epi<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.perm,   \
-                           d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                                   \
-        hipLaunchKernelGGL((k_a_force_pair<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.perm,  \
-                           d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                                   \
+        hipLaunchKernelGGL((k_a_force_epi<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.fpack,  \
+                           d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                           \
+        hipLaunchKernelGGL((k_a_force_pair<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.fpack, \
+                           d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                           \
         hipLaunchKernelGGL(k_a_prep, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.inst, d.inst_of,  \
                            d.n);                                                                                  \
-        hipLaunchKernelGGL((k_a_density<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.perm,     \
-                           d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n);                                     \
-        hipLaunchKernelGGL((k_a_density_pos<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.perm, \
-                           d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n);                                     \
+        launch_pack_density(d);                                                                                   \
+        hipLaunchKernelGGL((k_a_density<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a,             \
+                           (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n);    \
+        hipLaunchKernelGGL((k_a_density_pos<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a,         \
+                           (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n);    \
     }
     A_DISPATCH(d.kernel, d.dim, CALL_A)
 #undef CALL_A

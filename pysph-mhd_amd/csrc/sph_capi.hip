@@ -135,6 +135,23 @@ struct sphmhd_ctx {
         double* seg = nullptr;       // every x kNState x n: the states of one segment's steps
         double* clk = nullptr;       // (every + 1) x 2 x ninst: their clocks, then the clock after
         int seg_cap = 0;
+        double* ain_live = nullptr;  // the entry rows of evaluations run on the live context
+        // each step of a segment after its evaluation: what its reverse reads
+        // (property rows, the evaluation's entry, force pack, cells, lists,
+        // instances), so a step is reversed without computing it again
+        struct Snap {
+            double* rows = nullptr;   // kSnapRows x n, then the entry rows (5 x n)
+            double* fpack = nullptr;  // kFRecMax x n
+            int32_t* ints = nullptr;  // perm, keys, nbcnt (3 x n)
+            int32_t* nb = nullptr;
+            int64_t nb_cap = 0;
+            int32_t* cells = nullptr; // cstart, cend of the used cells
+            int64_t cells_cap = 0;
+            Inst* inst = nullptr;
+            int stride = 0;
+            int64_t ncells = 0;
+        };
+        std::vector<Snap> snaps;
     } adj;
     // launch-shape tuner: the first kTuneSteps steps of sphmhd_step rotate
     // the tuned launches through the configurations (workgroup size, which is
@@ -643,6 +660,8 @@ static int count_unconverged(sphmhd_ctx* c, int64_t* out) {
 static void adj_free(sphmhd_ctx* c) {
     sphmhd_ctx::Adj& A = c->adj;
     AdjDev& a = A.a;
+    a.ain = A.ain_live;
+    A.ain_live = nullptr;
     for (void* p : {(void*)a.acc, (void*)a.argmax, (void*)a.par, (void*)a.dtrow, (void*)a.ain, (void*)a.pass,
                     (void*)a.npass, (void*)a.bsave, (void*)a.ai, (void*)A.init, (void*)A.final_clk, (void*)A.seg,
                     (void*)A.clk})
@@ -650,6 +669,10 @@ static void adj_free(sphmhd_ctx* c) {
     for (double* p : A.ckpt)
         if (p) (void)hipFree(p);
     A.ckpt.clear();
+    for (auto& sn : A.snaps)
+        for (void* p : {(void*)sn.rows, (void*)sn.fpack, (void*)sn.ints, (void*)sn.nb, (void*)sn.cells, (void*)sn.inst})
+            if (p) (void)hipFree(p);
+    A.snaps.clear();
     double* keep[SPHMHD_NFIELDS];
     std::memcpy(keep, a.b, sizeof keep);
     a = AdjDev{};
@@ -673,6 +696,78 @@ static RowList rows_all() {
     return L;
 }
 
+// the property rows the reverse kernels of a step read after its evaluation
+static const int kSnapFields[] = {
+    SPHMHD_F_X, SPHMHD_F_Y, SPHMHD_F_Z, SPHMHD_F_H, SPHMHD_F_RHO, SPHMHD_F_M, SPHMHD_F_BX, SPHMHD_F_BY,
+    SPHMHD_F_BZ, SPHMHD_F_U, SPHMHD_F_V, SPHMHD_F_W, SPHMHD_F_EPRED, SPHMHD_F_AU, SPHMHD_F_AV, SPHMHD_F_AW,
+    SPHMHD_F_ABEVOLX, SPHMHD_F_ABEVOLY, SPHMHD_F_ABEVOLZ, SPHMHD_F_UPRED, SPHMHD_F_VPRED, SPHMHD_F_WPRED,
+    SPHMHD_F_PSIPRED, SPHMHD_F_OMEGA, SPHMHD_F_H0, SPHMHD_F_CS, SPHMHD_F_BEVOLX, SPHMHD_F_BEVOLY,
+    SPHMHD_F_BEVOLZ, SPHMHD_F_BEVOLXPRED, SPHMHD_F_BEVOLYPRED, SPHMHD_F_BEVOLZPRED, SPHMHD_F_APSI,
+    SPHMHD_F_ALPHA10, SPHMHD_F_ALPHA1, SPHMHD_F_AE, SPHMHD_F_DT_ADAPT};
+constexpr int kSnapRows = (int)(sizeof(kSnapFields) / sizeof(int));
+
+static RowList rows_snap() {
+    RowList L;
+    L.count = kSnapRows;
+    for (int r = 0; r < kSnapRows; ++r) L.f[r] = kSnapFields[r];
+    return L;
+}
+
+template <class T>
+static int grow(T** p, int64_t* cap, int64_t need) {
+    if (*p && *cap >= need) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    int rc = dalloc(p, need);
+    if (rc) return rc;
+    *cap = need;
+    return 0;
+}
+
+// the step just evaluated, kept for its reverse (slot j of the segment);
+// the entry rows were written there before the evaluation
+static int adj_snap_save(sphmhd_ctx* c, sphmhd_ctx::Adj::Snap& sn) {
+    Dev& d = c->d;
+    const int64_t n = d.n;
+    int rc;
+    launch_a_rows(d, rows_snap(), sn.rows, false);
+    HIPCHK(hipMemcpyAsync(sn.fpack, d.fpack, sizeof(double) * kFRecMax * n, hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipMemcpyAsync(sn.ints, d.perm, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipMemcpyAsync(sn.ints + n, d.keys, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipMemcpyAsync(sn.ints + 2 * n, d.nbcnt, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, d.stream));
+    const int64_t nbn = (int64_t)d.nb_stride * n;
+    if ((rc = grow(&sn.nb, &sn.nb_cap, nbn))) return rc;
+    HIPCHK(hipMemcpyAsync(sn.nb, d.nb, sizeof(int32_t) * nbn, hipMemcpyDeviceToDevice, d.stream));
+    sn.stride = d.nb_stride;
+    const int64_t tc = std::max<int64_t>(c->hcur->total_cells, 1);
+    if ((rc = grow(&sn.cells, &sn.cells_cap, 2 * tc))) return rc;
+    HIPCHK(hipMemcpyAsync(sn.cells, d.cstart, sizeof(int32_t) * tc, hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipMemcpyAsync(sn.cells + tc, d.cend, sizeof(int32_t) * tc, hipMemcpyDeviceToDevice, d.stream));
+    sn.ncells = tc;
+    HIPCHK(hipMemcpyAsync(sn.inst, d.inst, sizeof(Inst) * d.ninst, hipMemcpyDeviceToDevice, d.stream));
+    return 0;
+}
+
+// the context as it was after that step's evaluation, for the reverse
+// kernels (the live rows elsewhere are never read by them)
+static Dev adj_snap_view(sphmhd_ctx* c, const sphmhd_ctx::Adj::Snap& sn) {
+    Dev v = c->d;
+    const int64_t n = v.n;
+    for (int r = 0; r < kSnapRows; ++r)
+        if (v.fields.f[kSnapFields[r]]) v.fields.f[kSnapFields[r]] = sn.rows + (int64_t)r * n;
+    v.fpack = sn.fpack;
+    v.perm = sn.ints;
+    v.keys = (uint32_t*)(sn.ints + n);
+    v.nbcnt = sn.ints + 2 * n;
+    v.nb = sn.nb;
+    v.nb_stride = sn.stride;
+    v.cstart = sn.cells;
+    v.cend = sn.cells + sn.ncells;
+    v.inst = sn.inst;
+    return v;
+}
+
 static int adj_check(sphmhd_ctx* c) {
     int rc = check_ready(c);
     if (rc) return rc;
@@ -693,6 +788,7 @@ static int adj_alloc_work(sphmhd_ctx* c) {
     if ((rc = dalloc(&a.par, (int64_t)SPHMHD_NTPARAMS * n))) return rc;
     if ((rc = dalloc(&a.dtrow, n))) return rc;
     if ((rc = dalloc(&a.ain, 5 * n))) return rc;
+    c->adj.ain_live = a.ain;
     if ((rc = dalloc(&a.pass, (int64_t)5 * a.passmax * n))) return rc;
     if ((rc = dalloc(&a.npass, n))) return rc;
     if ((rc = dalloc(&a.bsave, 3 * n))) return rc;
@@ -1623,6 +1719,14 @@ int sphmhd_adjoint_record(sphmhd_ctx* c, int32_t warm, int64_t nsteps, int32_t e
         if ((rc = dalloc(&A.clk, (int64_t)(every + 1) * 2 * d.ninst))) return rc;
         A.seg_cap = every;
     }
+    while ((int)A.snaps.size() < every) {
+        sphmhd_ctx::Adj::Snap sn;
+        if ((rc = dalloc(&sn.rows, (int64_t)(kSnapRows + 5) * n))) return rc;
+        if ((rc = dalloc(&sn.fpack, (int64_t)kFRecMax * n))) return rc;
+        if ((rc = dalloc(&sn.ints, 3 * n))) return rc;
+        if ((rc = dalloc(&sn.inst, d.ninst))) return rc;
+        A.snaps.push_back(sn);
+    }
     launch_a_rows(d, rows_all(), A.init, false);
     launch_a_clock(d, A.init + (int64_t)SPHMHD_NFIELDS * n, false);
     int worst = 0;
@@ -1692,32 +1796,40 @@ int sphmhd_adjoint_reverse(sphmhd_ctx* c, double* param_bars) {
     const int64_t sn = (int64_t)kNState * n;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t len = std::min<int64_t>(A.every, A.steps - s * A.every);
-        // the segment again from its checkpoint, keeping each step's state and clock
+        // the segment again from its checkpoint, keeping each step's state,
+        // clock and evaluation
         launch_a_rows(d, rows_state(), A.ckpt[s], true);
         launch_a_clock(d, A.ckpt[s] + sn, true);
         for (int64_t j = 0; j < len; ++j) {
+            sphmhd_ctx::Adj::Snap& sp = A.snaps[j];
             launch_a_rows(d, rows_state(), A.seg + j * sn, false);
             launch_a_clock(d, A.clk + 2 * ninst * j, false);
-            if ((rc = adj_plain_step(c))) return rc;
-        }
-        launch_a_clock(d, A.clk + 2 * ninst * len, false);
-        // its steps backwards: initialize, stage1 and the evaluation again, then reversed
-        for (int64_t j = len - 1; j >= 0; --j) {
-            launch_a_rows(d, rows_state(), A.seg + j * sn, true);
-            launch_a_clock(d, A.clk + 2 * ninst * j, true);
             launch_initialize(d);
             launch_stage1(d);
+            a.ain = sp.rows + (int64_t)kSnapRows * n;
             launch_a_save_entry(d, a);
             if ((rc = do_eval(c)) < 0) return rc;
+            if ((rc = adj_snap_save(c, sp))) return rc;
+            launch_stage2(d);
+            launch_clock(d);
+            HIPCHK(hipGetLastError());
+        }
+        launch_a_clock(d, A.clk + 2 * ninst * len, false);
+        // its steps backwards, each from its kept evaluation
+        for (int64_t j = len - 1; j >= 0; --j) {
+            const sphmhd_ctx::Adj::Snap& sp = A.snaps[j];
+            Dev v = adj_snap_view(c, sp);
+            a.ain = sp.rows + (int64_t)kSnapRows * n;
             a.snap = A.seg + j * sn;
-            launch_a_eval_begin(d, a, A.clk + 2 * ninst * (j + 1), 0);
-            launch_a_stage2(d, a);
-            launch_a_eval(d, a);
-            launch_a_stage1_init(d, a);
-            launch_a_reduce_dt(d, a);
+            launch_a_eval_begin(v, a, A.clk + 2 * ninst * (j + 1), 0);
+            launch_a_stage2(v, a);
+            launch_a_eval(v, a);
+            launch_a_stage1_init(v, a);
+            launch_a_reduce_dt(v, a);
             HIPCHK(hipGetLastError());
         }
     }
+    a.ain = A.ain_live;
     if (A.warm) {
         const double* clk_next = nseg ? A.ckpt[0] + sn : A.final_clk;
         if ((rc = adj_warm_reverse(c, clk_next))) return rc;

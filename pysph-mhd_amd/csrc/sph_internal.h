@@ -1297,6 +1297,7 @@ void launch_t_totals(Dev& d, TanDev& g, double* out);
 
 // adjoint (reverse-mode) step (sph_adjoint.hip)
 constexpr int kAdjBlock = 128;    // workgroup of the reverse pair and density kernels
+constexpr int kAdjLanes = 8;      // lanes sharing one particle in those kernels
 constexpr int kAdjAcc = 35;       // sums of the force pass (their bars: AdjDev::acc rows)
 constexpr int kAdjPassMax = 32;   // density passes one evaluation's reverse sweep holds per particle
 
@@ -1339,13 +1340,14 @@ struct AdjInst {
 
 struct AdjDev {
     double* b[SPHMHD_NFIELDS];  // bar rows (the caller's), null where the field is null
-    double* acc;        // 35 x n: bars of the force pass's sums (storage order)
-    int32_t* argmax;    // n: the neighbour attaining dt_cfl's maximum (-1: none)
+    double* acc;        // 35 x n: bars of the force pass's sums (by sorted position)
+    int32_t* argmax;    // n: sorted position of the neighbour attaining dt_cfl's maximum (-1: none)
     double* par;        // SPHMHD_NTPARAMS x n: per-particle parameter bars
     double* dtrow;      // n: per-particle bars of the step's dt
     double* ain;        // 5 x n: h, converged, au, av, aw entering the evaluation
-    double* pass;       // 5 x passmax x n: per density pass h, sw, sg (values, then bars), dsw/dh, dsg/dh
-    int32_t* npass;     // n: density passes the particle ran
+    double* pass;       // 5 x passmax x n: per density pass h, sw, sg (values, then bars), dsw/dh,
+                        // dsg/dh (by sorted position)
+    int32_t* npass;     // n: density passes the particle ran (by sorted position)
     double* bsave;      // 3 x n: warm start: the bar of the saved initial B
     AdjInst* ai;        // ninst
     const double* snap; // kNState x n: the state the step being reversed started from
