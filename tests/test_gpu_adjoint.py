@@ -16,7 +16,10 @@
     dE/d amplitude, dE/d alpha_max and dE/d cleaning damping against central
     differences of the plain step at eps = 1e-6 and 1e-5 relative, to 1e-5;
   * torch.autograd through RolloutEnergy: the gradient reaches the amplitude
-    the initial fields were built from in torch.
+    the initial fields were built from in torch;
+  * 3-D (a jittered 6^3 blast: the Cullen-Dehnen velocity-gradient inversion,
+    z components) and frozen particles (itype != 1) against the taped
+    oracle; an uneven two-instance ensemble equals each instance alone.
 
 Fields compared by relative L2 over the particles; a field whose reference
 gradient is zero must be zero to 1e-12 of the largest gradient's scale.
@@ -172,3 +175,98 @@ def test_rollout_energy_autograd_reaches_the_amplitude(gpu):
     g_amp = sum(float(np.dot(grads[k], tan[k])) for k in tan)
     assert float(amp.grad) == pytest.approx(2.0 * g_amp, rel=1e-10)
     assert np.allclose(theta.grad.numpy(), 2.0 * np.array([gpar[n] for n in TPARAMS]), rtol=1e-12, atol=0)
+
+
+def _compare_fields(grads, ref, label, tol):
+    scale = max(float(np.abs(ref[k]).max()) for k in INPUTS)
+    worst = 0.0
+    for k in INPUTS:
+        a, b = grads[k], ref[k]
+        nb = float(np.linalg.norm(b))
+        if nb <= 1e-12 * scale:
+            assert float(np.abs(a).max()) <= 1e-12 * scale, (label, k, float(np.abs(a).max()))
+            continue
+        e = float(np.linalg.norm(a - b)) / nb
+        worst = max(worst, e)
+        assert e <= tol, (label, k, e)
+    return worst
+
+
+def test_adjoint_3d_matches_taped_oracle(gpu):
+    """3-D: a jittered 6^3 blast (quintic), warm start and one step: the
+    velocity-gradient inversion of the Cullen-Dehnen switch and the z
+    components."""
+    from pyoracle import tape_rollout
+    from sphmhd.adjoint import energy_gradient
+    from sphmhd.cases.baseline import blast_3d, jittered
+
+    s = jittered(blast_3d(6), 0.1, seed=4)
+    _print("adjoint vs taped oracle: blast 6^3 jittered, warm start + 1 step")
+    value, grads, gpar = energy_gradient(s, 1, every=1)
+    tvals, tgrads, tgpar, nodes = tape_rollout(s.params, s.offsets, s.fields, nsteps=1)
+    assert value == pytest.approx(tvals[0], rel=1e-12)
+    inputs = INPUTS + ("z",)
+    worst = _compare_fields(grads, tgrads, "3-D", 1e-8)
+    e = float(np.linalg.norm(grads["z"] - tgrads["z"]) / np.linalg.norm(tgrads["z"]))
+    assert e <= 1e-8, e
+    for p, name in enumerate(TPARAMS):
+        ref = tgpar[0, p]
+        assert abs(gpar[name] - ref) <= 1e-8 * abs(ref) + 1e-14 * abs(value), (name, gpar[name], ref)
+    _print(f"  3-D worst field rel-L2 {max(worst, e):.2e} over {inputs}")
+
+
+def test_adjoint_frozen_particles_match_taped_oracle(gpu):
+    """Particles with itype != 1 keep their state through the stages
+    (integrator.py:38, 67): their bars pass through them."""
+    from pyoracle import tape_rollout
+    from sphmhd.adjoint import energy_gradient
+
+    s = _small()
+    f = dict(s.fields)
+    it = np.ones(f["x"].size, dtype=np.int32)
+    it[::7] = 3
+    f["itype"] = it
+    s = type(s)(s.name, s.params, s.offsets, f, info=s.info)
+    value, grads, gpar = energy_gradient(s, 3, every=2)
+    tvals, tgrads, tgpar, _ = tape_rollout(s.params, s.offsets, s.fields, nsteps=3)
+    assert value == pytest.approx(tvals[0], rel=1e-12)
+    worst = _compare_fields(grads, tgrads, "frozen", 1e-8)
+    for p, name in enumerate(TPARAMS):
+        ref = tgpar[0, p]
+        assert abs(gpar[name] - ref) <= 1e-8 * abs(ref) + 1e-14 * abs(value), (name, gpar[name], ref)
+    _print(f"  frozen particles: worst field rel-L2 {worst:.2e}")
+
+
+def test_adjoint_ensemble_instances_equal_alone(gpu):
+    """Two uneven Orszag-Tang instances in one context: each instance's
+    gradient (fields and parameters) equals the instance run alone."""
+    import torch
+
+    from sphmhd.adjoint import AdjointSystem
+    from sphmhd.cases.baseline import Setup, ensemble_ot, jittered
+
+    ens = jittered(ensemble_ot(2, seed=3, sizes=[14, 18]), 0.1, seed=9)
+
+    def grad_of(setup):
+        sysm = AdjointSystem(setup.params, setup.offsets)
+        sysm.set_fields(dict(setup.fields))
+        sysm.record(3, 2)
+        sysm.zero_bars()
+        sysm.seed_totals({"ekin": 1.0, "emag": 1.0})
+        pb = sysm.reverse()
+        torch.cuda.synchronize()
+        g = {k: sysm.gradient(k) for k in INPUTS}
+        sysm.close()
+        return g, pb
+
+    g_all, pb_all = grad_of(ens)
+    for k in range(2):
+        a, b = int(ens.offsets[k]), int(ens.offsets[k + 1])
+        alone = Setup(f"inst{k}", [ens.params[k]], np.array([0, b - a], dtype=np.int64),
+                      {name: np.asarray(v)[a:b].copy() for name, v in ens.fields.items()})
+        g1, pb1 = grad_of(alone)
+        for name in INPUTS:
+            ref = g1[name]
+            got = g_all[name][a:b]
+            assert np.allclose(got, ref, rtol=1e-11, atol=1e-13 * float(np.abs(ref).max() + 1e-300)), (k, name)
+        assert np.allclose(pb_all[k], pb1[0], rtol=1e-11, atol=1e-18), (k, pb_all[k], pb1[0])
