@@ -73,6 +73,12 @@ def _pair(s):
 
 @pytest.mark.parametrize("name", sorted(CASE_NX))
 def test_reference_case_matches_oracle(gpu, name):
+    """Floor: the cleaning fields against 1e-3 of their natural scale (|B|/h,
+    rho|B|/h, |B|, v_w|B|/h, v_w|B|/(rho h)), and the fields an exact
+    lattice holds at round-off against 1e-3 of a physical scale: v_w for
+    velocities and dh/dt, v_w^2/h for accelerations, v_w/h for div v,
+    v_w^3/h for de/dt, |B| for field components, alpha_max for the switch;
+    every other field against its own norm."""
     s = case_setup(name, CASE_NX[name])
     p = s.params[0]
     live = (s.itype != 3) if s.itype is not None else None
@@ -122,7 +128,14 @@ class _Wrap:
 def test_reference_application_runs(gpu, tmp_path, name):
     """The application of a reference case (TOML, warm start, initial dt,
     solver loop, dumps) on the HIP path ends where the oracle run through
-    the same sequence ends."""
+    the same sequence ends.
+
+    Floor: the cleaning fields against 1e-3 of their natural scale (|B|/h,
+    rho|B|/h, |B|, v_w|B|/h, v_w|B|/(rho h)), and the fields an exact
+    lattice holds at round-off against 1e-3 of a physical scale: v_w for
+    velocities and dh/dt, v_w^2/h for accelerations, v_w/h for div v,
+    v_w^3/h for de/dt, |B| for field components, alpha_max for the switch;
+    every other field against its own norm."""
     from pyoracle import OracleSystem
 
     app = case_app(name, CASE_NX[name], directory=str(tmp_path / "out"))
@@ -162,7 +175,14 @@ def test_reference_case_100_steps(gpu, name, nx):
     """A hundred steps of a reference case against the oracle: every field to
     1e-10 (physical floors as above) at steps 50 and 100, and the conserved
     totals to 1e-8 (momentum against sqrt(2 M E_kin), the total being zero
-    up to round-off)."""
+    up to round-off).
+
+    Floor: the cleaning fields against 1e-3 of their natural scale (|B|/h,
+    rho|B|/h, |B|, v_w|B|/h, v_w|B|/(rho h)), and the fields an exact
+    lattice holds at round-off against 1e-3 of a physical scale: v_w for
+    velocities and dh/dt, v_w^2/h for accelerations, v_w/h for div v,
+    v_w^3/h for de/dt, |B| for field components, alpha_max for the switch;
+    every other field against its own norm."""
     s = case_setup(name, nx)
     p = s.params[0]
     live = (s.itype != 3) if s.itype is not None else None

@@ -57,7 +57,12 @@ def test_one_particle_does_not_converge(gpu):
 def test_ragged_ensemble_with_empty_instance(gpu):
     """Instances of 576, 0 and 1024 particles (in that order) in one context:
     each non-empty one matches the oracle run of the same ensemble, and the
-    empty one keeps the clock the oracle gives it."""
+    empty one keeps the clock the oracle gives it.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from pyoracle import OracleSystem
     from sphmhd.device import DeviceSystem
 
@@ -85,7 +90,12 @@ def test_ragged_ensemble_with_empty_instance(gpu):
 def test_particles_on_box_faces(gpu):
     """Particles exactly at lo, at hi (outside the half-open box, binned into
     the last cell and wrapped by the first stage) and one ulp below hi: cells
-    and neighbour sets bit-exact, then three steps to 1e-10."""
+    and neighbour sets bit-exact, then three steps to 1e-10.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from pyoracle import OracleSystem
     from sphmhd.device import DeviceSystem
 

@@ -41,6 +41,10 @@ class _Wrap:
 
 
 def test_application_matches_oracle(gpu, tmp_path):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from pyoracle import OracleSystem
     from sphmhd.cases.apps import OrszagTang2D
     from sphmhd.cases.baseline import orszag_tang_2d

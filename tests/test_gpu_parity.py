@@ -107,6 +107,10 @@ def test_neighbours_bit_exact(gpu, which, prune, monkeypatch):
 
 
 def test_single_evaluation(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     s = _ot(64)
     dev, orc = _pair(s)
     for sysm in (dev, orc):
@@ -118,6 +122,10 @@ def test_single_evaluation(gpu):
 
 
 def test_warm_start(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     s = _ot(64)
     dev, orc = _pair(s)
     dev.warm_start()
@@ -134,6 +142,10 @@ def test_warm_start(gpu):
 
 
 def test_steps_lockstep(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     s = _ot(64)
     dev, orc = _pair(s)
     dev.warm_start()
@@ -235,6 +247,10 @@ def test_steps_enqueued_ahead_survive_redos(gpu, monkeypatch, spec):
 @pytest.mark.parametrize("kernel", ["cubic", "wendland_c2", "wendland_c4", "wendland_c6", "gaussian",
                                     "super_gaussian"])
 def test_kernels_steps(gpu, kernel):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     hf = 1.2 if kernel in ("cubic", "quintic", "gaussian", "super_gaussian") else 1.6
     s = _ot(32, kernel=kernel, hfact=hf)
     dev, orc = _pair(s)
@@ -246,6 +262,10 @@ def test_kernels_steps(gpu, kernel):
 
 
 def test_blast_3d_steps(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.cases.baseline import blast_3d, jittered
 
     s = jittered(blast_3d(16), 0.05, seed=2)
@@ -260,6 +280,10 @@ def test_blast_3d_steps(gpu):
 
 
 def test_rotor_variable_mass(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.cases.baseline import jittered, rotor_2d
 
     s = jittered(rotor_2d(48), 0.1, seed=4)
@@ -273,7 +297,12 @@ def test_rotor_variable_mass(gpu):
 
 
 def test_frozen_particles(gpu):
-    """itype != 1 particles are not integrated (integrator.py:39,69)."""
+    """itype != 1 particles are not integrated (integrator.py:39,69).
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     s = _ot(32)
     itype = np.ones(s.n, dtype=np.int32)
     itype[::7] = 3
@@ -289,6 +318,10 @@ def test_frozen_particles(gpu):
 
 
 def test_ensemble_per_instance(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.cases.baseline import ensemble_ot
 
     from sphmhd.cases.baseline import jittered
@@ -309,7 +342,12 @@ def test_ensemble_per_instance(gpu):
 
 def test_orszag_tang_128_100_steps(gpu):
     """The CPU-runnable workload: 128^2 Orszag-Tang, 100 PEC steps, compared
-    every 10 steps; conserved totals at step 100 to 1e-8."""
+    every 10 steps; conserved totals at step 100 to 1e-8.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.cases.baseline import orszag_tang_2d
 
     s = orszag_tang_2d(128)
@@ -336,7 +374,12 @@ def test_orszag_tang_128_100_steps(gpu):
 
 def test_neighbour_list_growth(gpu):
     """More neighbours than the initial list slots: the evaluation grows the
-    lists and re-runs only the list build and the force pass."""
+    lists and re-runs only the list build and the force pass.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     s = _ot(32, hfact=1.9)
     dev, orc = _pair(s)
     dev.warm_start()
@@ -353,7 +396,12 @@ def test_run_stops_at_tf(gpu, monkeypatch, graphs):
     """A run bounded by tf (inside the fourth step): sphmhd_step(100) takes
     four steps, the last cut to tf - t, and stops with the clock on tf; the
     step enqueued ahead of the host's wait (graphs on) exits at entry.  The
-    state equals the oracle's, which stops after the same step."""
+    state equals the oracle's, which stops after the same step.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.cases.baseline import jittered, orszag_tang_2d
     from sphmhd.device import DeviceSystem
     from pyoracle import OracleSystem

@@ -33,6 +33,10 @@ def _alfven(amplitude=0.1, alpha_max=1.0, cleaning_damping_factor=1.0):
 
 
 def test_tangent_values_equal_plain_step(gpu):
+    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
     from sphmhd.device import DeviceSystem
     from sphmhd.tangent import TangentSystem, _ic_tangent
 
@@ -92,7 +96,14 @@ def test_autograd_energy_rollout(gpu):
 def test_tangent_values_equal_plain_step_split_grid(gpu):
     """The reference rotor (3-D, its dense disk splits the cells): the
     tangent step's values, whose density pass scans per-particle stencils
-    as the plain pass does, are the plain step's to the parity standard."""
+    as the plain pass does, are the plain step's to the parity standard.
+
+    Floor: the cleaning fields against 1e-3 of their natural scale (|B|/h,
+    rho|B|/h, |B|, v_w|B|/h, v_w|B|/(rho h)), and the fields an exact
+    lattice holds at round-off against 1e-3 of a physical scale: v_w for
+    velocities and dh/dt, v_w^2/h for accelerations, v_w/h for div v,
+    v_w^3/h for de/dt, |B| for field components, alpha_max for the switch;
+    every other field against its own norm."""
     from helpers import CASE_NX, case_setup, init_fields, physical_scales
     from sphmhd.device import DeviceSystem
     from sphmhd.tangent import TangentSystem
