@@ -88,6 +88,7 @@ struct Inst {
     // density-iteration state (the group control's; read by the passes)
     unsigned long long hmax_bits;       // max h for the current pass
     int32_t iters, done;
+    int32_t spread;  // lanes per particle of this instance in the next pass (set with its count)
     // reduction targets (bit patterns of positive doubles order like integers)
     // force pass: min dt_adapt and the warning bits (1: density hit
     // max_iterations, 2: NaN)
@@ -124,6 +125,9 @@ struct Ctl {
     // 1 while a multi-step run is before its last step: the force pass then
     // skips the diagnostic rows no later kernel reads (set by the host)
     int32_t lean;
+    // the next density pass maps spread lanes to each particle (the largest
+    // Inst::spread of the instances still iterating)
+    int32_t spread;
     // list build
     alignas(128) int64_t pairs;
     int32_t max_nb;
@@ -198,6 +202,26 @@ __device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
 #pragma clang fp contract(off)
     double s = dx * dx + dy * dy;
     return s + dz * dz;
+}
+
+// sqrt of a squared distance: the Goldschmidt step and the two Newton
+// corrections of the library's correctly rounded sqrt, without its rescaling
+// of inputs below 2^-767 and its class test (zero, infinity, NaN).  Equal to
+// sqrt(r2) bit for bit for every r2 >= 2^-600; below, the result is sqrt of
+// 2^-600 (2^-300), which a distance of zero (a particle with itself) turns
+// into a q whose square vanishes beside 1 and whose product with a kernel
+// derivative vanishes beside W(0), so the density sums keep their bits.
+__device__ __forceinline__ double sqrt_r2(double r2) {
+    const double x = ::fmax(r2, 0x1p-600);
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;
+    const double e = ::fma(-h, s, 0.5);
+    s = ::fma(s, e, s);
+    h = ::fma(h, e, h);
+    double d = ::fma(-s, s, x);
+    s = ::fma(d, h, s);
+    d = ::fma(-s, s, x);
+    return ::fma(d, h, s);
 }
 
 __device__ __forceinline__ int cell_coord(double x, double lo, double ceff, int n) {
@@ -928,6 +952,21 @@ __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, in
         f(j + 3, o3);
     }
     for (; j < b; ++j) f(j, rec[j]);
+}
+
+// f(j, rec) for j = a + g, a + g + G, ... < b: lane g of a group of G lanes
+// sharing one destination, four loads in flight per lane
+template <class T, class F>
+__device__ __forceinline__ void scan_range4_strided(const T* __restrict__ rec, int a, int b, int g, int G, F&& f) {
+    int j = a + g;
+    for (; j + 3 * G < b; j += 4 * G) {
+        const T o0 = rec[j], o1 = rec[j + G], o2 = rec[j + 2 * G], o3 = rec[j + 3 * G];
+        f(j, o0);
+        f(j + G, o1);
+        f(j + 2 * G, o2);
+        f(j + 3 * G, o3);
+    }
+    for (; j < b; j += G) f(j, rec[j]);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,32 +37,62 @@ __device__ __forceinline__ double iroot_dim(double x, int dim) { return dim == 2
 // the evaluation (its own h is frozen and the sum only involves its h, x and
 // the neighbours' m, x), so skipping it is exact.  Particles that stay
 // unconverged are appended to act_out.
-template <int K, int DIM>
+//
+// A later pass over few particles runs G lanes per particle (SPREAD): lane g
+// takes every G-th candidate of each stencil range, an xor butterfly
+// combines the G partial sums (every lane then holds the same bits) and the
+// group's first lane finishes the particle.  Such a pass is bound by the
+// latency of one particle's scan, which G lanes cut G-fold.  G is a power of
+// two up to 64 chosen per instance by the group control from that
+// instance's own count and size (Inst::spread), so an instance's bits do not
+// depend on the launch shape or on the other instances of an ensemble; the
+// launch maps Ctl::spread lanes (the largest G) to each particle and a
+// particle uses the first G of them.
+constexpr int kSpreadMax = 64;            // lanes per particle at most (a wave)
+constexpr int64_t kSpreadSlots = 262144;  // lanes an instance's spread pass fills at most (four waves per SIMD)
+
+__device__ __forceinline__ int spread_lanes(int64_t count, int64_t n) {
+    const int64_t cap = n < kSpreadSlots ? n : kSpreadSlots;
+    int G = 1;
+    while (G < kSpreadMax && count * (2 * G) <= cap) G *= 2;
+    return G;
+}
+
+template <int K, int DIM, bool SPREAD>
 __device__ __forceinline__ void density_block(Fields F, const double4* __restrict__ dpack,
                                               const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                               const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                               Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                               const int32_t* __restrict__ act_in, const int32_t* cnt_in,
-                                              int32_t* __restrict__ act_out, int32_t* cnt_out, int64_t tile) {
+                                              int32_t* __restrict__ act_out, int32_t* cnt_out, int64_t tile,
+                                              int G) {
     const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
     const int64_t idx0 = tile * B;
-    const int64_t items = act_in ? (int64_t)*cnt_in : n;
+    // Gm lanes mapped to each item (B is a multiple of Gm, so they are
+    // consecutive lanes of one wave)
+    const int Gm = SPREAD ? G : 1;
+    const int64_t items = (act_in ? (int64_t)*cnt_in : n) * Gm;
     if (idx0 >= items) return;  // whole block past the end (the block barriers below need every wave)
     const int64_t idx = idx0 + threadIdx.x;
     const bool valid = idx < items;
-    int64_t t = valid ? (act_in ? (int64_t)act_in[idx] : idx) : 0;
+    const int g = SPREAD ? (int)(threadIdx.x & (Gm - 1)) : 0;
+    const int64_t item = SPREAD ? idx / Gm : idx;
+    int64_t t = valid ? (act_in ? (int64_t)act_in[item] : item) : 0;
     int ki = valid ? inst_of[t] : 0;
     const Inst& s = inst[ki];
+    const int Gk = SPREAD ? min(max(s.spread, 1), Gm) : 1;  // this particle's lanes: the first Gk of its Gm
     bool live = valid && !s.done;
     int32_t i = valid ? perm[t] : 0;
     double hi = live ? F.f[SPHMHD_F_H][i] : 0.0;
-    bool active = live && F.f[SPHMHD_F_CONVERGED][i] != 1.0;
+    bool active = live && F.f[SPHMHD_F_CONVERGED][i] != 1.0 && g < Gk;
     double hout = hi;
     bool unconv = false;
 
+    double sw = 0.0, sg = 0.0;
+    double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
     if (active) {
         const double R = kernel_radius(K);
-        const double4 me = dpack[t];
+        me = dpack[t];
         const double h1 = 1.0 / hi;
         const double Rh = R * hi;
         const double Rh2 = Rh * Rh;
@@ -78,31 +108,48 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
             const double q = ceil(Rh / s.ceff[a]);
             sr[a] = q < 1.0 ? 1 : (q < (double)s.srad[a] ? (int)q : s.srad[a]);
         }
-        double sw = 0.0, sg = 0.0;
         // one loop with the ranges' image shifts, one with min_image per
         // candidate (small periodic grids); both give the same r^2 bits
         auto scan = [&](auto shift_c) {
             constexpr bool kShift = decltype(shift_c)::value;
+            auto body = [&](int j, const double4& o, double sx, double sy, double sz) {
+                const double dx = kShift ? (me.x - o.x) + sx : min_image(me.x - o.x, L0, H0, per);
+                const double dy = kShift ? (me.y - o.y) + sy : min_image(me.y - o.y, L1, H1, per);
+                const double dz = DIM == 3 ? (kShift ? (me.z - o.z) + sz : min_image(me.z - o.z, L2, H2, per)) : 0.0;
+                const double r2 = r2_exact(dx, dy, dz);
+                if (r2 < Rh2) {
+                    const double q = sqrt_r2(r2) * h1;
+                    double w, dw;
+                    shape<K, DIM>(q, w, dw);
+                    sw += o.w * w;
+                    sg += o.w * ((double)DIM * w + q * dw);
+                }
+            };
             for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double sx, double sy,
                                                                           double sz) {
-                scan_range4(dpack, a, b, [&](int j, const double4& o) {
-                    const double dx = kShift ? (me.x - o.x) + sx : min_image(me.x - o.x, L0, H0, per);
-                    const double dy = kShift ? (me.y - o.y) + sy : min_image(me.y - o.y, L1, H1, per);
-                    const double dz =
-                        DIM == 3 ? (kShift ? (me.z - o.z) + sz : min_image(me.z - o.z, L2, H2, per)) : 0.0;
-                    const double r2 = r2_exact(dx, dy, dz);
-                    if (r2 < Rh2) {
-                        const double q = sqrt(r2) * h1;
-                        double w, dw;
-                        shape<K, DIM>(q, w, dw);
-                        sw += o.w * w;
-                        sg += o.w * ((double)DIM * w + q * dw);
-                    }
-                });
+                auto f = [&](int j, const double4& o) { body(j, o, sx, sy, sz); };
+                if constexpr (SPREAD) scan_range4_strided(dpack, a, b, g, Gk, f);
+                else scan_range4(dpack, a, b, f);
             }, sr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});
+    }
+    if constexpr (SPREAD) {
+        // every lane of the wave takes part; a particle's first Gk lanes
+        // exchange only among themselves
+        for (int off = 1; off < Gm; off <<= 1) {
+            const double ow = __shfl_xor(sw, off), og = __shfl_xor(sg, off);
+            if (off < Gk) {
+                sw += ow;
+                sg += og;
+            }
+        }
+        if (g != 0) live = active = false;
+    }
+
+    if (active) {
+        const double h1 = 1.0 / hi;
         const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
         const double rho_sum = sw * fac;
         const double dwdh = -h1 * (fac * sg);
@@ -211,9 +258,10 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
 // (next_list), so passes can be launched back to back without the host.
 // One workgroup.
 __device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) {
-    __shared__ int alldone;
+    __shared__ int alldone, spread;
     if (threadIdx.x == 0) {
         alldone = 1;
+        spread = 1;
         *act_count(ctl, next_list) = 0;
     }
     __syncthreads();
@@ -228,16 +276,23 @@ __device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) 
             bool stop = !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
             if (s.density_iterate && !(s.iters >= s.min_iter && conv) && s.iters >= s.max_iter) s.status |= 1;
             s.unconv = 0;
+            s.spread = spread_lanes(unconv, s.off1 - s.off0);
             s.hmax_bits = hconv > hnext ? hconv : hnext;
             s.hmax_next_bits = 0ull;
             stencil_from_hmax(s);
             if (stop) s.done = 1;
             atomicMax(&ctl->iters_max, s.iters);
         }
-        if (!s.done) atomicAnd(&alldone, 0);
+        if (!s.done) {
+            atomicAnd(&alldone, 0);
+            atomicMax(&spread, s.spread);
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) ctl->all_done = alldone;
+    if (threadIdx.x == 0) {
+        ctl->all_done = alldone;
+        ctl->spread = spread;
+    }
 }
 
 __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
@@ -255,15 +310,27 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
                                                     int ninst, int next_list, int32_t* arrive, int32_t* queue) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int B = blockDim.x;
+    const int G = act_in ? max(ctl->spread, 1) : 1;
+    auto run = [&](int64_t tile) {
+        if (G == 1)
+            density_block<K, DIM, false>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
+                                         act_out, cnt_out, tile, 1);
+        else
+            density_block<K, DIM, true>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
+                                        act_out, cnt_out, tile, G);
+    };
     if (queue) {  // persistent: tiles of this pass's items from the work queues
-        const int64_t items = act_in ? (int64_t)*cnt_in : n;
+        const int64_t items = (act_in ? (int64_t)*cnt_in : n) * G;
         int polls = 0;
-        for (int64_t tile; (tile = pull_tile(queue, (items + B - 1) / B, polls)) >= 0;)
-            density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out,
-                                  cnt_out, tile);
+        for (int64_t tile; (tile = pull_tile(queue, (items + B - 1) / B, polls)) >= 0;) run(tile);
+    } else if (act_in) {
+        // a later pass's items fill the first tiles: blocks take them in
+        // dispatch order, which spreads them over every XCD (an ensemble's
+        // G * count can pass the grid: then blocks take further tiles)
+        const int64_t tiles = ((int64_t)*cnt_in * G + B - 1) / B;
+        for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) run(tile);
     } else {
-        density_block<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in, act_out, cnt_out,
-                              xcd_tile((n + B - 1) / B));
+        run(xcd_tile((n + B - 1) / B));
     }
     if (last_to_arrive(arrive)) {
         g0_control_body(inst, ninst, ctl, next_list);

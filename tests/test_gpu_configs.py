@@ -164,10 +164,12 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
     """Every step of 500, every state field (and the rates every 100 steps).
     The rotor's shocks amplify rounding: the oracle run against a copy of
     itself whose x and y positions are moved by one ulp (helpers.nudged)
-    drifts past 1e-10 in psi and alpha after about 300 steps and reaches
-    ~5e-9 by step 500.  So at each step the GPU must lie within the larger of
-    1e-10 and 10 times that measured amplification (its largest field), and
-    it must meet 1e-10 outright over the first 300 steps.
+    stays near 3e-12 for 150 steps, passes 1e-11 near step 200 and 1e-10 at
+    step 273 (in psi), and reaches ~7e-8 by step 500.  So at each step the
+    GPU must lie within the larger of 1e-10 and 10 times that measured
+    amplification (its largest field): 1e-10 outright wherever the oracle's
+    own one-ulp drift is at most 1e-11, which must be the case on at least
+    150 of the steps.
 
     Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
@@ -184,7 +186,7 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
     for sysm in (dev, orc, nudged):
         sysm.warm_start()
     worst_strict = 0.0
-    last_strict = 0
+    strict = 0  # steps whose bar is 1e-10 itself
     for k in range(500):
         for sysm in (dev, orc, nudged):
             sysm.step(1)
@@ -193,12 +195,12 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
         grown = max(sens.values())
         tol = max(TOL, 10.0 * grown)
         w = compare(dev, orc, names, tol, f"C2 step {k + 1}")
-        if max(w.values()) <= TOL:
-            last_strict = k + 1 if last_strict == k else last_strict
-        if k < 300:
+        if tol == TOL:
+            strict += 1
             worst_strict = max(worst_strict, max(w.values()))
-    print(f"C2 rotor 128^2: worst rel-L2 over steps 1-300 {worst_strict:.3e}; "
-          f"within 1e-10 through step {last_strict}; oracle's own 1-ulp drift at 500: {max(sens.values()):.2e}")
+    print(f"C2 rotor 128^2: {strict} steps at 1e-10 itself (oracle's 1-ulp drift <= 1e-11), worst rel-L2 "
+          f"there {worst_strict:.3e}; oracle's own 1-ulp drift at 500: {max(sens.values()):.2e}")
+    assert strict >= 150, f"the oracle's own one-ulp drift was above 1e-11 at {500 - strict} of 500 steps"
     assert worst_strict <= TOL
     _totals_close(dev.totals()[0], orc.totals()[0], "C2 totals")
 
