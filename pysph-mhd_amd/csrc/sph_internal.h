@@ -206,13 +206,14 @@ __device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
 
 // sqrt of a squared distance: the Goldschmidt step and the two Newton
 // corrections of the library's correctly rounded sqrt, without its rescaling
-// of inputs below 2^-767 and its class test (zero, infinity, NaN).  Equal to
-// sqrt(r2) bit for bit for every r2 >= 2^-600; below, the result is sqrt of
-// 2^-600 (2^-300), which a distance of zero (a particle with itself) turns
-// into a q whose square vanishes beside 1 and whose product with a kernel
-// derivative vanishes beside W(0), so the density sums keep their bits.
+// of inputs below 2^-767 and its class test (zero, infinity, NaN).  The
+// input is r2 + 2^-600, which is r2 itself for every r2 >= 2^-546 (2^-600 is
+// below half its ulp), so the result equals sqrt(r2) bit for bit there; a
+// distance of zero (a particle with itself) gives 2^-300, a q whose square
+// vanishes beside 1 and whose product with a kernel derivative vanishes
+// beside W(0), so the density sums keep their bits.
 __device__ __forceinline__ double sqrt_r2(double r2) {
-    const double x = ::fmax(r2, 0x1p-600);
+    const double x = r2 + 0x1p-600;
     const double y = __builtin_amdgcn_rsq(x);
     double s = x * y, h = 0.5 * y;
     const double e = ::fma(-h, s, 0.5);

@@ -299,8 +299,10 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
     g0_control_body(inst, ninst, ctl, next_list);
 }
 
-// one density pass; its last workgroup runs the group control
-template <int K, int DIM>
+// one density pass; its last workgroup runs the group control.  LATER: a
+// pass after the first (act_in set), with the spread lanes; the first pass
+// has an instantiation of its own that carries none of that code.
+template <int K, int DIM, bool LATER>
 __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, const double4* __restrict__ dpack,
                                                     const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
@@ -310,20 +312,20 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
                                                     int ninst, int next_list, int32_t* arrive, int32_t* queue) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int B = blockDim.x;
-    const int G = act_in ? max(ctl->spread, 1) : 1;
+    const int G = LATER ? max(ctl->spread, 1) : 1;
     auto run = [&](int64_t tile) {
-        if (G == 1)
+        if (!LATER || G == 1)
             density_block<K, DIM, false>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
                                          act_out, cnt_out, tile, 1);
         else
-            density_block<K, DIM, true>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
-                                        act_out, cnt_out, tile, G);
+            density_block<K, DIM, LATER>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
+                                         act_out, cnt_out, tile, G);
     };
     if (queue) {  // persistent: tiles of this pass's items from the work queues
         const int64_t items = (act_in ? (int64_t)*cnt_in : n) * G;
         int polls = 0;
         for (int64_t tile; (tile = pull_tile(queue, (items + B - 1) / B, polls)) >= 0;) run(tile);
-    } else if (act_in) {
+    } else if (LATER) {
         // a later pass's items fill the first tiles: blocks take them in
         // dispatch order, which spreads them over every XCD (an ensemble's
         // G * count can pass the grid: then blocks take further tiles)
@@ -977,14 +979,16 @@ void launch_density(Dev& d) {
     int32_t* cout = act_count(d.ctl, p & 1);
     const int B = d.blk[LC_DENSITY];
     int32_t* q = d.persist[LC_DENSITY] ? d.queue : nullptr;
-#define CALL_DENS(KK, DD)                                                                                   \
-    { const unsigned g = q ? persistent_grid((const void*)k_density<KK, DD>, B, 0, nblk(d.n, B))            \
+#define CALL_DENS_L(KK, DD, LL)                                                                             \
+    { const unsigned g = q ? persistent_grid((const void*)k_density<KK, DD, LL>, B, 0, nblk(d.n, B))        \
                            : xcd_grid(nblk(d.n, B));                                                        \
-      hipLaunchKernelGGL((k_density<KK, DD>), dim3(g), dim3(B), 0, d.stream, d.fields,                      \
+      hipLaunchKernelGGL((k_density<KK, DD, LL>), dim3(g), dim3(B), 0, d.stream, d.fields,                  \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
                          ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive, q); }
+#define CALL_DENS(KK, DD) { if (p == 0) CALL_DENS_L(KK, DD, false) else CALL_DENS_L(KK, DD, true) }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS
+#undef CALL_DENS_L
 }
 
 void launch_g0_control(Dev& d) {
