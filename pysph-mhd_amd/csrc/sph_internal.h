@@ -917,6 +917,50 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
     double sx0[2];
     const int nseg0 = x_pieces(lo[0], hi[0], n0, xa0, xb0);
     x_shifts(lo[0], hi[0], n0, s.L[0], sx0);
+    if (!pr) {
+        // every row has the same x pieces: the next row's bounds are loaded
+        // while the current row's candidates are scanned, so a row costs no
+        // extra memory round trip before its first candidate
+        auto row_of = [&](int qy, int qz) {
+            const int wy = ((qy % n1) + n1) % n1;
+            const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+            return s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+        };
+        int qy = lo[1], qz = lo[2];
+        int64_t row = row_of(qy, qz);
+        int a0 = cstart[row + xa0[0]], b0 = cend[row + xb0[0]];
+        int a1 = nseg0 == 2 ? cstart[row + xa0[1]] : 0, b1 = nseg0 == 2 ? cend[row + xb0[1]] : 0;
+        while (true) {
+            int ny = qy + 1, nz = qz;
+            if (ny > hi[1]) {
+                ny = lo[1];
+                ++nz;
+            }
+            const bool more = nz <= hi[2];
+            int na0 = 0, nb0 = 0, na1 = 0, nb1 = 0;
+            if (more) {
+                const int64_t nrow = row_of(ny, nz);
+                na0 = cstart[nrow + xa0[0]];
+                nb0 = cend[nrow + xb0[0]];
+                if (nseg0 == 2) {
+                    na1 = cstart[nrow + xa0[1]];
+                    nb1 = cend[nrow + xb0[1]];
+                }
+            }
+            const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
+            const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
+            f(a0, b0, sx0[0], sy, sz);
+            if (nseg0 == 2) f(a1, b1, sx0[1], sy, sz);
+            if (!more) break;
+            qy = ny;
+            qz = nz;
+            a0 = na0;
+            b0 = nb0;
+            a1 = na1;
+            b1 = nb1;
+        }
+        return;
+    }
     for (int qz = lo[2]; qz <= hi[2]; ++qz) {
         const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
         const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
@@ -941,7 +985,9 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
     }
 }
 
-// f(j, rec) for j in [a, b), four independent loads in flight per thread
+// f(j, rec) for j in [a, b), four independent loads in flight per thread;
+// the last one to three candidates load together too (a stencil row holds a
+// few candidates in 3-D, so one at a time cost a memory round trip each)
 template <class T, class F>
 __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, int b, F&& f) {
     int j = a;
@@ -952,7 +998,13 @@ __device__ __forceinline__ void scan_range4(const T* __restrict__ rec, int a, in
         f(j + 2, o2);
         f(j + 3, o3);
     }
-    for (; j < b; ++j) f(j, rec[j]);
+    if (j < b) {
+        const int r = b - j;  // 1 to 3; the loads past the end repeat rec[j]
+        const T o0 = rec[j], o1 = rec[r > 1 ? j + 1 : j], o2 = rec[r > 2 ? j + 2 : j];
+        f(j, o0);
+        if (r > 1) f(j + 1, o1);
+        if (r > 2) f(j + 2, o2);
+    }
 }
 
 // f(j, rec) for j = a + g, a + g + G, ... < b: lane g of a group of G lanes
