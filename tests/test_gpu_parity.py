@@ -339,6 +339,56 @@ def test_ensemble_per_instance(gpu):
     assert np.allclose(td, to, rtol=1e-12, atol=0)
     assert len(set(np.round(dto, 12))) > 1  # instances really take different steps
 
+def test_spread_lanes_per_instance_equal_alone(gpu):
+    """Later density passes spread a particle over G lanes, G chosen per
+    instance from its own unconverged count and size.  Here instance 0
+    starts from h 25% too large (every particle stays unconverged after the
+    first pass: G = 1 in the second) beside an instance whose h is already
+    converged (taken from a warm start of it alone: nothing stays
+    unconverged, G = 64); both iterate at least twice, so the second pass of
+    the first evaluation maps 64 lanes per particle and instance 0 uses one
+    of them.  Each instance's fields after the warm start and two steps
+    equal, bit for bit, those of the same instance run alone, and both match
+    the oracle to 1e-10.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
+    from sphmhd.cases.baseline import ensemble_ot, jittered
+    from sphmhd.device import DeviceSystem
+
+    s = jittered(ensemble_ot(2, sizes=[48, 32]), 0.1, seed=5)
+    f = init_fields(s)
+    a0, a1, a2 = (int(v) for v in s.offsets)
+    f["h"] = f["h"].copy()
+    f["h"][a0:a1] *= 1.25
+    settle = DeviceSystem([s.params[1]], np.array([0, a2 - a1], dtype=np.int64))
+    settle.set_fields({nm: v[a1:a2] for nm, v in f.items()})
+    settle.warm_start()
+    f["h"][a1:a2] = settle.get("h")
+    settle.close()
+    dev, orc = _pair(s)
+    dev.set_fields(f)
+    orc.set_fields(f)
+    for sysm in (dev, orc):
+        sysm.warm_start()
+        sysm.step(2)
+    got = {nm: dev.get(nm) for nm in STATE + RATES}
+    compare(dev, orc, STATE + RATES, TOL, "spread ensemble")
+    passes = []
+    for k in range(2):
+        a, b = int(s.offsets[k]), int(s.offsets[k + 1])
+        one = DeviceSystem([s.params[k]], np.array([0, b - a], dtype=np.int64))
+        one.set_fields({nm: v[a:b] for nm, v in f.items()})
+        one.warm_start()
+        one.step(2)
+        for nm in STATE + RATES:
+            assert np.array_equal(got[nm][a:b], one.get(nm)), (k, nm)
+        passes.append(one.stats()["density_iterations"])
+        one.close()
+    assert passes[0] > passes[1], passes  # the perturbed instance iterates longer
+
 
 def test_orszag_tang_128_100_steps(gpu):
     """The CPU-runnable workload: 128^2 Orszag-Tang, 100 PEC steps, compared
