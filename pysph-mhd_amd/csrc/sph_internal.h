@@ -835,13 +835,17 @@ struct Prune {
     // trims [xl, xh] of the stencil row starting at global cell `row`;
     // false when nothing is left
     __device__ __forceinline__ bool trim(double g2yz, int64_t row, int& xl, int& xh) const {
-        if (far(g2yz, hrow ? bitsd(hrow[row]) : hb)) return false;
+        // without per-cell values a row's largest h (when given) bounds the
+        // h of every neighbour in it together with hb
+        const double hr = hrow ? bitsd(hrow[row]) : hb;
+        const double hrb = hcell ? hr : ::fmin(hr, hb);
+        if (far(g2yz, hrb)) return false;
         if (!hcell) {
             // start from the span the reach leaves along x, widened by a
             // cell each side (the gap tests below trim it exactly: a cell's
             // gap grows away from the particle), keeping the edge cells of
             // a periodic axis, whose gap counts as zero
-            const double r = R * ::fmax(hi, hb);
+            const double r = R * ::fmax(hi, hrb);
             const double rx = ::sqrt(::fmax(r * r * (1.0 + 1.0e-12) - g2yz, 0.0));
             const int a = (int)::floor((x[0] - rx - lo[0]) / c[0]) - 1;
             const int b = (int)::floor((x[0] + rx - lo[0]) / c[0]) + 1;
@@ -858,7 +862,7 @@ struct Prune {
         }
         auto skip = [&](int q) {
             const double g = gap(0, q);
-            if (!hcell) return far(g2yz + g * g, hb);
+            if (!hcell) return far(g2yz + g * g, hrb);
             const int w = q < 0 ? q + n[0] : (q >= n[0] ? q - n[0] : q);
             return far(g2yz + g * g, bitsd(hcell[row + w]));
         };
@@ -868,7 +872,9 @@ struct Prune {
     }
 };
 
-// hb: the bound on every h for cells without values (0 for a reach of R h_i)
+// hb: the bound on every h for cells without values (0 for a reach of R h_i);
+// hrow (the largest h of each row, at the row's first cell) drops and trims
+// rows by min(the row's largest h, hb) where no per-cell values are given
 template <int DIM>
 __device__ __forceinline__ void prune_init(Prune& p, const Inst& s, int64_t key, double x, double y, double z,
                                            double R, double hi, double hb, const unsigned long long* hcell,
@@ -891,7 +897,7 @@ __device__ __forceinline__ void prune_init(Prune& p, const Inst& s, int64_t key,
     p.hi = hi;
     p.hb = hb;
     p.hcell = hcell;
-    p.hrow = hcell ? hrow : nullptr;
+    p.hrow = hrow;
 }
 
 template <int DIM, class F>

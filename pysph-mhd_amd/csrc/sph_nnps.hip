@@ -510,11 +510,15 @@ __device__ __forceinline__ void nb_block(const double4* __restrict__ xh, const u
         const double hreach_i = narrow ? bitsd(hreach[skey[t]]) : 0.0;
         Prune pr;
         if (PRUNE && s.range_shift) {
+            // a particle whose reach passes its own h (near larger h) drops
+            // and trims each row by the row's largest h: only rows holding
+            // such h need the long reach (identical lists, fewer candidates)
             if (narrow)
-                prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, hreach_i, nullptr, nullptr);
+                prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, hreach_i, nullptr,
+                                hrow && hreach_i > hi ? hrow : nullptr);
             else
                 prune_init<DIM>(pr, s, (int64_t)skey[t], xi, yi, zi, R, hi, bitsd(s.hmax_bits),
-                                hcell && s.split > 1 ? hcell : nullptr, hrow);
+                                hcell && s.split > 1 ? hcell : nullptr, hcell && s.split > 1 ? hrow : nullptr);
         }
         int sri[3] = {s.srad[0], s.srad[1], s.srad[2]};
         if (narrow) {
