@@ -212,10 +212,17 @@ __global__ void k_a_pend(const Inst* __restrict__ inst, AdjInst* ai, const doubl
 // sum of the per-particle dt bars of one instance (one block per instance)
 __global__ void k_a_reduce_dt(const Inst* __restrict__ inst, AdjInst* ai, const double* __restrict__ row) {
     const Inst& s = inst[blockIdx.x];
-    double acc = 0.0;
-    for (int64_t i = s.off0 + threadIdx.x; i < s.off1; i += blockDim.x) acc += row[i];
+    // four partial sums, so four loads are in flight per thread
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t stride = blockDim.x;
+    int64_t i = s.off0 + threadIdx.x;
+    for (; i + 3 * stride < s.off1; i += 4 * stride) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += row[i + q * stride];
+    }
+    for (; i < s.off1; i += stride) acc[0] += row[i];
     __shared__ double red[kBlock];
-    red[threadIdx.x] = acc;
+    red[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
@@ -246,13 +253,11 @@ __global__ void k_a_reduce_par(const Inst* __restrict__ inst, AdjInst* ai, const
 // rows of the listed properties to or from a buffer (rows in list order)
 __global__ void k_a_rows(Fields F, RowList L, double* __restrict__ buf, int64_t n, int restore) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int r = 0; r < L.count; ++r) {
-        double* f = F.f[L.f[r]];
-        if (!f) continue;
-        if (restore) f[i] = buf[(int64_t)r * n + i];
-        else buf[(int64_t)r * n + i] = f[i];
-    }
+    const int r = blockIdx.y;  // one row per grid row: the row's pointer is uniform
+    double* f = F.f[L.f[r]];
+    if (i >= n || !f) return;
+    if (restore) f[i] = buf[(int64_t)r * n + i];
+    else buf[(int64_t)r * n + i] = f[i];
 }
 
 // per-instance clock {t, dt} to or from a device array
@@ -1036,6 +1041,12 @@ __global__ void k_a_zero_rows(double* __restrict__ p, int64_t count) {
 
 static inline unsigned ablk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
+// Elementwise reverse kernels run 64-thread workgroups: a rollout the
+// adjoint differentiates is small (C5: 16,296 particles), and 256-thread
+// workgroups left three quarters of the CUs idle while each thread walked a
+// long chain of dependent loads.
+constexpr int kAdjElem = 64;
+
 #define A_DISPATCH(KID, DIMV, CALL)                                                       \
     switch (KID) {                                                                        \
     case SPHMHD_CUBIC: if (DIMV == 2) CALL(SPHMHD_CUBIC, 2) else CALL(SPHMHD_CUBIC, 3) break; \
@@ -1048,8 +1059,9 @@ static inline unsigned ablk(int64_t n, int b = kBlock) { return (unsigned)((n + 
     }
 
 void launch_a_rows(Dev& d, const RowList& L, double* buf, bool restore) {
-    if (d.n) hipLaunchKernelGGL(k_a_rows, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, L, buf, d.n,
-                                restore ? 1 : 0);
+    if (d.n && L.count)
+        hipLaunchKernelGGL(k_a_rows, dim3(ablk(d.n, kAdjElem), L.count), dim3(kAdjElem), 0, d.stream, d.fields, L, buf,
+                           d.n, restore ? 1 : 0);
 }
 
 void launch_a_clock(Dev& d, double* clk, bool restore) {
@@ -1058,17 +1070,17 @@ void launch_a_clock(Dev& d, double* clk, bool restore) {
 }
 
 void launch_a_save_entry(Dev& d, AdjDev& a) {
-    if (d.n) hipLaunchKernelGGL(k_a_save_entry, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a.ain, d.n);
+    if (d.n) hipLaunchKernelGGL(k_a_save_entry, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a.ain, d.n);
 }
 
 void launch_a_seed_totals(Dev& d, AdjDev& a, const Weights& w) {
     if (d.n)
-        hipLaunchKernelGGL(k_a_seed_totals, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.inst,
+        hipLaunchKernelGGL(k_a_seed_totals, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a, d.inst,
                            d.inst_of, d.n, w);
 }
 
 void launch_a_zero(Dev& d, double* p, int64_t count) {
-    if (count > 0) hipLaunchKernelGGL(k_a_zero_rows, dim3(ablk(count)), dim3(kBlock), 0, d.stream, p, count);
+    if (count > 0) hipLaunchKernelGGL(k_a_zero_rows, dim3(ablk(count, kAdjElem)), dim3(kAdjElem), 0, d.stream, p, count);
 }
 
 // the pending dt's bar for the evaluation just recomputed: clk_next is the
@@ -1076,7 +1088,7 @@ void launch_a_zero(Dev& d, double* p, int64_t count) {
 void launch_a_eval_begin(Dev& d, AdjDev& a, const double* clk_next, int mode) {
     hipLaunchKernelGGL(k_a_begin_eval, dim3(ablk(d.ninst)), dim3(kBlock), 0, d.stream, a.ai, d.ninst);
     if (d.n)
-        hipLaunchKernelGGL(k_a_argmin, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, d.inst, a.ai, d.inst_of,
+        hipLaunchKernelGGL(k_a_argmin, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, d.inst, a.ai, d.inst_of,
                            d.n);
     hipLaunchKernelGGL(k_a_pend, dim3(ablk(d.ninst)), dim3(kBlock), 0, d.stream, d.inst, a.ai, clk_next, d.ninst,
                        mode);
@@ -1084,7 +1096,7 @@ void launch_a_eval_begin(Dev& d, AdjDev& a, const double* clk_next, int mode) {
 
 void launch_a_stage2(Dev& d, AdjDev& a) {
     if (d.n)
-        hipLaunchKernelGGL(k_a_stage2, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.inst, d.inst_of,
+        hipLaunchKernelGGL(k_a_stage2, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a, d.inst, d.inst_of,
                            d.n);
 }
 
@@ -1097,7 +1109,7 @@ void launch_a_eval(Dev& d, AdjDev& a) {
                            d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                           \
         hipLaunchKernelGGL((k_a_force_pair<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a, d.fpack, \
                            d.perm, d.nb, d.nbcnt, d.inst, d.inst_of, d.n, d.nb_stride);                           \
-        hipLaunchKernelGGL(k_a_prep, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.inst, d.inst_of,  \
+        hipLaunchKernelGGL(k_a_prep, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a, d.inst, d.inst_of,  \
                            d.n);                                                                                  \
         launch_pack_density(d);                                                                                   \
         hipLaunchKernelGGL((k_a_density<KK, DD>), dim3(g), dim3(kAdjBlock), 0, d.stream, d.fields, a,             \
@@ -1111,7 +1123,7 @@ void launch_a_eval(Dev& d, AdjDev& a) {
 
 void launch_a_stage1_init(Dev& d, AdjDev& a) {
     if (d.n)
-        hipLaunchKernelGGL(k_a_stage1_init, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.inst,
+        hipLaunchKernelGGL(k_a_stage1_init, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a, d.inst,
                            d.inst_of, d.n);
 }
 
@@ -1124,11 +1136,11 @@ void launch_a_reduce_par(Dev& d, AdjDev& a) {
 }
 
 void launch_a_sync(Dev& d, AdjDev& a) {
-    if (d.n) hipLaunchKernelGGL(k_a_sync, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, d.fields, a, d.n);
+    if (d.n) hipLaunchKernelGGL(k_a_sync, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, d.fields, a, d.n);
 }
 
 void launch_a_bsave(Dev& d, AdjDev& a, bool take) {
-    if (d.n) hipLaunchKernelGGL(k_a_bsave, dim3(ablk(d.n)), dim3(kBlock), 0, d.stream, a, d.n, take ? 1 : 0);
+    if (d.n) hipLaunchKernelGGL(k_a_bsave, dim3(ablk(d.n, kAdjElem)), dim3(kAdjElem), 0, d.stream, a, d.n, take ? 1 : 0);
 }
 
 }  // namespace sph

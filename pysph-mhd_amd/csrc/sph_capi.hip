@@ -740,7 +740,10 @@ static int adj_snap_save(sphmhd_ctx* c, sphmhd_ctx::Adj::Snap& sn) {
     if ((rc = grow(&sn.nb, &sn.nb_cap, nbn))) return rc;
     HIPCHK(hipMemcpyAsync(sn.nb, d.nb, sizeof(int32_t) * nbn, hipMemcpyDeviceToDevice, d.stream));
     sn.stride = d.nb_stride;
-    const int64_t tc = std::max<int64_t>(c->hcur->total_cells, 1);
+    // the whole cell table: the recomputation enqueues evaluations without
+    // waiting for their control records, so the host does not know this
+    // evaluation's cell count
+    const int64_t tc = std::max<int64_t>(d.cell_cap, 1);
     if ((rc = grow(&sn.cells, &sn.cells_cap, 2 * tc))) return rc;
     HIPCHK(hipMemcpyAsync(sn.cells, d.cstart, sizeof(int32_t) * tc, hipMemcpyDeviceToDevice, d.stream));
     HIPCHK(hipMemcpyAsync(sn.cells + tc, d.cend, sizeof(int32_t) * tc, hipMemcpyDeviceToDevice, d.stream));
@@ -1796,23 +1799,43 @@ int sphmhd_adjoint_reverse(sphmhd_ctx* c, double* param_bars) {
     const int64_t sn = (int64_t)kNState * n;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t len = std::min<int64_t>(A.every, A.steps - s * A.every);
-        // the segment again from its checkpoint, keeping each step's state,
-        // clock and evaluation
-        launch_a_rows(d, rows_state(), A.ckpt[s], true);
-        launch_a_clock(d, A.ckpt[s] + sn, true);
-        for (int64_t j = 0; j < len; ++j) {
-            sphmhd_ctx::Adj::Snap& sp = A.snaps[j];
-            launch_a_rows(d, rows_state(), A.seg + j * sn, false);
-            launch_a_clock(d, A.clk + 2 * ninst * j, false);
-            launch_initialize(d);
-            launch_stage1(d);
-            a.ain = sp.rows + (int64_t)kSnapRows * n;
-            launch_a_save_entry(d, a);
-            if ((rc = do_eval(c)) < 0) return rc;
-            if ((rc = adj_snap_save(c, sp))) return rc;
-            launch_stage2(d);
-            launch_clock(d);
-            HIPCHK(hipGetLastError());
+        // The segment again from its checkpoint, keeping each step's state,
+        // clock and evaluation.  The forward run took these very steps, so
+        // the recomputed evaluations are enqueued without a host wait each,
+        // with the density passes the recent evaluations needed; one that
+        // needed more passes or longer lists raises the abort flag, every
+        // kernel after it exits at entry, and the segment is then recomputed
+        // with the wait after each evaluation.
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            const bool wait_each = attempt > 0;
+            launch_a_rows(d, rows_state(), A.ckpt[s], true);
+            launch_a_clock(d, A.ckpt[s] + sn, true);
+            const int spec = next_spec(c);
+            for (int64_t j = 0; j < len; ++j) {
+                sphmhd_ctx::Adj::Snap& sp = A.snaps[j];
+                launch_a_rows(d, rows_state(), A.seg + j * sn, false);
+                launch_a_clock(d, A.clk + 2 * ninst * j, false);
+                launch_initialize(d);
+                launch_stage1(d);
+                a.ain = sp.rows + (int64_t)kSnapRows * n;
+                launch_a_save_entry(d, a);
+                if (wait_each) {
+                    if ((rc = do_eval(c)) < 0) return rc;
+                } else if ((rc = eval_front(c, spec))) {
+                    return rc;
+                }
+                if ((rc = adj_snap_save(c, sp))) return rc;
+                launch_stage2(d);
+                launch_clock(d);
+                HIPCHK(hipGetLastError());
+            }
+            if (wait_each) break;
+            if ((rc = read_ctl(c))) return rc;
+            // evaluations that exited at entry ended nothing: take the count
+            // of ended evaluations from the device
+            c->seq_next = c->hcur->seq;
+            if (!c->hcur->abort) break;
+            launch_clear_abort(d);
         }
         launch_a_clock(d, A.clk + 2 * ninst * len, false);
         // its steps backwards, each from its kept evaluation
