@@ -125,6 +125,26 @@ def test_adjoint_checkpoint_interval_does_not_change_the_gradient(gpu):
         assert np.array_equal(got[2], ref[2]), every
 
 
+def test_adjoint_recomputation_redone_with_waits_equals_default(gpu, monkeypatch):
+    """The reverse sweep recomputes each segment without a host wait per
+    evaluation; an evaluation that needs more density passes than were
+    launched raises the abort flag and the segment is recomputed with a wait
+    after each evaluation.  With one speculative pass (SPHMHD_SPEC=1) every
+    evaluation aborts, so every segment takes that second path: the value and
+    the gradient equal the default run's bit for bit."""
+    from sphmhd.adjoint import energy_gradient
+
+    s = _small()
+    _print("adjoint recomputation with waits: alfven 10x17, 6 steps, segments of 3")
+    value, grads, gpar = energy_gradient(s, 6, every=3)
+    monkeypatch.setenv("SPHMHD_SPEC", "1")
+    v1, g1, p1 = energy_gradient(s, 6, every=3)
+    assert v1 == value
+    for k in INPUTS:
+        assert np.array_equal(g1[k], grads[k]), k
+    assert list(p1.values()) == list(gpar.values())
+
+
 def test_c5_adjoint_400_steps_vs_central_differences(gpu):
     """C5: the 97 x 168 Alfven wave, 400 steps, checkpoints every 20 steps;
     the three gradients of the tangent sweep's test, from one reverse sweep."""
