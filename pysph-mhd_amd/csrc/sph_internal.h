@@ -225,6 +225,13 @@ __device__ __forceinline__ double sqrt_r2(double r2) {
     return ::fma(d, h, s);
 }
 
+// sqrt of a non-negative x by sqrt_r2's steps, exactly 0 at 0: equal to
+// sqrt(x) bit for bit for x = 0 and every x >= 2^-546 (NaN stays NaN)
+__device__ __forceinline__ double sqrt_nn(double x) {
+    const double s = sqrt_r2(x);
+    return x == 0.0 ? 0.0 : s;
+}
+
 __device__ __forceinline__ int cell_coord(double x, double lo, double ceff, int n) {
     double f = floor((x - lo) / ceff);
     if (f < 0.0) return 0;

@@ -575,7 +575,7 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
             double dy = min_image(yi - o[R::Y], Ly, hLy, periodic);
             double dz = DIM == 3 ? min_image(zi - o[R::Z], Lz, hLz, periodic) : 0.0;
             const double r2 = r2_exact(dx, dy, dz);
-            const double r = sqrt(r2);
+            const double r = sqrt_r2(r2);  // r2 = 0 (the particle itself) gives 2^-300: below both cut-offs
 
             // kernel gradients DWI, DWJ (0 for r <= 1e-12)
             double gi = 0.0, gj = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
@@ -605,7 +605,7 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
                 const double griA = near8 ? 0.0 : gri;
                 const double grjA = near8 ? 0.0 : grj;
                 A.div += mj * dotA * griA;
-                const double vsig2 = sqrt(fabs(pi_ - pj) / (0.5 * (rhoi + rhoj)));
+                const double vsig2 = sqrt_nn(fabs(pi_ - pj) / (0.5 * (rhoi + rhoj)));
                 double c = fmax(vwi - 2.0 * dotA, vwj - 2.0 * dotA);
                 c = fmax(c, 0.0);
                 A.dtcfl = fmax(A.dtcfl, c);
@@ -646,7 +646,7 @@ __device__ __forceinline__ void force_block(Fields F, const double* __restrict__
                 const double pmj = mj * r2ij * grj;
                 const double brt = -(pmi * pbi_);
                 const double tx = dvx - dot * ex, ty = dvy - dot * ey, tz = dvz - dot * ez;
-                const double vsb = sqrt(tx * tx + ty * ty + tz * tz);
+                const double vsb = sqrt_nn(tx * tx + ty * ty + tz * tz);
                 const double diss = 0.5 * (pmi + pmj) * vsb * ar;
                 const double dpsi = pmi * psichi + pmj * psichj;
                 A.bx += brt * dvx + diss * dbx - dpsi * ex;
