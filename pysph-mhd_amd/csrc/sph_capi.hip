@@ -917,6 +917,8 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if (const char* g = std::getenv("SPHMHD_SPEC")) c->spec_fixed = std::atoi(g);
         // SPHMHD_PRUNE_LISTS=1 / 0: always / never the pruned list kernel
         if (const char* g = std::getenv("SPHMHD_PRUNE_LISTS")) c->prune_mode = std::strcmp(g, "0") != 0;
+        // SPHMHD_DENSITY_TRIM=0: the 3-D first density pass scans whole stencil rows (same results)
+        if (const char* g = std::getenv("SPHMHD_DENSITY_TRIM")) d.dens_trim = std::strcmp(g, "0") != 0;
         d.prune_lists = c->prune_mode == 1;
         if (const char* g = std::getenv("SPHMHD_TUNE"))
             if (std::strcmp(g, "0") == 0) c->tune.state = 3;

@@ -998,6 +998,79 @@ __device__ __forceinline__ void for_each_row_range(const Inst& s, int64_t key, c
     }
 }
 
+// for_each_row_range for a particle at (x, y, z) that needs only the
+// candidates within sqrt(rt2) of it (with range shifts): rows farther than
+// that in y, z are passed over and the others trimmed along x to the cells
+// the radius reaches, by a margin far above the rounding of the binning and
+// of the arithmetic, keeping the edge cells of a periodic axis (they can hold
+// particles binned by clamping).  Every candidate within the radius is
+// visited, in the same order as by the full stencil.
+template <int DIM, class F>
+__device__ __forceinline__ void for_each_row_range_trim(const Inst& s, int64_t key, double x, double y, double z,
+                                                        double rt2, const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ cend, F&& f, const int* srad) {
+    const int n0 = s.n[0], n1 = s.n[1], n2 = s.n[2];
+    const int64_t local = key - s.cell_base;
+    const int cx = (int)(local % n0);
+    const int cy = (int)((local / n0) % n1);
+    const int cz = DIM == 3 ? (int)(local / ((int64_t)n0 * n1)) : 0;
+    int lo[3], hi[3];
+    const int c[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) {
+        const int na = s.n[a], sr = srad[a];
+        if (a >= DIM) { lo[a] = 0; hi[a] = 0; }
+        else if (s.periodic && 2 * sr + 1 >= na) { lo[a] = 0; hi[a] = na - 1; }
+        else if (s.periodic) { lo[a] = c[a] - sr; hi[a] = c[a] + sr; }
+        else { lo[a] = max(c[a] - sr, 0); hi[a] = min(c[a] + sr, na - 1); }
+    }
+    const int per = s.periodic;
+    // gap between the particle and unwrapped cell q of axis a
+    auto gap = [&](int a, int q, double xa) {
+        if (q == c[a]) return 0.0;
+        if (per) {
+            const int w = q < 0 ? q + s.n[a] : (q >= s.n[a] ? q - s.n[a] : q);
+            if (w == 0 || w == s.n[a] - 1) return 0.0;
+        }
+        const double g = q > c[a] ? (s.lo[a] + q * s.ceff[a]) - xa : xa - (s.lo[a] + (q + 1) * s.ceff[a]);
+        const double m = 1.0e-9 * s.ceff[a] + 1.0e-12 * (::fabs(s.lo[a]) + ::fabs(s.L[a]));
+        return g > m ? g - m : 0.0;
+    };
+    const double r2 = rt2 * (1.0 + 1.0e-12);
+    const double inv0 = 1.0 / s.ceff[0];
+    const double xr = (x - s.lo[0]) * inv0;  // x in cells
+    for (int qz = lo[2]; qz <= hi[2]; ++qz) {
+        const int wz = DIM == 3 ? ((qz % n2) + n2) % n2 : 0;
+        const double sz = DIM == 3 ? (qz < 0 ? s.L[2] : (qz >= n2 ? -s.L[2] : 0.0)) : 0.0;
+        const double gz = DIM == 3 ? gap(2, qz, z) : 0.0;
+        for (int qy = lo[1]; qy <= hi[1]; ++qy) {
+            const double gy = gap(1, qy, y);
+            const double g2 = gy * gy + gz * gz;
+            if (g2 > r2) continue;
+            const double wc = ::sqrt(r2 - g2) * inv0 + 1.0e-9;  // half-width in cells, widened
+            int xl = (int)::floor(xr - wc), xh = (int)::floor(xr + wc);
+            xl = xl > lo[0] ? xl : lo[0];
+            xh = xh < hi[0] ? xh : hi[0];
+            if (per) {
+                for (int q : {-1, 0, n0 - 1, n0})
+                    if (q >= lo[0] && q <= hi[0]) {
+                        xl = q < xl ? q : xl;
+                        xh = q > xh ? q : xh;
+                    }
+            }
+            if (xl > xh) continue;
+            const int wy = ((qy % n1) + n1) % n1;
+            const double sy = qy < 0 ? s.L[1] : (qy >= n1 ? -s.L[1] : 0.0);
+            const int64_t row = s.cell_base + (int64_t)n0 * (wy + (int64_t)n1 * wz);
+            int xa[2], xb[2];
+            double sx[2];
+            const int nseg = x_pieces(xl, xh, n0, xa, xb);
+            x_shifts(xl, xh, n0, s.L[0], sx);
+            f(cstart[row + xa[0]], cend[row + xb[0]], sx[0], sy, sz);
+            if (nseg == 2) f(cstart[row + xa[1]], cend[row + xb[1]], sx[1], sy, sz);
+        }
+    }
+}
+
 // f(j, rec) for j in [a, b), four independent loads in flight per thread;
 // the last one to three candidates load together too (a stencil row holds a
 // few candidates in 3-D, so one at a time cost a memory round trip each)
@@ -1318,6 +1391,7 @@ struct Dev {
     int64_t cell_cap;
     int32_t nb_stride;      // neighbour-list capacity per particle
     int32_t pass;           // density pass index within the evaluation
+    int32_t dens_trim = 1;  // 3-D first density pass trims its stencil rows to R h_i (same results)
     bool prune_lists;       // list builds use the pruned kernel (some grid was split)
     hipStream_t stream;
     // device arrays

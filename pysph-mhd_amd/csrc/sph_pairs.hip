@@ -58,14 +58,14 @@ __device__ __forceinline__ int spread_lanes(int64_t count, int64_t n) {
     return G;
 }
 
-template <int K, int DIM, bool SPREAD>
+template <int K, int DIM, bool SPREAD, bool TRIM = false>
 __device__ __forceinline__ void density_block(Fields F, const double4* __restrict__ dpack,
                                               const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                               const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
                                               Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                               const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                               int32_t* __restrict__ act_out, int32_t* cnt_out, int64_t tile,
-                                              int G) {
+                                              int G, int trim = 1) {
     const int B = blockDim.x;  // 64, 128 or 256 (the launch shape is tuned)
     const int64_t idx0 = tile * B;
     // Gm lanes mapped to each item (B is a multiple of Gm, so they are
@@ -125,12 +125,20 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
                     sg += o.w * ((double)DIM * w + q * dw);
                 }
             };
-            for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, [&](int a, int b, double sx, double sy,
-                                                                          double sz) {
+            auto rf = [&](int a, int b, double sx, double sy, double sz) {
                 auto f = [&](int j, const double4& o) { body(j, o, sx, sy, sz); };
                 if constexpr (SPREAD) scan_range4_strided(dpack, a, b, g, Gk, f);
                 else scan_range4(dpack, a, b, f);
-            }, sr);
+            };
+            // the first pass in 3-D scans only the cells within R h_i of the
+            // particle (the same hits in the same order, fewer candidates)
+            if constexpr (DIM == 3 && !SPREAD && kShift && TRIM) {
+                if (trim) {
+                    for_each_row_range_trim<DIM>(s, (int64_t)skey[t], me.x, me.y, me.z, Rh2, cstart, cend, rf, sr);
+                    return;
+                }
+            }
+            for_each_row_range<DIM>(s, (int64_t)skey[t], cstart, cend, rf, sr);
         };
         if (s.range_shift) scan(std::true_type{});
         else scan(std::false_type{});
@@ -309,12 +317,16 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
                                                     Inst* inst, const int32_t* __restrict__ inst_of, int64_t n,
                                                     const int32_t* __restrict__ act_in, const int32_t* cnt_in,
                                                     int32_t* __restrict__ act_out, int32_t* cnt_out, Ctl* ctl,
-                                                    int ninst, int next_list, int32_t* arrive, int32_t* queue) {
+                                                    int ninst, int next_list, int32_t* arrive, int32_t* queue,
+                                                    int trim) {
     if (ctl->abort) return;  // the whole launch: the step after an unfinished evaluation
     const int B = blockDim.x;
     const int G = LATER ? max(ctl->spread, 1) : 1;
     auto run = [&](int64_t tile) {
-        if (!LATER || G == 1)
+        if (!LATER)
+            density_block<K, DIM, false, true>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
+                                               act_out, cnt_out, tile, 1, trim);
+        else if (G == 1)
             density_block<K, DIM, false>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
                                          act_out, cnt_out, tile, 1);
         else
@@ -984,7 +996,7 @@ void launch_density(Dev& d) {
                            : xcd_grid(nblk(d.n, B));                                                        \
       hipLaunchKernelGGL((k_density<KK, DD, LL>), dim3(g), dim3(B), 0, d.stream, d.fields,                  \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
-                         ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive, q); }
+                         ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive, q, d.dens_trim); }
 #define CALL_DENS(KK, DD) { if (p == 0) CALL_DENS_L(KK, DD, false) else CALL_DENS_L(KK, DD, true) }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS

@@ -279,6 +279,31 @@ def test_blast_3d_steps(gpu):
         compare(dev, orc, STATE + RATES + ("z",), TOL, f"3d step {k + 1}")
 
 
+def test_trimmed_density_rows_equal_full_rows(gpu, monkeypatch):
+    """The 3-D first density pass trims each stencil row to the cells within
+    R h_i of the particle (rows farther in y, z passed over): it visits the
+    same hits in the same order, so a jittered 20^3 blast through the warm
+    start and four steps equals, bit for bit, the run whose first pass scans
+    whole rows (SPHMHD_DENSITY_TRIM=0).  A missed candidate near the kernel's
+    edge would change the sums by far less than the oracle tolerance, so the
+    comparison is bitwise."""
+    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.device import DeviceSystem
+
+    s = jittered(blast_3d(20), 0.1, seed=4)
+    got = []
+    for trim in ("1", "0"):
+        monkeypatch.setenv("SPHMHD_DENSITY_TRIM", trim)
+        dev = DeviceSystem(s.params, s.offsets)
+        dev.set_fields(init_fields(s))
+        dev.warm_start()
+        dev.step(4)
+        got.append({nm: dev.get(nm) for nm in STATE + RATES + ("z",)})
+        dev.close()
+    for nm in got[0]:
+        assert np.array_equal(got[0][nm], got[1][nm]), nm
+
+
 def test_rotor_variable_mass(gpu):
     """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
