@@ -22,7 +22,7 @@ NEAR_TOL = 1.0e-14
 PROPS = ("x", "y", "z", "u", "v", "w", "rho", "h", "m", "e", "Bx", "By", "Bz", "alpha1")
 
 # fields that go through a transcendental function, per case
-BASELINE_NEAR = {
+WORKLOAD_NEAR = {
     "orszag_tang_2d": ("u", "v", "Bx", "By"),
     "rotor_2d": (),
     "blast_3d": (),
@@ -73,7 +73,7 @@ def test_baseline_fields_on_device(gpu, name, args):
     dev = DeviceSystem(lean.params, lean.offsets, device=gpu)
     load_setup(dev, lean)
     torch.cuda.synchronize()
-    _check(name, host.fields, dev.state(PROPS), BASELINE_NEAR[name])
+    _check(name, host.fields, dev.state(PROPS), WORKLOAD_NEAR[name])
     dev.close()
 
 
@@ -91,7 +91,7 @@ def test_ensemble_fields_on_device(gpu):
     dev = DeviceSystem(lean.params, lean.offsets, device=gpu)
     load_setup(dev, lean)
     torch.cuda.synchronize()
-    _check("ensemble_ot", host.fields, dev.state(PROPS), BASELINE_NEAR["orszag_tang_2d"])
+    _check("ensemble_ot", host.fields, dev.state(PROPS), WORKLOAD_NEAR["orszag_tang_2d"])
     dev.close()
 
 
