@@ -159,7 +159,7 @@ def calibrate_copy():
 def make_setup(name: str, host_fields: bool = True):
     """The workload's setup; host_fields=False leaves the initial fields to
     the GPU generator (cases.device.load_setup)."""
-    from sphmhd.cases import baseline as B
+    from sphmhd.cases import workloads as B
 
     hf = host_fields
     if name == "rotor512":
@@ -242,7 +242,7 @@ def run_tangent(steps: int, warmup: int = 2):
     gradient of any final-time total in those three inputs."""
     import torch
 
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
     from sphmhd.device import DeviceSystem
     from sphmhd.tangent import TangentSystem, _ic_tangent
 
@@ -288,7 +288,7 @@ def run_adjoint(steps: int = 400, every: int = 20):
     import torch
 
     from sphmhd.adjoint import ENERGY, AdjointSystem
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
     from sphmhd.device import DeviceSystem
 
     setup = alfven_2d(97, 168)
@@ -450,7 +450,7 @@ def cpu_ot128(steps: int = 100, steps_one_thread: int = 20):
     sys.path.insert(0, str(ROOT / "oracle"))
     from pyoracle import OracleSystem
 
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     cores = len(os.sched_getaffinity(0))
     s = orszag_tang_2d(128)

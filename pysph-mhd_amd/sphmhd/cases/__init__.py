@@ -1,2 +1,2 @@
-"""Initial conditions: the benchmark workloads (baseline) and the reference's
+"""Initial conditions: the benchmark workloads (workloads) and the reference's
 case applications (reference-mode HCP slabs)."""

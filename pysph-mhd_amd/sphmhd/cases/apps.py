@@ -11,7 +11,7 @@ from __future__ import annotations
 
 from ..application import MHDApplication
 from ..base import get_particle_array
-from . import baseline as B
+from . import workloads as B
 
 
 def _pa(fields):

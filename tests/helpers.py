@@ -90,7 +90,7 @@ def case_setup(name, nx=None):
     """Setup (parameters from the application's plan, domain and solver;
     initial properties from its particle array) of a reference case."""
     from sphmhd.backend import plan_params
-    from sphmhd.cases.baseline import Setup
+    from sphmhd.cases.workloads import Setup
 
     app = case_app(name, nx)
     pa = app.particles[0]

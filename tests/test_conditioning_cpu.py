@@ -42,7 +42,7 @@ def _drift(setup, steps=3):
 
 @pytest.mark.parametrize("n,ill", [(64, True), (67, False)])
 def test_exact_lattice_conditioning(n, ill):
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     worst, field = _drift(orszag_tang_2d(n))
     if ill:
@@ -52,7 +52,7 @@ def test_exact_lattice_conditioning(n, ill):
 
 
 def test_jitter_restores_conditioning():
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     worst, field = _drift(jittered(orszag_tang_2d(64), 0.1, seed=64))
     assert worst < 1e-11, (worst, field)

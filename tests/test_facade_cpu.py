@@ -152,7 +152,7 @@ def test_device_path_fails_loudly_without_gpu():
 
     if torch.cuda.is_available():
         pytest.skip("a GPU is present")
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
     from sphmhd.device import DeviceSystem
 
     s = orszag_tang_2d(8)

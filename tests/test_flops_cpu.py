@@ -34,7 +34,7 @@ STATE = ("x", "y", "z", "u", "v", "w", "e", "h", "rho", "omega", "au", "av", "ae
 
 
 def _setup(workload):
-    from sphmhd.cases import baseline as B
+    from sphmhd.cases import workloads as B
 
     return {
         "blast128": lambda: B.blast_3d(24),
@@ -89,7 +89,7 @@ def test_counting_build_computes_the_oracle(which):
     """The counted library is the oracle: a warm start and three steps give
     every field bit for bit."""
     from pyoracle import OracleSystem
-    from sphmhd.cases import baseline as B
+    from sphmhd.cases import workloads as B
 
     s = B.jittered(B.orszag_tang_2d(24)) if which == "ot" else B.jittered(B.blast_3d(10))
     out = []

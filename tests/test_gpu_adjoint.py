@@ -39,7 +39,7 @@ TPARAMS = ("artificial_viscosity", "artificial_thermal_conductivity", "artificia
 
 
 def _small(amplitude=0.1, **kw):
-    from sphmhd.cases.baseline import alfven_2d, jittered
+    from sphmhd.cases.workloads import alfven_2d, jittered
 
     return jittered(alfven_2d(10, 17, amplitude=amplitude, **kw), 0.1, seed=5)
 
@@ -85,7 +85,7 @@ def test_adjoint_matches_taped_oracle(gpu):
 
 def test_adjoint_equals_tangent_directions(gpu):
     from sphmhd.adjoint import energy_gradient
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
     from sphmhd.tangent import energy_rollout
 
     def setup_fn(amplitude=0.1, alpha_max=1.0, cleaning_damping_factor=1.0):
@@ -149,7 +149,7 @@ def test_c5_adjoint_400_steps_vs_central_differences(gpu):
     """C5: the 97 x 168 Alfven wave, 400 steps, checkpoints every 20 steps;
     the three gradients of the tangent sweep's test, from one reverse sweep."""
     from sphmhd.adjoint import energy_gradient
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
     from sphmhd.tangent import energy_primal
 
     steps = 400
@@ -218,7 +218,7 @@ def test_adjoint_3d_matches_taped_oracle(gpu):
     components."""
     from pyoracle import tape_rollout
     from sphmhd.adjoint import energy_gradient
-    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.cases.workloads import blast_3d, jittered
 
     s = jittered(blast_3d(6), 0.1, seed=4)
     _print("adjoint vs taped oracle: blast 6^3 jittered, warm start + 1 step")
@@ -263,7 +263,7 @@ def test_adjoint_ensemble_instances_equal_alone(gpu):
     import torch
 
     from sphmhd.adjoint import AdjointSystem
-    from sphmhd.cases.baseline import Setup, ensemble_ot, jittered
+    from sphmhd.cases.workloads import Setup, ensemble_ot, jittered
 
     ens = jittered(ensemble_ot(2, seed=3, sizes=[14, 18]), 0.1, seed=9)
 

@@ -60,10 +60,10 @@ def _check(label, host: dict, dev: dict, near):
     ("blast_3d", dict(nx=20)),
     ("alfven_2d", dict(nx=17, ny=30)),
 ])
-def test_baseline_fields_on_device(gpu, name, args):
+def test_workload_fields_on_device(gpu, name, args):
     import torch
 
-    from sphmhd.cases import baseline as B
+    from sphmhd.cases import workloads as B
     from sphmhd.cases.device import load_setup
     from sphmhd.device import DeviceSystem
 
@@ -81,7 +81,7 @@ def test_ensemble_fields_on_device(gpu):
     """Uneven instances, each generated into its slice of the storage rows."""
     import torch
 
-    from sphmhd.cases import baseline as B
+    from sphmhd.cases import workloads as B
     from sphmhd.cases.device import load_setup
     from sphmhd.device import DeviceSystem
 

@@ -61,7 +61,7 @@ def _totals_close(td, to, label):
 # ---- C5: differentiable Alfven rollout ------------------------------------------
 
 def _alfven(nx=97, ny=168, **kw):
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
 
     return alfven_2d(nx, ny, **kw)
 
@@ -176,7 +176,7 @@ def test_c2_rotor_128_every_step_500_steps(gpu):
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
     from pyoracle import OracleSystem
-    from sphmhd.cases.baseline import rotor_2d
+    from sphmhd.cases.workloads import rotor_2d
 
     s = rotor_2d(128)
     dev, orc = _pair(s)
@@ -212,7 +212,7 @@ def test_c3_blast_32_vs_oracle(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import blast_3d
+    from sphmhd.cases.workloads import blast_3d
 
     s = blast_3d(32)
     dev, orc = _pair(s)
@@ -236,7 +236,7 @@ def test_c3_blast_128_properties(gpu):
     permutation of the state, whose grid the scan-cost model sizes from the
     h histogram, equal the oracle's bit for bit."""
     from pyoracle import OracleSystem
-    from sphmhd.cases.baseline import blast_3d
+    from sphmhd.cases.workloads import blast_3d
     from sphmhd.device import DeviceSystem
 
     s = blast_3d(128)
@@ -278,7 +278,7 @@ def test_c4_ensemble_64_each_instance_equals_alone(gpu):
     bit for bit, those of the same instance run alone in a context of its
     own (each particle's sums run over the same neighbours in the same
     order; the instances only share launches)."""
-    from sphmhd.cases.baseline import ensemble_ot
+    from sphmhd.cases.workloads import ensemble_ot
     from sphmhd.device import DeviceSystem
 
     s = ensemble_ot(64)
@@ -330,7 +330,7 @@ def test_c4_ensemble_64_jittered_vs_oracle(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import ensemble_ot, jittered
+    from sphmhd.cases.workloads import ensemble_ot, jittered
 
     s = jittered(ensemble_ot(64), 0.1, seed=64)
     dev, orc = _pair(s)
@@ -364,7 +364,7 @@ def test_c4_ensemble_64_exact_lattices_vs_oracle(gpu):
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
     from pyoracle import OracleSystem
-    from sphmhd.cases.baseline import ensemble_ot
+    from sphmhd.cases.workloads import ensemble_ot
 
     s = ensemble_ot(64)
     dev, orc = _pair(s)

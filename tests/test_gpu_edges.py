@@ -16,7 +16,7 @@ TOL = 1.0e-10
 
 
 def _ot(nx, seed):
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     return jittered(orszag_tang_2d(nx), 0.1, seed=seed)
 

@@ -47,7 +47,7 @@ def test_application_matches_oracle(gpu, tmp_path):
     other field against its own norm."""
     from pyoracle import OracleSystem
     from sphmhd.cases.apps import OrszagTang2D
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     cfg = _config(tmp_path, 128, 10)
     app = OrszagTang2D(cfg)

@@ -15,7 +15,7 @@ SHAPES = [(256, 256, 256), (64, 64, 64), (128, 128, 128), (64, 256, 128), (256, 
 
 
 def _case(which):
-    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered, orszag_tang_2d, rotor_2d
+    from sphmhd.cases.workloads import blast_3d, ensemble_ot, jittered, orszag_tang_2d, rotor_2d
 
     if which == "ot2d":
         return jittered(orszag_tang_2d(40), 0.1, seed=40)

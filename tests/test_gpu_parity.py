@@ -40,7 +40,7 @@ def _pair(setup, threads=None):
 
 
 def _ot(nx, jitter=True, **kw):
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = orszag_tang_2d(nx, **kw)
     return jittered(s, 0.1, seed=nx) if jitter else s
@@ -85,7 +85,7 @@ def test_neighbours_bit_exact(gpu, which, prune, monkeypatch):
     large (blocks straddling instances scan from global memory); "forced"
     builds the lists with the pruned kernel (stencil cells beyond R h_max of
     the particle passed over) on these unsplit grids."""
-    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered, rotor_2d
+    from sphmhd.cases.workloads import blast_3d, ensemble_ot, jittered, rotor_2d
 
     if prune == "forced":
         monkeypatch.setenv("SPHMHD_PRUNE_LISTS", "1")
@@ -266,7 +266,7 @@ def test_blast_3d_steps(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.cases.workloads import blast_3d, jittered
 
     s = jittered(blast_3d(16), 0.05, seed=2)
     dev, orc = _pair(s)
@@ -287,7 +287,7 @@ def test_trimmed_density_rows_equal_full_rows(gpu, monkeypatch):
     whole rows (SPHMHD_DENSITY_TRIM=0).  A missed candidate near the kernel's
     edge would change the sums by far less than the oracle tolerance, so the
     comparison is bitwise."""
-    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.cases.workloads import blast_3d, jittered
     from sphmhd.device import DeviceSystem
 
     s = jittered(blast_3d(20), 0.1, seed=4)
@@ -309,7 +309,7 @@ def test_rotor_variable_mass(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import jittered, rotor_2d
+    from sphmhd.cases.workloads import jittered, rotor_2d
 
     s = jittered(rotor_2d(48), 0.1, seed=4)
     dev, orc = _pair(s)
@@ -347,9 +347,9 @@ def test_ensemble_per_instance(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import ensemble_ot
+    from sphmhd.cases.workloads import ensemble_ot
 
-    from sphmhd.cases.baseline import jittered
+    from sphmhd.cases.workloads import jittered
 
     s = jittered(ensemble_ot(5, sizes=[24, 40, 32, 48, 28]), 0.1, seed=11)
     dev, orc = _pair(s)
@@ -380,7 +380,7 @@ def test_spread_lanes_per_instance_equal_alone(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import ensemble_ot, jittered
+    from sphmhd.cases.workloads import ensemble_ot, jittered
     from sphmhd.device import DeviceSystem
 
     s = jittered(ensemble_ot(2, sizes=[48, 32]), 0.1, seed=5)
@@ -423,7 +423,7 @@ def test_orszag_tang_128_100_steps(gpu):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(128)
     dev, orc = _pair(s, threads=16)
@@ -477,7 +477,7 @@ def test_run_stops_at_tf(gpu, monkeypatch, graphs):
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
     from sphmhd.device import DeviceSystem
     from pyoracle import OracleSystem
 
@@ -519,7 +519,7 @@ def test_strain_diagnostics_on_request(gpu, which):
     3-D the inverted one.  Velocity gradients are measured against 1e-3 of
     v_w / h where a component is analytically zero (w and z gradients in
     2-D)."""
-    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.cases.workloads import blast_3d, jittered
     from sphmhd.device import DeviceSystem
 
     s = _ot(40) if which == "ot2d" else jittered(blast_3d(14), 0.1, seed=3)

@@ -26,7 +26,7 @@ THETA = (0.1, 1.0, 1.0)
 
 
 def _alfven(amplitude=0.1, alpha_max=1.0, cleaning_damping_factor=1.0):
-    from sphmhd.cases.baseline import alfven_2d
+    from sphmhd.cases.workloads import alfven_2d
 
     return alfven_2d(24, 42, amplitude=amplitude, alpha_max=alpha_max,
                      cleaning_damping_factor=cleaning_damping_factor)

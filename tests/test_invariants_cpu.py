@@ -41,7 +41,7 @@ def _eval_from_initial(s, **extra):
 
 
 def test_uniform_state_has_no_forces():
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(24)
     n = s.n
@@ -60,7 +60,7 @@ def test_uniform_state_has_no_forces():
 
 
 def test_galilean_invariance():
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(20), 0.2, seed=1)
     a = _eval_from_initial(s)
@@ -72,7 +72,7 @@ def test_galilean_invariance():
 
 
 def test_rotation_by_90_degrees():
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(20), 0.2, seed=2)
     a = _eval_from_initial(s)
@@ -91,7 +91,7 @@ def test_rotation_by_90_degrees():
 
 
 def test_momentum_conserved_without_divb_correction():
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(24, b0_factor=0.05), 0.2, seed=3)
     o = _eval_from_initial(s)
@@ -107,7 +107,7 @@ def test_momentum_conserved_without_divb_correction():
 
 
 def _blast_interior(nx=14, seed=4):
-    from sphmhd.cases.baseline import blast_3d, jittered
+    from sphmhd.cases.workloads import blast_3d, jittered
 
     s = jittered(blast_3d(nx), 0.15, seed=seed)
     s.fields["e"] = np.full(s.n, 1.5)
@@ -147,7 +147,7 @@ def test_density_fixed_point_matches_bisection():
     sum_j m W(r_j, h) = m / (h/k)^2; solved here independently by bisection
     over the same kernel function."""
     import pyoracle as po
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(32)
     o = _oracle(s)
@@ -178,7 +178,7 @@ def test_uniform_flow_drifts_and_alpha_decays():
     """A uniform state moving at constant velocity: no force, exact ballistic
     drift (wrapped), and alpha relaxes by the implicit update
     alpha1 = alpha10 / (1 + dt * 0.1 cs / h) since alpha_loc = 0."""
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(16)
     s.params[0].alpha_min = 0.0

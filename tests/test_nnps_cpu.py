@@ -14,7 +14,7 @@ import pytest
 
 
 def _setup(kind, seed=0):
-    from sphmhd.cases.baseline import blast_3d, ensemble_ot, jittered, orszag_tang_2d, rotor_2d
+    from sphmhd.cases.workloads import blast_3d, ensemble_ot, jittered, orszag_tang_2d, rotor_2d
 
     if kind == "ot2d":
         return jittered(orszag_tang_2d(24), 0.2, seed=seed)
@@ -54,7 +54,7 @@ def test_cell_list_equals_brute_force(kind):
 
 
 def test_non_periodic_box():
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(16), 0.2, seed=9)
     s.params[0].periodic = 0
@@ -65,7 +65,7 @@ def test_non_periodic_box():
 
 
 def test_minimum_image_pairs_across_the_boundary():
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(16)
     o = _oracle(s)
@@ -126,7 +126,7 @@ def test_3d_cells_follow_the_bulk_h(bulk_frac):
     cells for: c = R h_max / 4), and the search still equals the all-pairs
     one."""
     from pyoracle import OracleSystem
-    from sphmhd.cases.baseline import blast_3d
+    from sphmhd.cases.workloads import blast_3d
 
     nx = 20
     s = blast_3d(nx)

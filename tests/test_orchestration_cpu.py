@@ -34,7 +34,7 @@ def _oracle(setup):
 
 
 def _uniform(nx=16, vel=(0.0, 0.0, 0.0), bfield=(0.2, 0.1, 0.05), chf=1.0, adaptive=True):
-    from sphmhd.cases.baseline import orszag_tang_2d
+    from sphmhd.cases.workloads import orszag_tang_2d
 
     s = orszag_tang_2d(nx, cleaning_speed_factor=chf)
     s.params[0].adaptive_timestep = 1 if adaptive else 0
@@ -77,7 +77,7 @@ def test_pending_step_is_the_minimum_over_particles():
     """Two regions: the right half ten times hotter.  The pending step is the
     smallest per-particle step, and the particle attaining it lies in the hot
     half, where the sound speed is larger."""
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(24), 0.1, seed=5)
     hot = s.fields["x"] > 0.5
@@ -103,7 +103,7 @@ def test_step_clipped_to_tf_and_run_stops(adaptive):
     """A run bounded by tf: the steps before it are those of an unbounded run
     (bit for bit), the step that would pass tf is cut to tf - t, the clock
     lands on tf and the run stops there instead of taking zero-length steps."""
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     base = jittered(orszag_tang_2d(16), 0.1, seed=2)
     base.params[0].adaptive_timestep = 1 if adaptive else 0
@@ -137,7 +137,7 @@ def test_initial_step_equals_application_formula(adaptive):
     and the solver's re-read (solver.py:40: the pending minimum when
     adaptive, the application's step otherwise) give the same first step."""
     from sphmhd.application import initial_time_step
-    from sphmhd.cases.baseline import jittered, rotor_2d
+    from sphmhd.cases.workloads import jittered, rotor_2d
 
     s = jittered(rotor_2d(32), 0.1, seed=7)
     s.params[0].adaptive_timestep = 1 if adaptive else 0
@@ -179,7 +179,7 @@ def test_warm_start_sequence_invariants():
     every particle stays converged and h, omega, rho keep evaluation 1's
     bits.  The field is the initial one put back and re-synced with that rho:
     B = (B0 / rho) * rho exactly."""
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     s = jittered(orszag_tang_2d(24), 0.1, seed=9)
     one = _oracle(s)

@@ -33,7 +33,7 @@ STEPS = 2
 
 
 def _setup(amplitude=0.1):
-    from sphmhd.cases.baseline import alfven_2d, jittered
+    from sphmhd.cases.workloads import alfven_2d, jittered
 
     return jittered(alfven_2d(10, 17, amplitude=amplitude), 0.1, seed=5)
 

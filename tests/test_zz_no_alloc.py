@@ -37,7 +37,7 @@ def _system(setup):
 def test_totals_noalloc_matches_totals_twice(gpu):
     import torch
 
-    from sphmhd.cases.baseline import ensemble_ot, jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import ensemble_ot, jittered, orszag_tang_2d
 
     big = _system(jittered(ensemble_ot(6, sizes=[96, 40, 131, 64, 77, 150]), 0.1, seed=3))
     small = _system(jittered(orszag_tang_2d(24), 0.1, seed=4))
@@ -51,7 +51,7 @@ def test_totals_noalloc_matches_totals_twice(gpu):
 
 def test_totals_noalloc_rejects_bad_output(gpu):
     from sphmhd._abi import SPHMHDError
-    from sphmhd.cases.baseline import jittered, orszag_tang_2d
+    from sphmhd.cases.workloads import jittered, orszag_tang_2d
 
     dev = _system(jittered(orszag_tang_2d(16), 0.1, seed=5))
     with pytest.raises(SPHMHDError):
