@@ -156,11 +156,13 @@ def test_steps_lockstep(gpu):
         compare(dev, orc, STATE + RATES, TOL, f"step {k + 1}")
 
 
-def test_fused_multistep_equals_single_steps(gpu):
-    """sphmhd_step(n) (fused stage kernels) against n calls of sphmhd_step(1),
-    the diagnostic rows its earlier steps skip included."""
+def test_fused_multistep_equals_single_steps(gpu, monkeypatch):
+    """sphmhd_step(n) (fused stage kernels, later steps replayed from a graph)
+    against n calls of sphmhd_step(1), the diagnostic rows its earlier steps
+    skip included."""
     from sphmhd.device import DeviceSystem
 
+    monkeypatch.setenv("SPHMHD_GRAPHS", "1")
     s = _ot(40)
     a = DeviceSystem(s.params, s.offsets)
     b = DeviceSystem(s.params, s.offsets)
