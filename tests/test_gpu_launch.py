@@ -70,7 +70,7 @@ def test_persistent_launches_give_the_same_steps(gpu, which):
         assert np.array_equal(clock[0], ref_clock[0]) and np.array_equal(clock[1], ref_clock[1])
 
 
-def test_autotuned_run_equals_fixed_run(gpu):
+def test_autotuned_run_equals_fixed_run(gpu, monkeypatch):
     """The first steps of a context rotate the configurations (shape and
     plain or persistent launch, timed with events) and keep the fastest; the
     trajectory is that of any fixed configuration."""
@@ -80,6 +80,7 @@ def test_autotuned_run_equals_fixed_run(gpu):
 
     if os.environ.get("SPHMHD_PERSIST") is not None or os.environ.get("SPHMHD_TUNE") == "0":
         pytest.skip("launch tuning turned off by the environment (SPHMHD_PERSIST / SPHMHD_TUNE=0)")
+    monkeypatch.delenv("SPHMHD_SPEC", raising=False)  # one force launch per step below
     s = _case("ot2d")
     ref, ref_clock, _ = _run(s, (256, 256, 256), 15)
     d = DeviceSystem(s.params, s.offsets)
