@@ -399,8 +399,12 @@ static int force_pass(sphmhd_ctx* c, bool first) {
 // fit its slots: the remaining passes run (one wait each) or the lists grow,
 // and the force pass is redone from the kept previous accelerations.  After a
 // redo one extra speculative pass is launched for the next 16 evaluations.
-static int next_spec(const sphmhd_ctx* c) {
-    return c->spec_fixed > 0 ? c->spec_fixed : std::max(c->spec_passes + c->spec_margin, 1);
+// With fused density passes the first pass ends the iteration (passes:
+// false), so one is launched; the tangent's passes are launched one by one.
+static int next_spec(const sphmhd_ctx* c, bool passes = false) {
+    if (c->spec_fixed > 0) return c->spec_fixed;
+    if (c->d.dens_fused && !passes) return 1;
+    return std::max(c->spec_passes + c->spec_margin, 1);
 }
 
 static const int kPhaseTimer[kSegs] = {T_STAGE, T_NNPS, T_DENSITY, T_LISTS, T_FORCE};
@@ -919,6 +923,11 @@ int sphmhd_create(const sphmhd_params* params, int32_t n_instances, const int64_
         if (const char* g = std::getenv("SPHMHD_PRUNE_LISTS")) c->prune_mode = std::strcmp(g, "0") != 0;
         // SPHMHD_DENSITY_TRIM=0: the 3-D first density pass scans whole stencil rows (same results)
         if (const char* g = std::getenv("SPHMHD_DENSITY_TRIM")) d.dens_trim = std::strcmp(g, "0") != 0;
+        // later density passes inside the first launch in 2-D, as launches of
+        // their own in 3-D (there the fused passes gained nothing on the
+        // blast); SPHMHD_DENSITY_FUSED=1 / 0 chooses either way
+        d.dens_fused = p0.dim == 2;
+        if (const char* g = std::getenv("SPHMHD_DENSITY_FUSED")) d.dens_fused = std::strcmp(g, "0") != 0;
         d.prune_lists = c->prune_mode == 1;
         if (const char* g = std::getenv("SPHMHD_TUNE"))
             if (std::strcmp(g, "0") == 0) c->tune.state = 3;
@@ -1507,7 +1516,7 @@ static int t_eval(sphmhd_ctx* c) {
     if (rc) return rc;
     launch_t_eval_begin(d, g);
     launch_pack_density(d);
-    const int spec = next_spec(c);
+    const int spec = next_spec(c, true);
     int passes = 0;
     tbegin(c, T_DENSITY);
     for (; passes < spec; ++passes) {

@@ -98,6 +98,7 @@ struct Inst {
     alignas(128) unsigned long long hmax_next_bits;  // max h of still-unconverged particles after the running pass
     unsigned long long hmax_conv_bits;               // max h of converged particles (frozen for the evaluation)
     int32_t unconv;
+    int32_t passes;  // most passes a particle took (the passes iterated within one launch)
     // the evaluation's start: h_max, h_min and the particles' box, reduced as
     // maxima of kRed bit patterns (see red_values); red_stage accumulates
     // them in the fused stage kernel (zero between steps)
@@ -327,6 +328,7 @@ __device__ inline void eval_begin_body(Inst* inst, int ninst, Ctl* ctl, int rese
         if (reset_dt) s.dtp_bits = 0x7FF0000000000000ull;  // +inf
         s.iters = 0;
         s.unconv = 0;
+        s.passes = 0;
         s.done = 0;
         s.status = 0;
     }
@@ -1392,6 +1394,7 @@ struct Dev {
     int32_t nb_stride;      // neighbour-list capacity per particle
     int32_t pass;           // density pass index within the evaluation
     int32_t dens_trim = 1;  // 3-D first density pass trims its stencil rows to R h_i (same results)
+    int32_t dens_fused = 1;  // the first density pass iterates its unconverged particles to the end (2-D)
     bool prune_lists;       // list builds use the pruned kernel (some grid was split)
     hipStream_t stream;
     // device arrays

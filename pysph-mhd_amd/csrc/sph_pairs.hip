@@ -58,7 +58,239 @@ __device__ __forceinline__ int spread_lanes(int64_t count, int64_t n) {
     return G;
 }
 
-template <int K, int DIM, bool SPREAD, bool TRIM = false>
+// DensityIteration.post_loop (equations.py:54-86) from one pass's sums:
+// writes h, omega and the converged flag; returns whether the particle stays
+// unconverged (hout = its next h, else its final h)
+template <int K, int DIM>
+__device__ __forceinline__ bool density_newton(const Fields& F, const Inst& s, int32_t i, double hi, double mi,
+                                               double sw, double sg, double& hout) {
+    const double h1 = 1.0 / hi;
+    const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
+    const double rho_sum = sw * fac;
+    const double dwdh = -h1 * (fac * sg);
+    const double hi0 = F.f[SPHMHD_F_H0][i];
+    const double rhoi = mi / ipow_dim(hi / s.hfact, DIM);
+    const double dhdrhoi = -hi / (DIM * rhoi);
+    const double omegai = 1.0 - dhdrhoi * dwdh;
+    F.f[SPHMHD_F_OMEGA][i] = 1.0 / omegai;
+    const double func = rhoi - rho_sum;
+    const double dfdh1 = omegai > 0.0 ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + 2.220446049250313e-16);
+    double hnew = hi - func * dfdh1;
+    if (hnew > 1.2 * hi) hnew = 1.2 * hi;
+    else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
+    const double diff = fabs(hnew - hi) / hi0;
+    const bool fin = ((diff < 1.0e-4) && (omegai > 0.0) && (hi > 0.0)) || !s.density_iterate;
+    if (!fin) {
+        hout = hnew;
+        F.f[SPHMHD_F_CONVERGED][i] = 0.0;
+    } else {
+        hout = s.hfact * iroot_dim(mi / rho_sum, DIM);
+        F.f[SPHMHD_F_CONVERGED][i] = 1.0;
+    }
+    F.f[SPHMHD_F_H][i] = hout;
+    return !fin;
+}
+
+// Lane g of G lanes' part of one particle's density sums at h: every G-th
+// candidate of each stencil range.  The stencil follows this h alone (the
+// pass's stencil, sized for the h_max its pass began with, may be smaller by
+// now), and so does the choice between the ranges' image shifts and a
+// minimum image per candidate, which give the same r^2 bits.
+template <int K, int DIM>
+__device__ __forceinline__ void density_lane_sums(const Inst& s, const double4* __restrict__ dpack, int64_t key,
+                                                  const int32_t* __restrict__ cstart,
+                                                  const int32_t* __restrict__ cend, const double4& me, double hi,
+                                                  int g, int G, double& sw, double& sg) {
+    const double R = kernel_radius(K);
+    const double h1 = 1.0 / hi;
+    const double Rh = R * hi;
+    const double Rh2 = Rh * Rh;
+    const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+    const double H0 = s.halfL[0], H1 = s.halfL[1], H2 = s.halfL[2];
+    const int per = s.periodic;
+    int sr[3] = {0, 0, 0};
+    bool shift = true;
+    for (int a = 0; a < DIM; ++a) {
+        const double q = ceil(Rh / s.ceff[a]);
+        sr[a] = q < 1.0 ? 1 : (q < 1.0e6 ? (int)q : 1000000);
+        if (per && !((int64_t)s.n[a] > 2 * ((int64_t)sr[a] + 1))) shift = false;
+    }
+    auto scan = [&](auto shift_c) {
+        constexpr bool kShift = decltype(shift_c)::value;
+        auto body = [&](int, const double4& o, double sx, double sy, double sz) {
+            const double dx = kShift ? (me.x - o.x) + sx : min_image(me.x - o.x, L0, H0, per);
+            const double dy = kShift ? (me.y - o.y) + sy : min_image(me.y - o.y, L1, H1, per);
+            const double dz = DIM == 3 ? (kShift ? (me.z - o.z) + sz : min_image(me.z - o.z, L2, H2, per)) : 0.0;
+            const double r2 = r2_exact(dx, dy, dz);
+            if (r2 < Rh2) {
+                const double q = sqrt_r2(r2) * h1;
+                double w, dw;
+                shape<K, DIM>(q, w, dw);
+                sw += o.w * w;
+                sg += o.w * ((double)DIM * w + q * dw);
+            }
+        };
+        auto rf = [&](int a, int b, double sx, double sy, double sz) {
+            scan_range4_strided(dpack, a, b, g, G, [&](int j, const double4& o) { body(j, o, sx, sy, sz); });
+        };
+        for_each_row_range<DIM>(s, key, cstart, cend, rf, sr);
+    };
+    if (shift) scan(std::true_type{});
+    else scan(std::false_type{});
+}
+
+// Per instance, what the block's particles that finished in one round leave
+// for the group control: the largest h of those converged (hc) and of those
+// left unconverged (hu), how many were left unconverged (uc) and the most
+// passes one took (np).  Waves reduce by shuffles and thread 0 issues one
+// atomic per quantity and instance run; a wave holding several instances
+// falls back to lane atomics.  Every thread of the block calls it.
+__device__ __forceinline__ void reduce_finished(Inst* inst, int ki, bool fin, unsigned long long hc,
+                                                unsigned long long hu, int uc, int np) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    constexpr int NW = kBlock / kWave;
+    const int nw = blockDim.x / kWave;
+    __shared__ int rk[NW], ruc[NW], rnp[NW];
+    __shared__ unsigned long long rhc[NW], rhu[NW];
+    if (!fin) {
+        hc = hu = 0ull;
+        uc = np = 0;
+    }
+    const unsigned long long fm = __ballot(fin);
+    const int k0 = fm ? __shfl(ki, __ffsll((long long)fm) - 1) : -1;
+    const bool uniform = __all(!fin || ki == k0);
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(hc, off);
+            hc = o > hc ? o : hc;
+            o = __shfl_xor(hu, off);
+            hu = o > hu ? o : hu;
+            uc += __shfl_xor(uc, off);
+            const int op = __shfl_xor(np, off);
+            np = op > np ? op : np;
+        }
+    } else if (fin) {
+        if (hc) atomicMax(&inst[ki].hmax_conv_bits, hc);
+        if (hu) atomicMax(&inst[ki].hmax_next_bits, hu);
+        if (uc) atomicAdd(&inst[ki].unconv, uc);
+        if (np > 1) atomicMax(&inst[ki].passes, np);
+    }
+    if (lane == 0) {
+        rk[w] = uniform ? k0 : -1;
+        rhc[w] = hc;
+        rhu[w] = hu;
+        ruc[w] = uc;
+        rnp[w] = np;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < nw;) {
+            const int k = rk[q];
+            unsigned long long c = rhc[q], u = rhu[q];
+            int cu = ruc[q], mp = rnp[q];
+            int r = q + 1;
+            for (; r < nw && rk[r] == k; ++r) {
+                c = rhc[r] > c ? rhc[r] : c;
+                u = rhu[r] > u ? rhu[r] : u;
+                cu += ruc[r];
+                mp = rnp[r] > mp ? rnp[r] : mp;
+            }
+            if (k >= 0) {
+                if (c) atomicMax(&inst[k].hmax_conv_bits, c);
+                if (u) atomicMax(&inst[k].hmax_next_bits, u);
+                if (cu) atomicAdd(&inst[k].unconv, cu);
+                if (mp > 1) atomicMax(&inst[k].passes, mp);
+            }
+            q = r;
+        }
+    }
+    __syncthreads();
+}
+
+// The passes after the first within the first pass's workgroup.  A
+// particle's Newton iteration reads only its own h and its neighbours'
+// positions and masses, which no other particle's iteration changes, so the
+// block iterates each of its particles still unconverged to convergence (or
+// to its instance's max_iterations) at once: those particles are listed in
+// LDS and each pass gives each a fixed number of lanes (their sums combined
+// by an xor butterfly), as many rounds as the list needs.  A particle's bits
+// then depend on nothing but that lane count: not on the launch shape, the
+// block or the other instances of an ensemble.  The iteration ends within the
+// launch; the group control takes the passes from the largest count a
+// particle needed (at least min_iterations, as the group iteration runs).
+// Lanes per particle: in 2-D 4 (the rotor's density 65 -> 61 us per step
+// against 8; 16 slower); in 3-D 2, where the rows are many and short.
+constexpr int kFuseLanes2 = 4, kFuseLanes3 = 2;
+
+template <int K, int DIM>
+__device__ __forceinline__ void density_fused_passes(const Fields& F, const double4* __restrict__ dpack,
+                                                     const int32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ skey,
+                                                     const int32_t* __restrict__ cstart,
+                                                     const int32_t* __restrict__ cend, Inst* inst,
+                                                     const int32_t* __restrict__ inst_of, int ki, int64_t t,
+                                                     bool live, bool unconv, double hout) {
+    __shared__ int32_t lt[2][kBlock];
+    __shared__ double lh[2][kBlock];
+    __shared__ int lcnt[2];
+    if (threadIdx.x == 0) lcnt[0] = 0;
+    __syncthreads();
+    // the first pass's outcome: particles that go on are listed, the rest finish
+    const bool go = unconv && inst[ki].max_iter > 1;
+    const bool left = unconv && !go;
+    if (go) {
+        const int slot = atomicAdd(&lcnt[0], 1);
+        lt[0][slot] = (int32_t)t;
+        lh[0][slot] = hout;
+    }
+    reduce_finished(inst, ki, live && !go, unconv ? 0ull : dbits(hout), left ? dbits(hout) : 0ull, left ? 1 : 0, 1);
+    constexpr int kFuseLanes = DIM == 2 ? kFuseLanes2 : kFuseLanes3;
+    const int B = blockDim.x;
+    int cur = 0;
+    for (int p = 2;; ++p) {
+        const int cnt = lcnt[cur];
+        if (cnt == 0) break;
+        if (threadIdx.x == 0) lcnt[cur ^ 1] = 0;
+        __syncthreads();
+        for (int base = 0; base < cnt * kFuseLanes; base += B) {
+            const int idx = base + threadIdx.x;
+            const int item = idx / kFuseLanes, g = idx & (kFuseLanes - 1);
+            const bool v = item < cnt;
+            const int64_t tt = v ? (int64_t)lt[cur][item] : 0;
+            const double h = v ? lh[cur][item] : 0.0;
+            const int kk = v ? inst_of[tt] : 0;
+            const Inst& s = inst[kk];
+            double sw = 0.0, sg = 0.0;
+            double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (v) {
+                me = dpack[tt];
+                density_lane_sums<K, DIM>(s, dpack, (int64_t)skey[tt], cstart, cend, me, h, g, kFuseLanes, sw, sg);
+            }
+            for (int off = 1; off < kFuseLanes; off <<= 1) {
+                sw += __shfl_xor(sw, off);
+                sg += __shfl_xor(sg, off);
+            }
+            const bool lead = v && g == 0;
+            bool more = false, stays = false;
+            double ho = h;
+            if (lead) {
+                stays = density_newton<K, DIM>(F, s, perm[tt], h, me.w, sw, sg, ho);
+                more = stays && p < s.max_iter;
+                if (more) {
+                    const int slot = atomicAdd(&lcnt[cur ^ 1], 1);
+                    lt[cur ^ 1][slot] = (int32_t)tt;
+                    lh[cur ^ 1][slot] = ho;
+                }
+            }
+            const bool ends = stays && !more;
+            reduce_finished(inst, kk, lead && !more, stays ? 0ull : dbits(ho), ends ? dbits(ho) : 0ull,
+                            ends ? 1 : 0, p);
+        }
+        cur ^= 1;
+    }
+}
+
+template <int K, int DIM, bool SPREAD, bool TRIM = false, bool FUSE = false>
 __device__ __forceinline__ void density_block(Fields F, const double4* __restrict__ dpack,
                                               const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                               const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
@@ -156,35 +388,11 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
         if (g != 0) live = active = false;
     }
 
-    if (active) {
-        const double h1 = 1.0 / hi;
-        const double fac = DIM == 2 ? kernel_sigma(K, 2) * h1 * h1 : kernel_sigma(K, 3) * h1 * h1 * h1;
-        const double rho_sum = sw * fac;
-        const double dwdh = -h1 * (fac * sg);
-
-        // DensityIteration.post_loop (equations.py:54-86)
-        const double mi = me.w;
-        const double hi0 = F.f[SPHMHD_F_H0][i];
-        const double rhoi = mi / ipow_dim(hi / s.hfact, DIM);
-        const double dhdrhoi = -hi / (DIM * rhoi);
-        const double omegai = 1.0 - dhdrhoi * dwdh;
-        F.f[SPHMHD_F_OMEGA][i] = 1.0 / omegai;
-        const double func = rhoi - rho_sum;
-        const double dfdh1 = omegai > 0.0 ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + 2.220446049250313e-16);
-        double hnew = hi - func * dfdh1;
-        if (hnew > 1.2 * hi) hnew = 1.2 * hi;
-        else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
-        const double diff = fabs(hnew - hi) / hi0;
-        const bool fin = ((diff < 1.0e-4) && (omegai > 0.0) && (hi > 0.0)) || !s.density_iterate;
-        if (!fin) {
-            hout = hnew;
-            unconv = true;
-            F.f[SPHMHD_F_CONVERGED][i] = 0.0;
-        } else {
-            hout = s.hfact * iroot_dim(mi / rho_sum, DIM);
-            F.f[SPHMHD_F_CONVERGED][i] = 1.0;
-        }
-        F.f[SPHMHD_F_H][i] = hout;
+    if (active) unconv = density_newton<K, DIM>(F, s, i, hi, me.w, sw, sg, hout);
+    if constexpr (FUSE) {
+        density_fused_passes<K, DIM>(F, dpack, perm, skey, cstart, cend, inst, inst_of, ki, (int64_t)t, live,
+                                     unconv, hout);
+        return;
     }
 
     // Still-unconverged particles go to the next pass's list, and per instance
@@ -265,7 +473,9 @@ __device__ __forceinline__ void density_block(Fields F, const double4* __restric
 // Also clears the counter of the active list the next pass will write
 // (next_list), so passes can be launched back to back without the host.
 // One workgroup.
-__device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) {
+// fused: the pass iterated every particle to the end (k_density FUSE), and
+// the passes are the most any particle needed, at least min_iterations.
+__device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list, int fused = 0) {
     __shared__ int alldone, spread;
     if (threadIdx.x == 0) {
         alldone = 1;
@@ -279,9 +489,15 @@ __device__ void g0_control_body(Inst* inst, int ninst, Ctl* ctl, int next_list) 
             // unconv and the two h maxima were reduced by the pass's atomics
             const int unconv = ld_dev(&s.unconv);
             const unsigned long long hconv = ld_dev(&s.hmax_conv_bits), hnext = ld_dev(&s.hmax_next_bits);
-            s.iters += 1;
+            if (fused) {
+                const int np = max(ld_dev(&s.passes), 1);
+                s.iters = !s.density_iterate ? 1 : min(max(s.min_iter, np), s.max_iter);
+                s.passes = 0;
+            } else {
+                s.iters += 1;
+            }
             bool conv = unconv == 0;
-            bool stop = !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
+            bool stop = fused || !s.density_iterate || (s.iters >= s.min_iter && conv) || s.iters >= s.max_iter;
             if (s.density_iterate && !(s.iters >= s.min_iter && conv) && s.iters >= s.max_iter) s.status |= 1;
             s.unconv = 0;
             s.spread = spread_lanes(unconv, s.off1 - s.off0);
@@ -309,8 +525,10 @@ __global__ void k_g0_control(Inst* inst, int ninst, Ctl* ctl, int next_list) {
 
 // one density pass; its last workgroup runs the group control.  LATER: a
 // pass after the first (act_in set), with the spread lanes; the first pass
-// has an instantiation of its own that carries none of that code.
-template <int K, int DIM, bool LATER>
+// has an instantiation of its own that carries none of that code.  FUSE: a
+// first pass whose workgroups go on to iterate their unconverged particles to
+// the end (density_fused_passes), so that no later pass is needed.
+template <int K, int DIM, bool LATER, bool FUSE = false>
 __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, const double4* __restrict__ dpack,
                                                     const int32_t* __restrict__ perm, const uint32_t* __restrict__ skey,
                                                     const int32_t* __restrict__ cstart, const int32_t* __restrict__ cend,
@@ -324,8 +542,8 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
     const int G = LATER ? max(ctl->spread, 1) : 1;
     auto run = [&](int64_t tile) {
         if (!LATER)
-            density_block<K, DIM, false, true>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
-                                               act_out, cnt_out, tile, 1, trim);
+            density_block<K, DIM, false, true, FUSE>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in,
+                                                     cnt_in, act_out, cnt_out, tile, 1, trim);
         else if (G == 1)
             density_block<K, DIM, false>(F, dpack, perm, skey, cstart, cend, inst, inst_of, n, act_in, cnt_in,
                                          act_out, cnt_out, tile, 1);
@@ -347,7 +565,7 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
         run(xcd_tile((n + B - 1) / B));
     }
     if (last_to_arrive(arrive)) {
-        g0_control_body(inst, ninst, ctl, next_list);
+        g0_control_body(inst, ninst, ctl, next_list, FUSE ? 1 : 0);
         if (queue) reset_queues(queue);
     }
 }
@@ -979,9 +1197,10 @@ unsigned persistent_grid(const void* fn, int B, size_t lds, int64_t tiles) {
     default: if (DIMV == 2) CALL(SPHMHD_SUPER_GAUSSIAN, 2) else CALL(SPHMHD_SUPER_GAUSSIAN, 3) break; \
     }
 
-// pass 0 scans every particle and writes list 0; pass p > 0 reads list
-// (p - 1) % 2 and writes list p % 2; the pass ends with the group control
-// (the next pass writes list (p + 1) % 2)
+// pass 0 scans every particle and writes list 0 (with dens_fused it iterates
+// them to the end and lists none); pass p > 0 reads list (p - 1) % 2 and
+// writes list p % 2; the pass ends with the group control (the next pass
+// writes list (p + 1) % 2)
 void launch_density(Dev& d) {
     if (d.n == 0) return;
     const int p = d.pass;
@@ -991,13 +1210,16 @@ void launch_density(Dev& d) {
     int32_t* cout = act_count(d.ctl, p & 1);
     const int B = d.blk[LC_DENSITY];
     int32_t* q = d.persist[LC_DENSITY] ? d.queue : nullptr;
-#define CALL_DENS_L(KK, DD, LL)                                                                             \
-    { const unsigned g = q ? persistent_grid((const void*)k_density<KK, DD, LL>, B, 0, nblk(d.n, B))        \
+#define CALL_DENS_L(KK, DD, LL, FF)                                                                         \
+    { const unsigned g = q ? persistent_grid((const void*)k_density<KK, DD, LL, FF>, B, 0, nblk(d.n, B))    \
                            : xcd_grid(nblk(d.n, B));                                                        \
-      hipLaunchKernelGGL((k_density<KK, DD, LL>), dim3(g), dim3(B), 0, d.stream, d.fields,                  \
+      hipLaunchKernelGGL((k_density<KK, DD, LL, FF>), dim3(g), dim3(B), 0, d.stream, d.fields,              \
                          (const double4*)d.dpack, d.perm, d.keys, d.cstart, d.cend, d.inst, d.inst_of, d.n, \
                          ain, cin, aout, cout, d.ctl, d.ninst, (p + 1) & 1, d.arrive, q, d.dens_trim); }
-#define CALL_DENS(KK, DD) { if (p == 0) CALL_DENS_L(KK, DD, false) else CALL_DENS_L(KK, DD, true) }
+#define CALL_DENS(KK, DD)                                                                                   \
+    { if (p > 0) CALL_DENS_L(KK, DD, true, false)                                                           \
+      else if (d.dens_fused) CALL_DENS_L(KK, DD, false, true)                                               \
+      else CALL_DENS_L(KK, DD, false, false) }
     SPH_DISPATCH(d.kernel, d.dim, CALL_DENS)
 #undef CALL_DENS
 #undef CALL_DENS_L

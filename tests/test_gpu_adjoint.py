@@ -131,9 +131,12 @@ def test_adjoint_recomputation_redone_with_waits_equals_default(gpu, monkeypatch
     launched raises the abort flag and the segment is recomputed with a wait
     after each evaluation.  With one speculative pass (SPHMHD_SPEC=1) every
     evaluation aborts, so every segment takes that second path: the value and
-    the gradient equal the default run's bit for bit."""
+    the gradient equal the default run's bit for bit.  The density passes run
+    as launches of their own here (SPHMHD_DENSITY_FUSED=0): fused into the
+    first launch, one pass never falls short."""
     from sphmhd.adjoint import energy_gradient
 
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", "0")
     s = _small()
     _print("adjoint recomputation with waits: alfven 10x17, 6 steps, segments of 3")
     value, grads, gpar = energy_gradient(s, 6, every=3)

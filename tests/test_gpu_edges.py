@@ -83,7 +83,10 @@ def test_ragged_ensemble_with_empty_instance(gpu):
         compare(dev, orc, STATE + RATES, TOL, f"step {k + 1}")
     td, dtd = dev.clock()
     to, dto = orc.clock()
-    assert np.array_equal(td, to)
+    assert td[1] == to[1]  # the empty instance's clock, exactly
+    # the others sum steps whose dt follows h, which the density sums give
+    # to the last place of their own summation order
+    assert np.allclose(td, to, rtol=1e-12, atol=0)
     assert np.allclose(dtd, dto, rtol=1e-12, atol=0)
 
 

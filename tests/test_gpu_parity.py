@@ -121,21 +121,24 @@ def test_single_evaluation(gpu):
     assert np.allclose(dev.pending_dt(), orc.pending_dt(), rtol=1e-12, atol=0)
 
 
-def test_warm_start(gpu):
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_warm_start(gpu, monkeypatch, fused):
     """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
     s = _ot(64)
     dev, orc = _pair(s)
     dev.warm_start()
     orc.warm_start()
     compare(dev, orc, STATE + RATES, TOL, "warm start")
-    # The first evaluation starts from the lattice h and needs more density
-    # passes than the two launched ahead of the force pass, so the force pass
-    # was redone from the kept previous accelerations: the comparison above
-    # covers that path.
-    assert dev.stats()["force_redos"] >= 1
+    if fused == "0":
+        # The first evaluation starts from the lattice h and needs more
+        # density passes than the two launched ahead of the force pass, so
+        # the force pass was redone from the kept previous accelerations: the
+        # comparison above covers that path.
+        assert dev.stats()["force_redos"] >= 1
     td, dtd = dev.clock()
     to, dto = orc.clock()
     assert abs(dtd[0] - dto[0]) <= 1e-12 * dto[0]
@@ -210,17 +213,20 @@ def test_graph_replays_equal_plain_launches(gpu, monkeypatch):
     assert tm["force_launches"] >= 10 and tm["force_ms"] > 0.0
 
 
-@pytest.mark.parametrize("spec", ["1", "2"])
-def test_steps_enqueued_ahead_survive_redos(gpu, monkeypatch, spec):
-    """Graph steps are enqueued one ahead of the host's wait.  With one
-    density pass launched ahead of the force pass (below the two passes every
-    evaluation takes) every step ends needing a redo: its end raises the abort
-    flag, the step enqueued after it exits at entry, and the host finishes the
-    step and enqueues the next again.  The result equals plain launches bit
-    for bit, and each step is replayed from the graph exactly once."""
+@pytest.mark.parametrize("spec,fused", [("1", "0"), ("2", "0"), ("1", "1")])
+def test_steps_enqueued_ahead_survive_redos(gpu, monkeypatch, spec, fused):
+    """Graph steps are enqueued one ahead of the host's wait.  With the
+    density passes as launches of their own (SPHMHD_DENSITY_FUSED=0) and one
+    launched ahead of the force pass (below the two passes every evaluation
+    takes) every step ends needing a redo: its end raises the abort flag, the
+    step enqueued after it exits at entry, and the host finishes the step and
+    enqueues the next again.  The result equals plain launches bit for bit,
+    and each step is replayed from the graph exactly once.  With the passes
+    fused into the first launch one launch ends every iteration: no redo."""
     from sphmhd.device import DeviceSystem
 
     s = _ot(48)
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
     monkeypatch.setenv("SPHMHD_GRAPHS", "0")
     monkeypatch.delenv("SPHMHD_SPEC", raising=False)
     plain = DeviceSystem(s.params, s.offsets)
@@ -241,8 +247,10 @@ def test_steps_enqueued_ahead_survive_redos(gpu, monkeypatch, spec):
         assert np.array_equal(plain.clock()[1], ahead.clock()[1])
     st = ahead.stats()
     assert st["graph_replays"] == 10
-    if spec == "1":
+    if spec == "1" and fused == "0":
         assert st["force_redos"] - redos0 == 12  # every step was finished after its abort
+    if fused == "1":
+        assert st["force_redos"] == redos0
     assert st["evals"] == plain.stats()["evals"]
 
 
@@ -263,13 +271,19 @@ def test_kernels_steps(gpu, kernel):
     compare(dev, orc, STATE + RATES, TOL, kernel)
 
 
-def test_blast_3d_steps(gpu):
-    """Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_blast_3d_steps(gpu, monkeypatch, fused):
+    """In 3-D the later density passes run as launches of their own by
+    default; SPHMHD_DENSITY_FUSED=1 iterates them inside the first launch, as
+    in 2-D.  Either way the warm start and three steps match the oracle.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
     v_w|B|/(rho h)), since they are round-off where B is uniform; every
     other field against its own norm."""
     from sphmhd.cases.workloads import blast_3d, jittered
 
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
     s = jittered(blast_3d(16), 0.05, seed=2)
     dev, orc = _pair(s)
     dev.warm_start()
@@ -366,17 +380,20 @@ def test_ensemble_per_instance(gpu):
     assert np.allclose(td, to, rtol=1e-12, atol=0)
     assert len(set(np.round(dto, 12))) > 1  # instances really take different steps
 
-def test_spread_lanes_per_instance_equal_alone(gpu):
-    """Later density passes spread a particle over G lanes, G chosen per
-    instance from its own unconverged count and size.  Here instance 0
-    starts from h 25% too large (every particle stays unconverged after the
-    first pass: G = 1 in the second) beside an instance whose h is already
-    converged (taken from a warm start of it alone: nothing stays
-    unconverged, G = 64); both iterate at least twice, so the second pass of
-    the first evaluation maps 64 lanes per particle and instance 0 uses one
-    of them.  Each instance's fields after the warm start and two steps
-    equal, bit for bit, those of the same instance run alone, and both match
-    the oracle to 1e-10.
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_spread_lanes_per_instance_equal_alone(gpu, monkeypatch, fused):
+    """Later density passes launched on their own (SPHMHD_DENSITY_FUSED=0)
+    spread a particle over G lanes, G chosen per instance from its own
+    unconverged count and size.  Here instance 0 starts from h 25% too large
+    (every particle stays unconverged after the first pass: G = 1 in the
+    second) beside an instance whose h is already converged (taken from a
+    warm start of it alone: nothing stays unconverged, G = 64); both iterate
+    at least twice, so the second pass of the first evaluation maps 64 lanes
+    per particle and instance 0 uses one of them.  Fused into the first
+    launch (1), a workgroup iterates its unconverged particles with a fixed
+    lane count whatever its neighbours in the block.  Either way each
+    instance's fields after the warm start and two steps equal, bit for bit,
+    those of the same instance run alone, and both match the oracle to 1e-10.
 
     Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
     1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
@@ -385,6 +402,7 @@ def test_spread_lanes_per_instance_equal_alone(gpu):
     from sphmhd.cases.workloads import ensemble_ot, jittered
     from sphmhd.device import DeviceSystem
 
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
     s = jittered(ensemble_ot(2, sizes=[48, 32]), 0.1, seed=5)
     f = init_fields(s)
     a0, a1, a2 = (int(v) for v in s.offsets)
@@ -415,6 +433,95 @@ def test_spread_lanes_per_instance_equal_alone(gpu):
         passes.append(one.stats()["density_iterations"])
         one.close()
     assert passes[0] > passes[1], passes  # the perturbed instance iterates longer
+
+
+class _Held:
+    """Fields read back from a closed context, for compare()."""
+
+    def __init__(self, fields):
+        self.fields = fields
+
+    def get(self, name):
+        return self.fields[name]
+
+
+def _params_with(p0, **values):
+    import ctypes
+
+    p = type(p0)()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(p0), ctypes.sizeof(p))
+    for k, v in values.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_density_iteration_stops_at_max_iterations(gpu, monkeypatch, fused):
+    """With every other particle's h 30% too large and max_iterations = 3,
+    the Newton iteration (clipped to a 20% change per pass) cannot converge
+    for those: the group stops after the third pass with them unconverged,
+    their h the third pass's Newton step, the others converged, and the
+    evaluation returns the no-convergence warning.  The
+    fields then equal the oracle's to 1e-10, with the later passes fused into
+    the first launch (SPHMHD_DENSITY_FUSED=1) and launched on their own (0).
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
+    from pyoracle import OracleSystem
+    from sphmhd._abi import WNOCONV
+    from sphmhd.device import DeviceSystem
+
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
+    s = _ot(32)
+    prm = [_params_with(s.params[0], max_iterations=3)]
+    f = init_fields(s)
+    f["h"] = f["h"].copy()
+    f["h"][::2] *= 1.3
+    dev = DeviceSystem(prm, s.offsets)
+    orc = OracleSystem(prm, s.offsets)
+    for sysm in (dev, orc):
+        sysm.set_fields(f)
+        sysm.initialize()  # h0, the scale of the convergence test
+    assert orc.eval() == WNOCONV
+    assert dev.eval() == WNOCONV
+    assert int(orc.last_iterations()[0]) == 3
+    assert dev.stats()["last_iterations_max"] == 3
+    conv = dev.get("converged")
+    assert np.array_equal(conv, orc.get("converged"))
+    assert 0 < int((conv == 0.0).sum()) < conv.size  # some converged, some not
+    compare(dev, orc, STATE + RATES, TOL, f"max_iterations fused={fused}")
+
+
+def test_fused_density_passes_equal_separate_passes(gpu, monkeypatch):
+    """The density iteration with its later passes inside the first launch
+    (each workgroup iterating its own unconverged particles, 4 lanes each in
+    2-D) and with the later passes launched on their own (G lanes per
+    particle from the instance's count) take the same passes per evaluation,
+    and the fields after the warm start and three steps of a jittered 48^2
+    Orszag-Tang vortex started from h 25% too large agree to 1e-12 (the two
+    split a particle's sums over different lanes: the bits may differ in the
+    last place)."""
+    from sphmhd.device import DeviceSystem
+
+    s = _ot(48)
+    f = init_fields(s)
+    f["h"] = f["h"] * 1.25
+    runs = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
+        dev = DeviceSystem(s.params, s.offsets)
+        dev.set_fields(f)
+        dev.warm_start()
+        dev.step(3)
+        st = dev.stats()
+        runs.append(({nm: dev.get(nm) for nm in STATE + RATES}, st["density_iterations"], st["evals"]))
+        dev.close()
+    (a, ia, ea), (b, ib, eb) = runs
+    assert (ia, ea) == (ib, eb)
+    assert ia > 2 * ea  # the perturbed start iterates longer than min_iterations
+    compare(_Held(a), _Held(b), STATE + RATES, 1e-12, "fused vs separate density passes")
 
 
 def test_orszag_tang_128_100_steps(gpu):
