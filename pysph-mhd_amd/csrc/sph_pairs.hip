@@ -218,8 +218,9 @@ __device__ __forceinline__ void reduce_finished(Inst* inst, int ki, bool fin, un
 // block or the other instances of an ensemble.  The iteration ends within the
 // launch; the group control takes the passes from the largest count a
 // particle needed (at least min_iterations, as the group iteration runs).
-// Lanes per particle: in 2-D 4 (the rotor's density 65 -> 61 us per step
-// against 8; 16 slower); in 3-D 2, where the rows are many and short.
+// Lanes per particle: 4 in 2-D (the 512^2 rotor's density 61 us per step,
+// against 65 with 8 lanes and 68 with 16); 2 in 3-D, where the stencil rows
+// are many and short.
 constexpr int kFuseLanes2 = 4, kFuseLanes3 = 2;
 
 template <int K, int DIM>
