@@ -575,13 +575,19 @@ __global__ void __launch_bounds__(kBlock, DIM == 3 ? 4 : 1) k_density(Fields F, 
 // G1 + G2 + G4 and the force pack
 // ---------------------------------------------------------------------------
 
+// A particle's inputs to k_prep_force, all loaded before its first store: a
+// load after the property stores could alias them for the compiler, which
+// would then wait for another round trip to memory.
+struct PrepIn {
+    double h, m, epred, bxp, byp, bzp, x, y, z, au, av, aw, upred, vpred, wpred, omega, alpha, psip;
+};
+
 template <int DIM>
-__device__ __forceinline__ void write_force_record(double2* stage, const Fields& F, int32_t i,
-                                                   double x, double y, double z, double h, double m, double rho,
-                                                   double p, double e, double bx, double by, double bz, double vwave,
-                                                   double chf, double au, double av, double aw) {
+__device__ __forceinline__ void write_force_record(double2* stage, const PrepIn& in, double rho, double p, double bx,
+                                                   double by, double bz, double vwave, double chf) {
     using R = FRec<DIM>;
-    double* const* f = F.f;
+    const double x = in.x, y = in.y, z = in.z, h = in.h, m = in.m, e = in.epred;
+    const double au = in.au, av = in.av, aw = in.aw;
     double r[R::N];
     r[R::X] = x;
     r[R::Y] = y;
@@ -590,18 +596,18 @@ __device__ __forceinline__ void write_force_record(double2* stage, const Fields&
     r[R::M] = m;
     r[R::RHO] = rho;
     r[R::E] = e;
-    r[R::U] = f[SPHMHD_F_UPRED][i];
-    r[R::V] = f[SPHMHD_F_VPRED][i];
-    r[R::W] = f[SPHMHD_F_WPRED][i];
-    r[R::OM] = f[SPHMHD_F_OMEGA][i];
+    r[R::U] = in.upred;
+    r[R::V] = in.vpred;
+    r[R::W] = in.wpred;
+    r[R::OM] = in.omega;
     r[R::BX] = bx;
     r[R::BY] = by;
     r[R::BZ] = bz;
-    r[R::AL] = f[SPHMHD_F_ALPHA1][i];
+    r[R::AL] = in.alpha;
     r[R::AU] = au;
     r[R::AV] = av;
     if constexpr (DIM == 3) r[R::AW] = aw;
-    r[R::PSICH] = f[SPHMHD_F_PSIPRED][i] * (chf * vwave);
+    r[R::PSICH] = in.psip * (chf * vwave);
     r[R::VW] = vwave;
 #pragma unroll
     for (int q = 0; q < R::UNITS; ++q) stage[q] = make_double2(r[2 * q], r[2 * q + 1]);
@@ -617,18 +623,44 @@ __device__ __forceinline__ void prep_force_one(Fields F, const int32_t* __restri
     const int32_t i = perm[t];
     const int dim = s.dim;
     double** f = F.f;
-    const double h = f[SPHMHD_F_H][i];
-    const double m = f[SPHMHD_F_M][i];
+    // every input first (the previous accelerations from their rows on the
+    // first prep of an evaluation, which the force pass then overwrites, or
+    // back from aprev on a redone pass)
+    PrepIn in;
+    in.h = f[SPHMHD_F_H][i];
+    in.m = f[SPHMHD_F_M][i];
+    in.epred = f[SPHMHD_F_EPRED][i];
+    in.bxp = f[SPHMHD_F_BEVOLXPRED][i];
+    in.byp = f[SPHMHD_F_BEVOLYPRED][i];
+    in.bzp = f[SPHMHD_F_BEVOLZPRED][i];
+    in.x = f[SPHMHD_F_X][i];
+    in.y = f[SPHMHD_F_Y][i];
+    in.z = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
+    const double* asrc[3] = {f[SPHMHD_F_AU], f[SPHMHD_F_AV], f[SPHMHD_F_AW]};
+    if (!keep_prev) {
+        asrc[0] = aprev;
+        asrc[1] = aprev + n;
+        asrc[2] = aprev + 2 * n;
+    }
+    in.au = asrc[0][i];
+    in.av = asrc[1][i];
+    in.aw = asrc[2][i];
+    in.upred = f[SPHMHD_F_UPRED][i];
+    in.vpred = f[SPHMHD_F_VPRED][i];
+    in.wpred = f[SPHMHD_F_WPRED][i];
+    in.omega = f[SPHMHD_F_OMEGA][i];
+    in.alpha = f[SPHMHD_F_ALPHA1][i];
+    in.psip = f[SPHMHD_F_PSIPRED][i];
+    const double h = in.h, m = in.m, epred = in.epred;
     // DensityFromSmoothingLength (equations.py:118)
     const double rho = m / ipow_dim(h / s.hfact, dim);
     // IdealGasEOS (equations.py:105-106)
-    const double epred = f[SPHMHD_F_EPRED][i];
     const double p = s.gamma1 * rho * epred;
     const double cs = sqrt(s.gamma * p / rho);
     // MagneticStateReconstruction (equations.py:477-485)
-    const double bx = f[SPHMHD_F_BEVOLXPRED][i] * rho;
-    const double by = f[SPHMHD_F_BEVOLYPRED][i] * rho;
-    const double bz = f[SPHMHD_F_BEVOLZPRED][i] * rho;
+    const double bx = in.bxp * rho;
+    const double by = in.byp * rho;
+    const double bz = in.bzp * rho;
     f[SPHMHD_F_RHO][i] = rho;
     f[SPHMHD_F_P][i] = p;
     f[SPHMHD_F_CS][i] = cs;
@@ -641,27 +673,14 @@ __device__ __forceinline__ void prep_force_one(Fields F, const int32_t* __restri
 
     const double b2 = bx * bx + by * by + bz * bz;
     const double vwave = sqrt(cs * cs + b2 / (s.mu0 * rho));
-    const double x = f[SPHMHD_F_X][i], y = f[SPHMHD_F_Y][i], z = dim == 3 ? f[SPHMHD_F_Z][i] : 0.0;
-    xh[t] = make_double4(x, y, z, h);
-    // the previous accelerations (the force pass overwrites them): kept on the
-    // first prep of an evaluation, read back on a redone pass
-    double au, av, aw;
+    xh[t] = make_double4(in.x, in.y, in.z, h);
     if (keep_prev) {
-        au = f[SPHMHD_F_AU][i];
-        av = f[SPHMHD_F_AV][i];
-        aw = f[SPHMHD_F_AW][i];
-        aprev[i] = au;
-        aprev[n + i] = av;
-        aprev[2 * n + i] = aw;
-    } else {
-        au = aprev[i];
-        av = aprev[n + i];
-        aw = aprev[2 * n + i];
+        aprev[i] = in.au;
+        aprev[n + i] = in.av;
+        aprev[2 * n + i] = in.aw;
     }
-    if (dim == 2)
-        write_force_record<2>(stage, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
-    else
-        write_force_record<3>(stage, F, i, x, y, z, h, m, rho, p, epred, bx, by, bz, vwave, s.chf, au, av, aw);
+    if (dim == 2) write_force_record<2>(stage, in, rho, p, bx, by, bz, vwave, s.chf);
+    else write_force_record<3>(stage, in, rho, p, bx, by, bz, vwave, s.chf);
 }
 
 __global__ void k_prep_force(Fields F, const int32_t* __restrict__ perm, const Inst* __restrict__ inst,

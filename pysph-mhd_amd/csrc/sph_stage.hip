@@ -145,6 +145,10 @@ __device__ __forceinline__ double4 stage2_init_stage1_one(const Fields& F, const
     const double abx = FLD(ABEVOLX), aby = FLD(ABEVOLY), abz = FLD(ABEVOLZ), apsi = FLD(APSI);
     double x = FLD(X), y = FLD(Y), z = has_z ? FLD(Z) : 0.0;
     double h = FLD(H), alpha = FLD(ALPHA1);
+    // every input is loaded before the first store: a load after a store to
+    // another field could alias it for the compiler, which then waits for a
+    // second round trip to memory
+    const double rho = FLD(RHO), ah = FLD(AH), cs = FLD(CS), al = FLD(ALPHALOC);
 
     // stage2 (step n)
     if (fluid) {
@@ -153,7 +157,6 @@ __device__ __forceinline__ double4 stage2_init_stage1_one(const Fields& F, const
         v += dtb2 * av;
         w += dtb2 * aw;
         e = e + dtb2 * ae;
-        const double rho = FLD(RHO);
         FLD(P) = s.gamma1 * rho * e;
         bx = bx + dtb2 * abx;
         by = by + dtb2 * aby;
@@ -197,9 +200,8 @@ __device__ __forceinline__ double4 stage2_init_stage1_one(const Fields& F, const
         if (s.dim == 3) z = z + dtn * w;  // z0 == z here
         e = e0 + dtb2 * ae;
         epred = e0 + dtn * ae;
-        h = h0 + dtn * FLD(AH);
-        const double decay = 0.1 * FLD(CS) / h;
-        const double al = FLD(ALPHALOC);
+        h = h0 + dtn * ah;
+        const double decay = 0.1 * cs / h;
         alpha = a10 < al ? al : (a10 + dtn * al * decay) / (1.0 + dtn * decay);
         bx = bx0 + dtb2 * abx;
         by = by0 + dtb2 * aby;
