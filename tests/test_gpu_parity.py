@@ -494,6 +494,42 @@ def test_density_iteration_stops_at_max_iterations(gpu, monkeypatch, fused):
     compare(dev, orc, STATE + RATES, TOL, f"max_iterations fused={fused}")
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_ensemble_with_a_single_density_pass_instance(gpu, monkeypatch, fused):
+    """An ensemble whose second instance takes one density pass
+    (iterate_only_once: every particle finishes in the first pass) beside an
+    instance that iterates: the group control counts one pass for the one
+    and at least min_iterations for the other, with the later passes fused
+    into the first launch (SPHMHD_DENSITY_FUSED=1) or launched on their own
+    (0).  The warm start and two steps match the oracle.
+
+    Floor: divBsymm, divBdiff, psi, apsi and aBevol* are measured against
+    1e-3 of their natural scale (|B|/h, rho|B|/h, |B|, v_w|B|/h,
+    v_w|B|/(rho h)), since they are round-off where B is uniform; every
+    other field against its own norm."""
+    from pyoracle import OracleSystem
+    from sphmhd.cases.workloads import ensemble_ot, jittered
+    from sphmhd.device import DeviceSystem
+
+    monkeypatch.setenv("SPHMHD_DENSITY_FUSED", fused)
+    s = jittered(ensemble_ot(2, sizes=[32, 24]), 0.1, seed=6)
+    prm = [s.params[0], _params_with(s.params[1], density_iterate=0, min_iterations=0, max_iterations=1)]
+    f = init_fields(s)
+    dev = DeviceSystem(prm, s.offsets)
+    orc = OracleSystem(prm, s.offsets)
+    for sysm in (dev, orc):
+        sysm.set_fields(f)
+        sysm.warm_start()
+    compare(dev, orc, STATE + RATES, TOL, f"mixed iteration warm start fused={fused}")
+    for k in range(2):
+        dev.step(1)
+        orc.step(1)
+        compare(dev, orc, STATE + RATES, TOL, f"mixed iteration step {k + 1} fused={fused}")
+    it = orc.last_iterations()
+    assert int(it[1]) == 1 and int(it[0]) >= 2
+    assert dev.stats()["last_iterations_max"] == int(it.max())
+
+
 def test_fused_density_passes_equal_separate_passes(gpu, monkeypatch):
     """The density iteration with its later passes inside the first launch
     (each workgroup iterating its own unconverged particles, 4 lanes each in
